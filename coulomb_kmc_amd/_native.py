@@ -1,0 +1,128 @@
+"""
+Loader and build recipe of the HIP library (libkmc_amd.so, C ABI in include/kmc_amd.h).
+
+The library is built in-tree by ``build()`` (hipcc for gfx950) and loaded with
+ctypes.  There is no fallback: if the library is missing, ``lib()`` raises.
+"""
+
+import ctypes
+import os
+import subprocess
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SRC = os.path.join(_HERE, "csrc", "kmc_amd.hip")
+LIB_PATH = os.path.join(_HERE, "libkmc_amd.so")
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-fPIC",
+    "-shared",
+    "-std=c++17",
+    "-Wno-unused-value",
+    # packed FP32 instructions are disabled for all device code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+BC = {"free_space": 0, "27": 1, "pbc": 2, "ff-only": 3}
+PREC = {"fp64": 0, "mixed": 1}
+
+KMC_E_BAD_MOVE = 4
+KMC_E_STORE = 5
+KMC_E_STATE = 3
+
+# exported symbols (each declared in include/kmc_amd.h)
+EXPORTS = (
+    "kmc_last_error", "kmc_min_order", "kmc_max_order", "kmc_create", "kmc_destroy",
+    "kmc_set_particles", "kmc_set_lr_operator", "kmc_initialise", "kmc_get_expansions",
+    "kmc_set_expansions", "kmc_get_multipole", "kmc_get_store", "kmc_store_occmax",
+    "kmc_get_particles", "kmc_get_energy", "kmc_set_energy", "kmc_ncells", "kmc_propose",
+    "kmc_propose_with_dats", "kmc_accept", "kmc_step", "kmc_materialize_moves",
+    "kmc_check_flags", "kmc_set_tiles", "kmc_sweep_hop",
+)
+
+
+def hipcc():
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", "hipcc"):
+        if cand and (os.path.isabs(cand) and os.path.exists(cand) or not os.path.isabs(cand)):
+            return cand
+    return "hipcc"
+
+
+def build(verbose=False):
+    """Compile csrc/kmc_amd.hip into libkmc_amd.so (always recompiles)."""
+    tmp = LIB_PATH + ".tmp"
+    cmd = [hipcc()] + HIPCC_FLAGS + ["-o", tmp, _SRC]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        msg = "\n".join(l for l in res.stderr.splitlines() if "not a recognized feature" not in l)
+        raise RuntimeError("hipcc failed:\n" + msg[-4000:])
+    os.replace(tmp, LIB_PATH)
+    if verbose:
+        print("built", LIB_PATH)
+    return LIB_PATH
+
+
+_lib = None
+
+
+class KMCError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            "coulomb_kmc_amd: HIP library {} is not built; run __graft_entry__.build() "
+            "or coulomb_kmc_amd._native.build()".format(LIB_PATH))
+    L = ctypes.CDLL(LIB_PATH)
+    P, I64, I32, D = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_double
+    sig = {
+        "kmc_last_error": (ctypes.c_char_p, []),
+        "kmc_min_order": (I32, []),
+        "kmc_max_order": (I32, []),
+        "kmc_create": (ctypes.c_int, [ctypes.POINTER(P), I64, D, I32, I32, I32, D, D, I32]),
+        "kmc_destroy": (ctypes.c_int, [P]),
+        "kmc_set_particles": (ctypes.c_int, [P, P, P]),
+        "kmc_set_lr_operator": (ctypes.c_int, [P, I64, P, P, P]),
+        "kmc_initialise": (ctypes.c_int, [P, P, P]),
+        "kmc_get_expansions": (ctypes.c_int, [P, P]),
+        "kmc_set_expansions": (ctypes.c_int, [P, P]),
+        "kmc_get_multipole": (ctypes.c_int, [P, P]),
+        "kmc_get_store": (ctypes.c_int, [P, P, P, P, P]),
+        "kmc_store_occmax": (ctypes.c_int, [P, P]),
+        "kmc_get_particles": (ctypes.c_int, [P, P, P]),
+        "kmc_get_energy": (ctypes.c_int, [P, P]),
+        "kmc_set_energy": (ctypes.c_int, [P, D]),
+        "kmc_ncells": (ctypes.c_int, [P, P]),
+        "kmc_propose": (ctypes.c_int, [P, P, I64, I64, P, P, P, P, P, I32]),
+        "kmc_propose_with_dats": (ctypes.c_int, [P, P, I64, P, P, P, P, P, P, I32, P]),
+        "kmc_accept": (ctypes.c_int, [P, P, I64, P, P]),
+        "kmc_step": (ctypes.c_int, [P, P, I64, I64, P, P, P, P, I64, P, D, P, P, I32]),
+        "kmc_materialize_moves": (ctypes.c_int, [P, P, I64, I64, I64, P, P, P, P, P, P]),
+        "kmc_check_flags": (ctypes.c_int, [P, P, P]),
+        "kmc_set_tiles": (ctypes.c_int, [P, I32, I32]),
+        "kmc_sweep_hop": (ctypes.c_int, [P, P, D, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc, what=""):
+    """Turn a C-ABI status into the reference's exception types."""
+    if rc == 0:
+        return
+    msg = lib().kmc_last_error().decode(errors="replace")
+    if rc == KMC_E_BAD_MOVE:
+        raise RuntimeError(msg or "Bad proposed move was detected.")
+    if rc == KMC_E_STATE:
+        raise RuntimeError(msg or "Run initialise before this call")
+    raise KMCError("{} failed ({}): {}".format(what, rc, msg))

@@ -1,0 +1,1766 @@
+// MI355X (gfx950) FMM-KMC propose/accept.
+//
+// Device state of one system (all in HBM, owned by the context):
+//   particles   pos4[n]  (x, y, z, q)          cell[n] (linear leaf index)
+//   cell tables occ[ncells], slots[ncells*occmax] of (x, y, z, q) + id
+//   expansions  lexp[ncells][L(L+1)/2] complex (m >= 0 only: for real charges the
+//               m < 0 coefficients are the complex conjugates, so the reference's
+//               [re L^2 | im L^2] block is reconstructed exactly on demand)
+//   long range  M (packed like lexp), g = G M, E_lr = 0.5 M.g
+//
+// Kernels (the reference function each replaces):
+//   k_near        direct_new / direct_old        coulomb_kmc/kmc_local.py:928-1031
+//   k_far         indirect_interactions          coulomb_kmc/kmc_octal.py:495-707
+//   k_combine     self_interaction + diff_lib    kmc_fmm_self_interaction.py:190-242,
+//                                                kmc_fmm.py:583-663
+//   k_lr_prop     long_range_energy              kmc_full_long_range.py:710-858
+//   k_sweep       accept_local_exp (hop)         coulomb_kmc/kmc_octal.py:1008-1089
+//   k_accept_store extract + inject              coulomb_kmc/kmc_local.py:180-370
+//   k_multipole_delta long range accept          kmc_full_long_range.py:270-316
+//   k_dats_*      setup_move                     coulomb_kmc/kmc_mpi_decomp.py:501-616
+//
+// Solid harmonics are evaluated from Cartesian coordinates with the column
+// recurrence documented in coulomb_kmc_amd/harmonics.py (no trigonometry); the
+// coefficient tables are compile-time constants.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+#include <math.h>
+#include <string>
+#include <vector>
+#include <algorithm>
+
+#include "../../include/kmc_amd.h"
+
+// ============================================================================
+// errors
+// ============================================================================
+static thread_local std::string g_err;
+static int set_err(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+#define HIPCHK(x)                                                                         \
+    do {                                                                                  \
+        hipError_t e_ = (x);                                                              \
+        if (e_ != hipSuccess)                                                             \
+            return set_err(KMC_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+// ============================================================================
+// compile-time recurrence tables
+// ============================================================================
+constexpr int KMIN_L = 2;
+constexpr int KMAX_L = 20;
+
+constexpr double c_sqrt(double x) {
+    if (x <= 0.0) return 0.0;
+    double g = x > 1.0 ? x : 1.0;
+    for (int i = 0; i < 200; i++) {
+        const double n = 0.5 * (g + x / g);
+        if (n == g) break;
+        g = n;
+    }
+    return g;
+}
+struct RecTab {
+    double a[KMAX_L + 1][KMAX_L + 1];   // alpha_lm = (2l-1)/sqrt(l^2-m^2)
+    double b[KMAX_L + 1][KMAX_L + 1];   // beta_lm  = sqrt(((l-1)^2-m^2)/(l^2-m^2))
+    double s[KMAX_L + 1];               // s_m = (-1)^m (2m-1)!!/sqrt((2m)!)
+};
+constexpr RecTab make_rectab() {
+    RecTab t{};
+    t.s[0] = 1.0;
+    for (int m = 1; m <= KMAX_L; m++) t.s[m] = -t.s[m - 1] * c_sqrt((2.0 * m - 1.0) / (2.0 * m));
+    for (int m = 0; m <= KMAX_L; m++)
+        for (int l = m + 1; l <= KMAX_L; l++) {
+            const double d = (double)(l * l - m * m);
+            t.a[l][m] = (2.0 * l - 1.0) / c_sqrt(d);
+            const double e = (double)((l - 1) * (l - 1) - m * m);
+            t.b[l][m] = e > 0.0 ? c_sqrt(e / d) : 0.0;
+        }
+    return t;
+}
+static constexpr RecTab kRec = make_rectab();
+
+__host__ __device__ constexpr int hidx(int l, int m) { return l * (l + 1) / 2 + m; }
+__host__ __device__ constexpr int nhalf(int L) { return L * (L + 1) / 2; }
+
+// ============================================================================
+// small device helpers
+// ============================================================================
+struct Cplx {
+    double re, im;
+};
+__device__ __forceinline__ Cplx cmul(Cplx a, Cplx b) {
+    return {fma(a.re, b.re, -a.im * b.im), fma(a.re, b.im, a.im * b.re)};
+}
+template <int M>
+__device__ __forceinline__ Cplx cpow(Cplx u) {
+    if constexpr (M == 0) {
+        return {1.0, 0.0};
+    } else if constexpr (M == 1) {
+        return u;
+    } else if constexpr (M % 2 == 0) {
+        const Cplx h = cpow<M / 2>(u);
+        return cmul(h, h);
+    } else {
+        return cmul(cpow<M - 1>(u), u);
+    }
+}
+
+// 1/sqrt(x) to full FP64 accuracy: FP32 hardware estimate, two Newton steps in FP64.
+__device__ __forceinline__ double rsqrt_fp64(double x) {
+    double y = (double)__builtin_amdgcn_rsqf((float)x);
+    double h = fma(-0.5 * x * y, y, 0.5);
+    y = fma(y, h, y);
+    h = fma(-0.5 * x * y, y, 0.5);
+    return fma(y, h, y);
+}
+// 1/sqrt(x) with one Newton step (relative error ~3e-14): the mixed-precision path.
+__device__ __forceinline__ double rsqrt_mixed(double x) {
+    const double y = (double)__builtin_amdgcn_rsqf((float)x);
+    const double h = fma(-0.5 * x * y, y, 0.5);
+    return fma(y, h, y);
+}
+// plain FP64 (library sqrt and division, as the reference's 1.0/sqrt(r2))
+__device__ __forceinline__ double rsqrt_plain(double x) { return 1.0 / sqrt(x); }
+
+template <bool MIXED>
+__device__ __forceinline__ double rsqrt_sel(double x) {
+    if constexpr (MIXED) return rsqrt_mixed(x);
+    else return rsqrt_plain(x);
+}
+
+struct Geom {
+    double E;        // box side
+    double w;        // leaf width
+    double c0;       // centre of leaf 0 along an axis: -E/2 + w/2
+    int ncps;        // leaves per side
+    int bc;
+};
+
+// leaf of a coordinate: truncate (p + E/2)/w and clamp (coulomb_kmc/kmc_fmm.py:229-242)
+__host__ __device__ inline int bin_axis(double p, double E, double w, int ncps) {
+    long long c = (long long)((0.5 * E + p) / w);
+    if (c < 0) c = 0;
+    if (c > ncps - 1) c = ncps - 1;
+    return (int)c;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// stencil offsets, x fastest (coulomb_kmc/common.py:51-81)
+__device__ __forceinline__ void stencil(int k, int& ox, int& oy, int& oz) {
+    ox = k % 3 - 1;
+    oy = (k / 3) % 3 - 1;
+    oz = k / 9 - 1;
+}
+
+// ============================================================================
+// accept sweep / initial P2L: every leaf receives q I_l^m conj(...) from sources
+// ============================================================================
+// Sources come either from the accepted move (old position with -q, new position
+// with +q) or from a range of particles (+q, initialisation).  For '27'/PBC each
+// source is applied with its 27 images (image shift o*E, cell shift o*ncps) and the
+// reference's well-separated test (coulomb_kmc/kmc_octal.py:966-975).
+struct MoveRec {
+    double old_pos[3];
+    double new_pos[3];
+    double q;
+    double dE;
+    int32_t pid;
+    int32_t old_cell;
+    int32_t new_cell;
+    int32_t valid;
+};
+
+struct SweepArgs {
+    Geom g;
+    double2* lexp;             // [ncells][nh]
+    int ncells;
+    int mode;                  // 0: move record, 1: particle range
+    const MoveRec* rec;
+    const double4* src_pos4;   // mode 1
+    const int32_t* src_cell;   // mode 1
+    int src_begin, src_end;
+};
+
+template <int L, int M1, int M2>
+__device__ __forceinline__ void sweep_columns(const SweepArgs& a, int cell) {
+    constexpr int N1 = L - M1;
+    constexpr int N2 = (M2 != M1) ? (L - M2) : 0;
+    constexpr int NA = N2 > 0 ? N2 : 1;
+    double a1r[N1], a1i[N1], a2r[NA], a2i[NA];
+#pragma unroll
+    for (int k = 0; k < N1; k++) { a1r[k] = 0.0; a1i[k] = 0.0; }
+#pragma unroll
+    for (int k = 0; k < NA; k++) { a2r[k] = 0.0; a2i[k] = 0.0; }
+
+    const int n = a.g.ncps;
+    const int tx = cell % n, ty = (cell / n) % n, tz = cell / (n * n);
+    const double cx = a.g.c0 + tx * a.g.w;
+    const double cy = a.g.c0 + ty * a.g.w;
+    const double cz = a.g.c0 + tz * a.g.w;
+    const bool periodic = a.g.bc != KMC_BC_FREE_SPACE;
+    const int nimg = periodic ? 27 : 1;
+
+    int nsrc;
+    if (a.mode == 0) nsrc = a.rec->valid ? 2 : 0;
+    else nsrc = a.src_end - a.src_begin;
+
+    for (int s = 0; s < nsrc; s++) {
+        double sx, sy, sz, sq;
+        int sc;
+        if (a.mode == 0) {
+            const double* p = s == 0 ? a.rec->old_pos : a.rec->new_pos;
+            sx = p[0]; sy = p[1]; sz = p[2];
+            sq = s == 0 ? -1.0 * a.rec->q : a.rec->q;
+            sc = s == 0 ? a.rec->old_cell : a.rec->new_cell;
+        } else {
+            const double4 p = a.src_pos4[a.src_begin + s];
+            sx = p.x; sy = p.y; sz = p.z; sq = p.w;
+            sc = a.src_cell[a.src_begin + s];
+        }
+        const int scx = sc % n, scy = (sc / n) % n, scz = sc / (n * n);
+        for (int k = 0; k < nimg; k++) {
+            int ox = 0, oy = 0, oz = 0;
+            if (periodic) stencil(k, ox, oy, oz);
+            const int ddx = scx + ox * n - tx, ddy = scy + oy * n - ty, ddz = scz + oz * n - tz;
+            const bool ws = !(ddx < 2 && ddx > -2 && ddy < 2 && ddy > -2 && ddz < 2 && ddz > -2);
+            const double dx = (sx + ox * a.g.E) - cx;
+            const double dy = (sy + oy * a.g.E) - cy;
+            const double dz = (sz + oz * a.g.E) - cz;
+            double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+            r2 = ws ? r2 : 1.0;
+            const double qe = ws ? sq : 0.0;
+            const double ir = rsqrt_fp64(r2);
+            const double ir2 = ir * ir;
+            const double zr = dz * ir2;
+            const Cplx u = {dx * ir2, -dy * ir2};     // conj((x+iy)/r^2)
+            const double base = qe * ir;
+            const Cplx U1 = cpow<M1>(u);
+            {
+                const double f = base * kRec.s[M1];
+                const double wr = f * U1.re, wi = f * U1.im;
+                double c2 = 0.0, c1 = 1.0;
+                a1r[0] += wr;
+                a1i[0] += wi;
+#pragma unroll
+                for (int k2 = 1; k2 < N1; k2++) {
+                    const int l = M1 + k2;
+                    const double c = fma(kRec.a[l][M1] * zr, c1, -(kRec.b[l][M1] * ir2) * c2);
+                    c2 = c1;
+                    c1 = c;
+                    a1r[k2] = fma(c, wr, a1r[k2]);
+                    a1i[k2] = fma(c, wi, a1i[k2]);
+                }
+            }
+            if constexpr (N2 > 0) {
+                const Cplx U2 = cmul(U1, cpow<M2 - M1>(u));
+                const double f = base * kRec.s[M2];
+                const double wr = f * U2.re, wi = f * U2.im;
+                double c2 = 0.0, c1 = 1.0;
+                a2r[0] += wr;
+                a2i[0] += wi;
+#pragma unroll
+                for (int k2 = 1; k2 < N2; k2++) {
+                    const int l = M2 + k2;
+                    const double c = fma(kRec.a[l][M2] * zr, c1, -(kRec.b[l][M2] * ir2) * c2);
+                    c2 = c1;
+                    c1 = c;
+                    a2r[k2] = fma(c, wr, a2r[k2]);
+                    a2i[k2] = fma(c, wi, a2i[k2]);
+                }
+            }
+        }
+    }
+    // one FP64 add per coefficient into the stored expansion (kmc_octal.py:1084)
+    double2* le = a.lexp + (size_t)cell * nhalf(L);
+#pragma unroll
+    for (int k = 0; k < N1; k++) {
+        double2 v = le[hidx(M1 + k, M1)];
+        v.x += a1r[k];
+        v.y += a1i[k];
+        le[hidx(M1 + k, M1)] = v;
+    }
+    if constexpr (N2 > 0) {
+#pragma unroll
+        for (int k = 0; k < N2; k++) {
+            double2 v = le[hidx(M2 + k, M2)];
+            v.x += a2r[k];
+            v.y += a2i[k];
+            le[hidx(M2 + k, M2)] = v;
+        }
+    }
+}
+
+template <int L, int P>
+__device__ __forceinline__ void sweep_pair(const SweepArgs& a, int cell) {
+    if constexpr (P < (L + 1) / 2) {
+        sweep_columns<L, P, L - 1 - P>(a, cell);
+    }
+}
+
+template <int L, int P0>
+__device__ __forceinline__ void sweep_dispatch(const SweepArgs& a, int cell, int pair) {
+    if constexpr (P0 < (L + 1) / 2) {
+        if (pair == P0) sweep_pair<L, P0>(a, cell);
+        else sweep_dispatch<L, P0 + 1>(a, cell, pair);
+    }
+}
+
+// grid: x = leaf blocks (lane = leaf), y = column pair (m, L-1-m)
+template <int L>
+__global__ void __launch_bounds__(1024) k_sweep(SweepArgs a) {
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= a.ncells) return;
+    sweep_dispatch<L, 0>(a, cell, (int)blockIdx.y);
+}
+
+// ============================================================================
+// far field: phi = Re sum L_lm R_l^m(x - centre), one lane per evaluation point
+// ============================================================================
+struct EvalArgs {
+    Geom g;
+    int n;                       // evaluation points
+    const double4* pos4;         // (x, y, z, q)
+    const int32_t* cell;
+    const int32_t* excl;         // id excluded from the near field (-1: none)
+    double* out;
+    const double2* lexp;
+    const int* occ;
+    const double4* slot_pos;
+    const int32_t* slot_id;
+    int occmax;
+};
+
+template <int L>
+__global__ void __launch_bounds__(256) k_far(EvalArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const double4 p = a.pos4[i];
+    const int c = a.cell[i];
+    const int n = a.g.ncps;
+    const int tx = c % n, ty = (c / n) % n, tz = c / (n * n);
+    const double dx = p.x - (a.g.c0 + tx * a.g.w);
+    const double dy = p.y - (a.g.c0 + ty * a.g.w);
+    const double dz = p.z - (a.g.c0 + tz * a.g.w);
+    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    const double2* le = a.lexp + (size_t)c * nhalf(L);
+    const Cplx w = {dx, dy};
+    Cplx C = {1.0, 0.0};
+    double phi = 0.0;
+#pragma unroll
+    for (int m = 0; m < L; m++) {
+        double b2 = 0.0, b1 = kRec.s[m];
+        double2 v = le[hidx(m, m)];
+        double sr = b1 * v.x, si = b1 * v.y;
+#pragma unroll
+        for (int l = m + 1; l < L; l++) {
+            const double b = fma(kRec.a[l][m] * dz, b1, -(kRec.b[l][m] * r2) * b2);
+            b2 = b1;
+            b1 = b;
+            v = le[hidx(l, m)];
+            sr = fma(b, v.x, sr);
+            si = fma(b, v.y, si);
+        }
+        // Re(C_m * S_m), weight 2 for m > 0 (the m < 0 terms are the conjugates)
+        const double re = fma(C.re, sr, -C.im * si);
+        phi = (m == 0) ? re : fma(2.0, re, phi);
+        C = cmul(C, w);
+    }
+    a.out[i] = p.w * phi;
+}
+
+// ============================================================================
+// near field: q_t * sum over the 27 neighbour leaves of q_j / |x_t - x_j|
+// G lanes per evaluation point, neighbour slots strided over the G lanes,
+// shuffle reduction inside the lane group.
+// ============================================================================
+template <int G, bool MIXED>
+__global__ void __launch_bounds__(256) k_near(EvalArgs a) {
+    const int gid = (blockIdx.x * blockDim.x + threadIdx.x);
+    const int i = gid / G;
+    const int lane = gid % G;
+    const bool active = i < a.n;
+    double acc = 0.0;
+    if (active) {
+        const double4 p = a.pos4[i];
+        const int c = a.cell[i];
+        const int ex = a.excl[i];
+        const int n = a.g.ncps;
+        const int tx = c % n, ty = (c / n) % n, tz = c / (n * n);
+        const bool periodic = a.g.bc != KMC_BC_FREE_SPACE;
+        for (int k = 0; k < 27; k++) {
+            int ox, oy, oz;
+            stencil(k, ox, oy, oz);
+            int nx = tx + ox, ny = ty + oy, nz = tz + oz;
+            double shx = 0.0, shy = 0.0, shz = 0.0;
+            if (periodic) {
+                if (nx < 0) { nx += n; shx = -a.g.E; } else if (nx >= n) { nx -= n; shx = a.g.E; }
+                if (ny < 0) { ny += n; shy = -a.g.E; } else if (ny >= n) { ny -= n; shy = a.g.E; }
+                if (nz < 0) { nz += n; shz = -a.g.E; } else if (nz >= n) { nz -= n; shz = a.g.E; }
+            } else if (nx < 0 || nx >= n || ny < 0 || ny >= n || nz < 0 || nz >= n) {
+                continue;
+            }
+            const int nc = nx + n * (ny + n * nz);
+            const int occ = min(a.occ[nc], a.occmax);
+            const double px = p.x - shx, py = p.y - shy, pz = p.z - shz;
+            const size_t base = (size_t)nc * a.occmax;
+            for (int s = lane; s < occ; s += G) {
+                const double4 q = a.slot_pos[base + s];
+                const double dx = px - q.x, dy = py - q.y, dz = pz - q.z;
+                const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                const double contrib = q.w * rsqrt_sel<MIXED>(r2);
+                if (ex >= 0) acc += (a.slot_id[base + s] != ex) ? contrib : 0.0;
+                else acc += contrib;
+            }
+        }
+    }
+    if constexpr (G > 1) {
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, G);
+    }
+    if (active && lane == 0) a.out[i] = acc * a.pos4[i].w;
+}
+
+// ============================================================================
+// proposal setup (tuple form) and validation
+// ============================================================================
+struct SetupArgs {
+    Geom g;
+    double max_move;
+    int P, T, n;
+    const int64_t* pid;
+    const int64_t* es;
+    const double* new_pos;
+    const double* new_q;
+    const double4* ppos4;
+    const int32_t* pcell;
+    double4* epos4;
+    int32_t* ecell;
+    int32_t* eexcl;
+    int32_t* owner;
+    int* err;
+};
+
+// owner p of proposal t: es[p] <= t < es[p+1]; always returns a value in [0, P)
+__device__ __forceinline__ int owner_of(int t, const int64_t* es, int P) {
+    int lo = 0, hi = P - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (es[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// One thread per proposal t (< T) and per moving charge p (< P).
+// kmc_mpi_decomp.py:312-345 (domain and max_move checks, minimum image for '27'/PBC)
+__global__ void k_setup(SetupArgs a) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int bad = 0;
+    if (x < a.P) {
+        int i = (int)a.pid[x];
+        if (i < 0 || i >= a.n) { bad++; i = 0; }
+        a.epos4[a.T + x] = a.ppos4[i];
+        a.ecell[a.T + x] = a.pcell[i];
+        a.eexcl[a.T + x] = i;
+        if (a.es[x] > a.es[x + 1]) bad++;
+        if (x == 0 && a.es[0] != 0) bad++;
+        if (x == a.P - 1 && a.es[a.P] != a.T) bad++;
+    }
+    if (x < a.T) {
+        const int p = owner_of(x, a.es, a.P);
+        a.owner[x] = p;
+        int i = (int)a.pid[p];
+        if (i < 0 || i >= a.n) i = 0;
+        const double4 o = a.ppos4[i];
+        const int n = a.g.ncps;
+        const double px = a.new_pos[3 * x], py = a.new_pos[3 * x + 1], pz = a.new_pos[3 * x + 2];
+        const double nq = a.new_q ? a.new_q[x] : o.w;
+        a.epos4[x] = make_double4(px, py, pz, nq);
+        const int cx = bin_axis(px, a.g.E, a.g.w, n), cy = bin_axis(py, a.g.E, a.g.w, n), cz = bin_axis(pz, a.g.E, a.g.w, n);
+        a.ecell[x] = cx + n * (cy + n * cz);
+        a.eexcl[x] = -1;
+        const double np3[3] = {px, py, pz};
+        const double op3[3] = {o.x, o.y, o.z};
+        for (int d = 0; d < 3; d++) {
+            if (!(np3[d] >= -0.5 * a.g.E && np3[d] <= 0.5 * a.g.E)) bad++;
+            double dd = np3[d] - op3[d];
+            if (a.g.bc != KMC_BC_FREE_SPACE) {
+                if (dd < -0.5 * a.g.E) dd += a.g.E;
+                if (dd > 0.5 * a.g.E) dd -= a.g.E;
+            }
+            if (fabs(dd) > a.max_move) bad++;
+        }
+    }
+    if (bad) atomicAdd(a.err, bad);
+}
+
+// ============================================================================
+// self term + long-range difference + combine (diff_lib)
+// ============================================================================
+struct CombineArgs {
+    Geom g;
+    int T;
+    double eu;
+    double self27;               // -sum_{o != 0} 1/|o E|
+    const double4* epos4;
+    const int32_t* owner;
+    const double* UD;
+    const double* UI;
+    const double* lrd;           // old - new long-range energy, or null
+    const int64_t* rate_loc;     // output index, or null (identity)
+    double* out;
+};
+
+template <bool MIXED>
+__global__ void k_combine(CombineArgs a) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.T) return;
+    const int po = a.T + a.owner[t];
+    const double4 xo = a.epos4[po];
+    const double4 xn = a.epos4[t];
+    double e27;
+    if (a.g.bc == KMC_BC_FREE_SPACE) {
+        const double dx = xo.x - xn.x, dy = xo.y - xn.y, dz = xo.z - xn.z;
+        e27 = rsqrt_sel<MIXED>(fma(dx, dx, fma(dy, dy, dz * dz)));
+    } else if (a.g.bc == KMC_BC_FF_ONLY) {
+        e27 = 0.0;
+    } else {
+        e27 = a.self27;
+        for (int k = 0; k < 27; k++) {
+            int ox, oy, oz;
+            stencil(k, ox, oy, oz);
+            const double dx = ox * a.g.E + xo.x - xn.x;
+            const double dy = oy * a.g.E + xo.y - xn.y;
+            const double dz = oz * a.g.E + xo.z - xn.z;
+            e27 += rsqrt_sel<MIXED>(fma(dx, dx, fma(dy, dy, dz * dz)));
+        }
+    }
+    double usi = xo.w * xo.w * e27;
+    if (a.lrd) usi += a.lrd[t];
+    double de;
+    if (a.g.bc == KMC_BC_FF_ONLY) de = a.eu * (-1.0 * usi);
+    else de = a.eu * (((a.UD[t] + a.UI[t]) - a.UD[po] - a.UI[po]) - usi);
+    const int64_t o = a.rate_loc ? a.rate_loc[t] : (int64_t)t;
+    a.out[o] = de;
+}
+
+// ============================================================================
+// long range
+// ============================================================================
+// packed multipole of charge q at x about the origin: M_lm = q conj(R_l^m(x)), m >= 0.
+// Lane m (< L) writes column m into dst (stride 2 doubles per (l,m)).
+template <int L>
+__device__ __forceinline__ void multipole_column(int m, double x, double y, double z, double q, double* dst,
+                                                 double sign, bool accumulate) {
+    const double r2 = fma(x, x, fma(y, y, z * z));
+    Cplx C = {1.0, 0.0};
+    const Cplx w = {x, -y};
+    for (int k = 0; k < m; k++) C = cmul(C, w);
+    double b2 = 0.0, b1 = 0.0;
+#pragma unroll
+    for (int l = 0; l < L; l++) {
+        if (l < m) continue;
+        double b;
+        if (l == m) b = kRec.s[m];
+        else b = fma(kRec.a[l][m] * z, b1, -(kRec.b[l][m] * r2) * b2);
+        b2 = b1;
+        b1 = b;
+        const double vr = sign * q * b * C.re, vi = sign * q * b * C.im;
+        const int h = hidx(l, m);
+        if (accumulate) {
+            dst[2 * h] += vr;
+            dst[2 * h + 1] += vi;
+        } else {
+            dst[2 * h] = vr;
+            dst[2 * h + 1] = vi;
+        }
+    }
+}
+
+struct LRArgs {
+    int T;
+    int nrow;                    // 2 * nh
+    const double* gdat;
+    const int64_t* gptr;
+    const int32_t* gidx;
+    const double* gvec;          // G M
+    const double4* epos4;
+    const int32_t* owner;
+    double* lrd;                 // old - new
+};
+
+// one wave per proposal: delta = m(new) - m(old); d = g.delta + 0.5 delta.G.delta
+template <int L>
+__global__ void __launch_bounds__(64) k_lr_prop(LRArgs a) {
+    constexpr int NR = 2 * nhalf(L);
+    __shared__ double dl[NR];
+    const int t = blockIdx.x;
+    const int lane = threadIdx.x;
+    const double4 xn = a.epos4[t];
+    const double4 xo = a.epos4[a.T + a.owner[t]];
+    if (lane < L) {
+        multipole_column<L>(lane, xn.x, xn.y, xn.z, xn.w, dl, 1.0, false);
+        multipole_column<L>(lane, xo.x, xo.y, xo.z, xo.w, dl, -1.0, true);
+    }
+    __syncthreads();
+    double acc = 0.0;
+    for (int row = lane; row < NR; row += 64) {
+        double y = 0.0;
+        for (int64_t k = a.gptr[row]; k < a.gptr[row + 1]; k++) y = fma(a.gdat[k], dl[a.gidx[k]], y);
+        acc = fma(dl[row], fma(0.5, y, a.gvec[row]), acc);
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) a.lrd[t] = -acc;
+}
+
+// g = G M and E_lr = 0.5 M.g  (one block)
+__global__ void __launch_bounds__(256) k_lr_refresh(int nrow, const double* gdat, const int64_t* gptr,
+                                                     const int32_t* gidx, const double* M, double* g,
+                                                     double* elr) {
+    __shared__ double part[256];
+    double acc = 0.0;
+    for (int row = threadIdx.x; row < nrow; row += blockDim.x) {
+        double y = 0.0;
+        for (int64_t k = gptr[row]; k < gptr[row + 1]; k++) y = fma(gdat[k], M[gidx[k]], y);
+        g[row] = y;
+        acc = fma(M[row], y, acc);
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *elr = 0.5 * part[0];
+}
+
+// level-0 multipole of a particle range: per block partial sums (deterministic),
+// lane m of each wave owns column m.
+template <int L>
+__global__ void __launch_bounds__(64) k_multipole_partial(int n, int chunk, const double4* pos4, double* partial) {
+    constexpr int NR = 2 * nhalf(L);
+    const int b = blockIdx.x;
+    const int m = threadIdx.x;
+    double* dst = partial + (size_t)b * NR;
+    if (m >= L) return;
+    for (int l = m; l < L; l++) {
+        dst[2 * hidx(l, m)] = 0.0;
+        dst[2 * hidx(l, m) + 1] = 0.0;
+    }
+    const int i0 = b * chunk, i1 = min(n, i0 + chunk);
+    for (int i = i0; i < i1; i++) {
+        const double4 p = pos4[i];
+        multipole_column<L>(m, p.x, p.y, p.z, p.w, dst, 1.0, true);
+    }
+}
+__global__ void k_reduce_rows(int nblk, int nrow, const double* partial, double* out) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrow) return;
+    double s = 0.0;
+    for (int b = 0; b < nblk; b++) s += partial[(size_t)b * nrow + r];
+    out[r] = s;
+}
+
+// accept: M += m(old, -q); M += m(new, +q) (kmc_full_long_range.py:270-316)
+template <int L>
+__global__ void __launch_bounds__(64) k_multipole_delta(const MoveRec* rec, double* M) {
+    if (!rec->valid) return;
+    const int m = threadIdx.x;
+    if (m >= L) return;
+    multipole_column<L>(m, rec->old_pos[0], rec->old_pos[1], rec->old_pos[2], rec->q, M, -1.0, true);
+    multipole_column<L>(m, rec->new_pos[0], rec->new_pos[1], rec->new_pos[2], rec->q, M, 1.0, true);
+}
+
+// ============================================================================
+// accept: move record, cell tables
+// ============================================================================
+__global__ void k_prepare_record(Geom g, int pid, double nx, double ny, double nz, const double4* ppos4,
+                                 const int32_t* pcell, const double* dE, double* energy, MoveRec* rec) {
+    const double4 o = ppos4[pid];
+    rec->old_pos[0] = o.x; rec->old_pos[1] = o.y; rec->old_pos[2] = o.z;
+    rec->new_pos[0] = nx; rec->new_pos[1] = ny; rec->new_pos[2] = nz;
+    rec->q = o.w;
+    rec->pid = pid;
+    rec->old_cell = pcell[pid];
+    const int n = g.ncps;
+    rec->new_cell = bin_axis(nx, g.E, g.w, n) + n * (bin_axis(ny, g.E, g.w, n) + n * bin_axis(nz, g.E, g.w, n));
+    rec->dE = dE[0];
+    rec->valid = 1;
+    *energy += dE[0];
+}
+
+// kmc_local.py:180-271 (extract: swap the last occupant into the freed slot) then
+// kmc_local.py:274-359 (inject: append), then the particle arrays.
+__global__ void k_accept_store(const MoveRec* rec, int* occ, double4* slot_pos, int32_t* slot_id, int occmax,
+                               double4* ppos4, int32_t* pcell, int* flags) {
+    if (!rec->valid) return;
+    const int pid = rec->pid;
+    const int oc = rec->old_cell, nc = rec->new_cell;
+    const size_t ob = (size_t)oc * occmax, nb = (size_t)nc * occmax;
+    const int oo = occ[oc];
+    int index = -1, found = 0;
+    for (int s = 0; s < oo; s++)
+        if (slot_id[ob + s] == pid) { index = s; found++; }
+    if (found != 1) { atomicOr(flags, 1); return; }
+    slot_id[ob + index] = slot_id[ob + oo - 1];
+    slot_pos[ob + index] = slot_pos[ob + oo - 1];
+    occ[oc] = oo - 1;
+    const int no = occ[nc];
+    if (no >= occmax) { atomicOr(flags, 2); return; }
+    slot_id[nb + no] = pid;
+    slot_pos[nb + no] = make_double4(rec->new_pos[0], rec->new_pos[1], rec->new_pos[2], rec->q);
+    occ[nc] = no + 1;
+    ppos4[pid] = make_double4(rec->new_pos[0], rec->new_pos[1], rec->new_pos[2], rec->q);
+    pcell[pid] = nc;
+}
+
+// ============================================================================
+// Metropolis selection on the device (one block)
+// ============================================================================
+struct SelectArgs {
+    Geom g;
+    int T;
+    int n;
+    int64_t nsteps;
+    double kT;
+    const double* dE;
+    const double* uniform;
+    const int64_t* counter;
+    int64_t* selected;
+    const double4* epos4;
+    const int32_t* ecell;
+    const int32_t* owner;
+    const int64_t* pid;
+    const double4* ppos4;
+    const int32_t* pcell;
+    const int* occ;
+    int occmax;
+    double* energy;
+    MoveRec* rec;
+    int* flags;
+};
+
+__device__ __forceinline__ double rate_of(double de, double kT) {
+    if (!(de == de) || isinf(de)) return 0.0;
+    const double w = exp(-de / kT);
+    return w < 1.0 ? w : 1.0;
+}
+
+__global__ void __launch_bounds__(1024) k_select(SelectArgs a) {
+    __shared__ double sums[1024];
+    __shared__ int pick;
+    const int tid = threadIdx.x;
+    const int nt = blockDim.x;
+    const int chunk = (a.T + nt - 1) / nt;
+    const int b0 = tid * chunk, b1 = min(a.T, b0 + chunk);
+    double s = 0.0;
+    for (int t = b0; t < b1; t++) s += rate_of(a.dE[t], a.kT);
+    sums[tid] = s;
+    if (tid == 0) pick = -1;
+    __syncthreads();
+    // inclusive scan (Hillis-Steele)
+    for (int o = 1; o < nt; o <<= 1) {
+        const double v = tid >= o ? sums[tid - o] : 0.0;
+        __syncthreads();
+        sums[tid] += v;
+        __syncthreads();
+    }
+    const int64_t step = *a.counter % a.nsteps;
+    const double total = sums[nt - 1];
+    const double target = a.uniform[step] * total;
+    const double lo = tid > 0 ? sums[tid - 1] : 0.0;
+    if (target >= lo && target < sums[tid] && b1 > b0) {
+        double c = lo;
+        int k = b1 - 1;
+        for (int t = b0; t < b1; t++) {
+            c += rate_of(a.dE[t], a.kT);
+            if (target < c) { k = t; break; }
+        }
+        pick = k;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int k = pick;
+        if (k < 0) {   // target == total (u == 1) or no positive rate: last positive rate
+            for (int t = a.T - 1; t >= 0; t--)
+                if (rate_of(a.dE[t], a.kT) > 0.0) { k = t; break; }
+        }
+        MoveRec* r = a.rec;
+        a.selected[step] = k;
+        if (k < 0) { r->valid = 0; atomicOr(a.flags, 4); return; }
+        const int p = a.owner[k];
+        const int pid = (int)a.pid[p];
+        if (pid < 0 || pid >= a.n) { r->valid = 0; atomicOr(a.flags, 8); return; }
+        const double4 np = a.epos4[k];
+        const double4 op = a.ppos4[pid];
+        r->old_pos[0] = op.x; r->old_pos[1] = op.y; r->old_pos[2] = op.z;
+        r->new_pos[0] = np.x; r->new_pos[1] = np.y; r->new_pos[2] = np.z;
+        r->q = op.w;
+        r->pid = pid;
+        r->old_cell = a.pcell[pid];
+        r->new_cell = a.ecell[k];
+        r->dE = a.dE[k];
+        const bool full = (r->new_cell != r->old_cell) && (a.occ[r->new_cell] >= a.occmax);
+        if (full) { r->valid = 0; atomicOr(a.flags, 2); return; }
+        r->valid = 1;
+        *a.energy += a.dE[k];
+    }
+}
+
+__global__ void k_counter_inc(int64_t* c) { *c += 1; }
+
+// ============================================================================
+// propose_with_dats set-up (mask compaction, kmc_mpi_decomp.py:548-614)
+// ============================================================================
+__global__ void k_dats_count(int n, int64_t max_prop, const int64_t* smc, const int64_t* sites, const int64_t* masks,
+                             int64_t* cnt, int64_t* has) {
+    const int px = blockIdx.x * blockDim.x + threadIdx.x;
+    if (px >= n) return;
+    const int64_t mx = smc[sites[px]];
+    int64_t found = 0;
+    for (int64_t m = 0; m < max_prop && found < mx; m++)
+        if (masks[(int64_t)px * max_prop + m] > 0) found++;
+    cnt[px] = found;
+    has[px] = found > 0 ? 1 : 0;
+}
+
+// exclusive scan of n int64 values (single block, sequential chunks per thread)
+__global__ void __launch_bounds__(1024) k_scan_excl(int n, const int64_t* in, int64_t* out, int64_t* total) {
+    __shared__ int64_t sums[1024];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int chunk = (n + nt - 1) / nt;
+    const int b0 = tid * chunk, b1 = min(n, b0 + chunk);
+    int64_t s = 0;
+    for (int i = b0; i < b1; i++) s += in[i];
+    sums[tid] = s;
+    __syncthreads();
+    for (int o = 1; o < nt; o <<= 1) {
+        const int64_t v = tid >= o ? sums[tid - o] : 0;
+        __syncthreads();
+        sums[tid] += v;
+        __syncthreads();
+    }
+    int64_t c = tid > 0 ? sums[tid - 1] : 0;
+    for (int i = b0; i < b1; i++) {
+        out[i] = c;
+        c += in[i];
+    }
+    if (tid == nt - 1) *total = sums[nt - 1];
+}
+
+__global__ void k_dats_scatter(int n, int64_t max_prop, const int64_t* smc, const int64_t* sites, const int64_t* masks,
+                               const double* ppos, const double* pq, const int64_t* cnt, const int64_t* slot,
+                               const int64_t* off, int64_t* pid, int64_t* es, double* new_pos, double* new_q,
+                               int64_t* rate_loc, const int64_t* total_moves, const int64_t* total_parts,
+                               const double4* ppos4) {
+    const int px = blockIdx.x * blockDim.x + threadIdx.x;
+    if (px == 0) es[*total_parts] = *total_moves;
+    if (px >= n || cnt[px] == 0) return;
+    const int64_t p = slot[px];
+    pid[p] = px;
+    es[p] = off[px];
+    const int64_t mx = smc[sites[px]];
+    int64_t found = 0;
+    for (int64_t m = 0; m < max_prop && found < mx; m++)
+        if (masks[(int64_t)px * max_prop + m] > 0) {
+            const int64_t t = off[px] + found;
+            for (int d = 0; d < 3; d++) new_pos[3 * t + d] = ppos[(int64_t)px * max_prop * 3 + m * 3 + d];
+            new_q[t] = pq ? pq[(int64_t)px * max_prop + m] : ppos4[px].w;
+            rate_loc[t] = (int64_t)px * max_prop + m;
+            found++;
+        }
+}
+
+// synthetic move stream (benchmarks/tests)
+__global__ void k_materialize(Geom g, int n, int64_t nsteps, int P, int k, const int64_t* pids_all,
+                              const double* disp_all, const int64_t* counter, const double4* ppos4, int64_t* pid,
+                              int64_t* es, double* new_pos, int* flags) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int T = P * k;
+    const int64_t step = *counter % nsteps;
+    if (t <= P && t >= 0 && t < P + 1) es[t] = (int64_t)t * k;
+    if (t >= T) return;
+    const int p = t / k;
+    int64_t i = pids_all[step * P + p];
+    if (i < 0 || i >= n) { atomicOr(flags, 16); i = 0; }
+    if (t % k == 0) pid[p] = i;
+    const double4 o = ppos4[i];
+    const double* d = disp_all + (step * T + t) * 3;
+    double v[3] = {o.x + d[0], o.y + d[1], o.z + d[2]};
+    const double h = 0.5 * g.E;
+    for (int c = 0; c < 3; c++) {
+        if (g.bc == KMC_BC_FREE_SPACE) {
+            v[c] = v[c] < -h ? -h : (v[c] > h ? h : v[c]);
+        } else {
+            if (v[c] < -h) v[c] += g.E;
+            if (v[c] >= h) v[c] -= g.E;
+        }
+        new_pos[3 * t + c] = v[c];
+    }
+}
+
+__global__ void k_half_sum(int n, const double* a, const double* b, double* out) {
+    // out = 0.5 * sum (a + b)  (one block, deterministic)
+    __shared__ double part[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += a[i] + b[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = 0.5 * part[0];
+}
+
+__global__ void k_finish_energy(const double* e_sr, const double* elr, double eu, int bc, double* energy) {
+    double e = 0.0;
+    if (bc != KMC_BC_FF_ONLY) e += *e_sr;
+    if (bc == KMC_BC_PBC || bc == KMC_BC_FF_ONLY) e += *elr;
+    *energy = e * eu;
+}
+
+// ============================================================================
+// order dispatch tables
+// ============================================================================
+typedef void (*SweepLaunch)(SweepArgs, int, hipStream_t, int);
+typedef void (*EvalLaunch)(EvalArgs, hipStream_t);
+typedef void (*LRLaunch)(LRArgs, hipStream_t);
+typedef void (*MpLaunch)(int, int, const double4*, double*, int, hipStream_t);
+typedef void (*MdLaunch)(const MoveRec*, double*, hipStream_t);
+
+template <int L>
+static void launch_sweep(SweepArgs a, int ncells, hipStream_t s, int block) {
+    const int npairs = (L + 1) / 2;
+    int bx = (ncells + block - 1) / block;
+    if (bx % 8) bx += 8 - bx % 8;   // all column pairs of a leaf block on one XCD
+    hipLaunchKernelGGL(k_sweep<L>, dim3(bx, npairs), dim3(block), 0, s, a);
+}
+template <int L>
+static void launch_far(EvalArgs a, hipStream_t s) {
+    if (a.n <= 0) return;
+    hipLaunchKernelGGL(k_far<L>, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
+}
+template <int L>
+static void launch_lr(LRArgs a, hipStream_t s) {
+    if (a.T <= 0) return;
+    hipLaunchKernelGGL(k_lr_prop<L>, dim3(a.T), dim3(64), 0, s, a);
+}
+template <int L>
+static void launch_mp(int n, int chunk, const double4* p, double* partial, int nblk, hipStream_t s) {
+    hipLaunchKernelGGL(k_multipole_partial<L>, dim3(nblk), dim3(64), 0, s, n, chunk, p, partial);
+}
+template <int L>
+static void launch_md(const MoveRec* r, double* M, hipStream_t s) {
+    hipLaunchKernelGGL(k_multipole_delta<L>, dim3(1), dim3(64), 0, s, r, M);
+}
+
+struct OrderFns {
+    SweepLaunch sweep;
+    EvalLaunch far;
+    LRLaunch lr;
+    MpLaunch mp;
+    MdLaunch md;
+};
+template <int L>
+static OrderFns fns_for() {
+    return {launch_sweep<L>, launch_far<L>, launch_lr<L>, launch_mp<L>, launch_md<L>};
+}
+static bool order_fns(int L, OrderFns* f) {
+    switch (L) {
+#define KMC_CASE(x) case x: *f = fns_for<x>(); return true;
+        KMC_CASE(2) KMC_CASE(3) KMC_CASE(4) KMC_CASE(5) KMC_CASE(6) KMC_CASE(7) KMC_CASE(8)
+        KMC_CASE(9) KMC_CASE(10) KMC_CASE(11) KMC_CASE(12) KMC_CASE(13) KMC_CASE(14) KMC_CASE(15)
+        KMC_CASE(16) KMC_CASE(17) KMC_CASE(18) KMC_CASE(19) KMC_CASE(20)
+#undef KMC_CASE
+        default: return false;
+    }
+}
+
+template <bool MIXED>
+static void launch_near_t(EvalArgs a, int G, hipStream_t s) {
+    if (a.n <= 0) return;
+    const long long threads = (long long)a.n * G;
+    const int blocks = (int)((threads + 255) / 256);
+    switch (G) {
+        case 1: hipLaunchKernelGGL((k_near<1, MIXED>), dim3(blocks), dim3(256), 0, s, a); break;
+        case 4: hipLaunchKernelGGL((k_near<4, MIXED>), dim3(blocks), dim3(256), 0, s, a); break;
+        case 8: hipLaunchKernelGGL((k_near<8, MIXED>), dim3(blocks), dim3(256), 0, s, a); break;
+        case 16: hipLaunchKernelGGL((k_near<16, MIXED>), dim3(blocks), dim3(256), 0, s, a); break;
+        case 32: hipLaunchKernelGGL((k_near<32, MIXED>), dim3(blocks), dim3(256), 0, s, a); break;
+        default: hipLaunchKernelGGL((k_near<64, MIXED>), dim3(blocks), dim3(256), 0, s, a); break;
+    }
+}
+static void launch_near(EvalArgs a, int G, bool mixed, hipStream_t s) {
+    if (mixed) launch_near_t<true>(a, G, s);
+    else launch_near_t<false>(a, G, s);
+}
+
+// ============================================================================
+// context
+// ============================================================================
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t want) {
+        if (want <= n && p) return hipSuccess;
+        if (p) { hipFree(p); p = nullptr; n = 0; }
+        if (want == 0) want = 1;
+        hipError_t e = hipMalloc((void**)&p, want * sizeof(T));
+        if (e == hipSuccess) n = want;
+        return e;
+    }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+struct kmc_ctx {
+    int64_t n = 0;
+    int L = 0, r = 0, bc = 0, ncps = 0, nh = 0;
+    int64_t ncells = 0;
+    double E = 0, w = 0, max_move = 0, eu = 1.0, self27 = 0.0;
+    int occ_headroom = 8;
+    int occmax = 0;
+    int near_lanes = 8;
+    int sweep_block = 256;
+    bool have_particles = false, have_lr = false, initialised = false;
+    OrderFns fns{};
+
+    // particles
+    DBuf<double4> ppos4;
+    DBuf<int32_t> pcell;
+    std::vector<double> h_pos, h_q;
+    std::vector<int32_t> h_cell;
+    std::vector<int32_t> h_occ;
+    // cell tables
+    DBuf<int> occ;
+    DBuf<double4> slot_pos;
+    DBuf<int32_t> slot_id;
+    // expansions
+    DBuf<double2> lexp;
+    // long range
+    DBuf<double> M, gvec, gdat, elr;
+    DBuf<int64_t> gptr;
+    DBuf<int32_t> gidx;
+    int64_t gnnz = 0;
+    // scalars
+    DBuf<double> energy;
+    DBuf<int> flags, err;
+    DBuf<MoveRec> rec;
+    // propose work buffers
+    DBuf<double4> epos4;
+    DBuf<int32_t> ecell, eexcl, owner;
+    DBuf<double> UD, UI, lrd, dE1;
+    DBuf<int64_t> one_pid, one_es;
+    DBuf<double> one_pos;
+    // propose_with_dats work buffers
+    DBuf<int64_t> d_cnt, d_has, d_slot, d_off, d_pid, d_es, d_rate, d_tot;
+    DBuf<double> d_npos, d_nq;
+
+    Geom geom() const { return Geom{E, w, -0.5 * E + 0.5 * w, ncps, bc}; }
+};
+
+static int check_ctx(kmc_ctx* c) {
+    if (!c) return set_err(KMC_E_ARG, "null context");
+    return KMC_OK;
+}
+
+static int ensure_eval(kmc_ctx* c, int64_t T, int64_t P) {
+    const size_t ne = (size_t)(T + P);
+    HIPCHK(c->epos4.ensure(ne));
+    HIPCHK(c->ecell.ensure(ne));
+    HIPCHK(c->eexcl.ensure(ne));
+    HIPCHK(c->owner.ensure((size_t)std::max<int64_t>(T, 1)));
+    HIPCHK(c->UD.ensure(ne));
+    HIPCHK(c->UI.ensure(ne));
+    HIPCHK(c->lrd.ensure((size_t)std::max<int64_t>(T, 1)));
+    return KMC_OK;
+}
+
+static EvalArgs eval_args(kmc_ctx* c, int n, const double4* pos4, const int32_t* cell, const int32_t* excl, double* out) {
+    EvalArgs a;
+    a.g = c->geom();
+    a.n = n;
+    a.pos4 = pos4;
+    a.cell = cell;
+    a.excl = excl;
+    a.out = out;
+    a.lexp = c->lexp.p;
+    a.occ = c->occ.p;
+    a.slot_pos = c->slot_pos.p;
+    a.slot_id = c->slot_id.p;
+    a.occmax = c->occmax;
+    return a;
+}
+
+// propose kernels after the evaluation arrays are filled (T new points, then P old)
+static int run_propose_core(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, double* d_out, const int64_t* rate_loc,
+                            bool mixed) {
+    const int ne = (int)(T + P);
+    if (c->bc != KMC_BC_FF_ONLY) {
+        EvalArgs an = eval_args(c, ne, c->epos4.p, c->ecell.p, c->eexcl.p, c->UD.p);
+        launch_near(an, c->near_lanes, mixed, s);
+        EvalArgs af = eval_args(c, ne, c->epos4.p, c->ecell.p, c->eexcl.p, c->UI.p);
+        c->fns.far(af, s);
+    }
+    const bool lr = (c->bc == KMC_BC_PBC || c->bc == KMC_BC_FF_ONLY) && c->have_lr;
+    if (lr) {
+        LRArgs la;
+        la.T = (int)T;
+        la.nrow = 2 * c->nh;
+        la.gdat = c->gdat.p;
+        la.gptr = c->gptr.p;
+        la.gidx = c->gidx.p;
+        la.gvec = c->gvec.p;
+        la.epos4 = c->epos4.p;
+        la.owner = c->owner.p;
+        la.lrd = c->lrd.p;
+        c->fns.lr(la, s);
+    }
+    CombineArgs ca;
+    ca.g = c->geom();
+    ca.T = (int)T;
+    ca.eu = c->eu;
+    ca.self27 = c->self27;
+    ca.epos4 = c->epos4.p;
+    ca.owner = c->owner.p;
+    ca.UD = c->UD.p;
+    ca.UI = c->UI.p;
+    ca.lrd = lr ? c->lrd.p : nullptr;
+    ca.rate_loc = rate_loc;
+    ca.out = d_out;
+    if (T > 0) {
+        if (mixed) hipLaunchKernelGGL(k_combine<true>, dim3((T + 255) / 256), dim3(256), 0, s, ca);
+        else hipLaunchKernelGGL(k_combine<false>, dim3((T + 255) / 256), dim3(256), 0, s, ca);
+    }
+    HIPCHK(hipGetLastError());
+    return KMC_OK;
+}
+
+static int launch_setup(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, const int64_t* d_pid, const int64_t* d_es,
+                        const double* d_new_pos, const double* d_new_q) {
+    SetupArgs a;
+    a.g = c->geom();
+    a.max_move = c->max_move;
+    a.P = (int)P;
+    a.T = (int)T;
+    a.n = (int)c->n;
+    a.pid = d_pid;
+    a.es = d_es;
+    a.new_pos = d_new_pos;
+    a.new_q = d_new_q;
+    a.ppos4 = c->ppos4.p;
+    a.pcell = c->pcell.p;
+    a.epos4 = c->epos4.p;
+    a.ecell = c->ecell.p;
+    a.eexcl = c->eexcl.p;
+    a.owner = c->owner.p;
+    a.err = c->err.p;
+    const int64_t nthr = std::max(T, P);
+    if (P > 0) hipLaunchKernelGGL(k_setup, dim3((nthr + 255) / 256), dim3(256), 0, s, a);
+    HIPCHK(hipGetLastError());
+    return KMC_OK;
+}
+
+static int run_accept_tail(kmc_ctx* c, hipStream_t s) {
+    if (c->bc != KMC_BC_FF_ONLY) {
+        SweepArgs sa;
+        sa.g = c->geom();
+        sa.lexp = c->lexp.p;
+        sa.ncells = (int)c->ncells;
+        sa.mode = 0;
+        sa.rec = c->rec.p;
+        sa.src_pos4 = nullptr;
+        sa.src_cell = nullptr;
+        sa.src_begin = sa.src_end = 0;
+        c->fns.sweep(sa, (int)c->ncells, s, c->sweep_block);
+        hipLaunchKernelGGL(k_accept_store, dim3(1), dim3(1), 0, s, c->rec.p, c->occ.p, c->slot_pos.p, c->slot_id.p,
+                           c->occmax, c->ppos4.p, c->pcell.p, c->flags.p);
+    } else {
+        // no cell tables in 'ff-only': only the particle arrays move
+        hipLaunchKernelGGL(k_accept_store, dim3(1), dim3(1), 0, s, c->rec.p, c->occ.p, c->slot_pos.p, c->slot_id.p,
+                           c->occmax, c->ppos4.p, c->pcell.p, c->flags.p);
+    }
+    if ((c->bc == KMC_BC_PBC || c->bc == KMC_BC_FF_ONLY) && c->have_lr) {
+        c->fns.md(c->rec.p, c->M.p, s);
+        hipLaunchKernelGGL(k_lr_refresh, dim3(1), dim3(256), 0, s, 2 * c->nh, c->gdat.p, c->gptr.p, c->gidx.p, c->M.p,
+                           c->gvec.p, c->elr.p);
+    }
+    HIPCHK(hipGetLastError());
+    return KMC_OK;
+}
+
+static int grow_store(kmc_ctx* c, int newmax, hipStream_t s) {
+    if (newmax <= c->occmax) return KMC_OK;
+    const size_t nslots = (size_t)c->ncells * newmax;
+    double4* np = nullptr;
+    int32_t* ni = nullptr;
+    HIPCHK(hipMalloc((void**)&np, nslots * sizeof(double4)));
+    HIPCHK(hipMalloc((void**)&ni, nslots * sizeof(int32_t)));
+    HIPCHK(hipMemsetAsync(np, 0, nslots * sizeof(double4), s));
+    HIPCHK(hipMemsetAsync(ni, 0xff, nslots * sizeof(int32_t), s));
+    HIPCHK(hipMemcpy2DAsync(np, (size_t)newmax * sizeof(double4), c->slot_pos.p, (size_t)c->occmax * sizeof(double4),
+                            (size_t)c->occmax * sizeof(double4), (size_t)c->ncells, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpy2DAsync(ni, (size_t)newmax * sizeof(int32_t), c->slot_id.p, (size_t)c->occmax * sizeof(int32_t),
+                            (size_t)c->occmax * sizeof(int32_t), (size_t)c->ncells, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    c->slot_pos.release();
+    c->slot_id.release();
+    c->slot_pos.p = np;
+    c->slot_pos.n = nslots;
+    c->slot_id.p = ni;
+    c->slot_id.n = nslots;
+    c->occmax = newmax;
+    return KMC_OK;
+}
+
+// ============================================================================
+// C ABI
+// ============================================================================
+extern "C" {
+
+const char* kmc_last_error(void) { return g_err.c_str(); }
+int32_t kmc_min_order(void) { return KMIN_L; }
+int32_t kmc_max_order(void) { return KMAX_L; }
+
+int kmc_create(kmc_ctx** out, int64_t n, double extent, int32_t r, int32_t L, int32_t bc, double max_move,
+               double energy_unit, int32_t occ_headroom) {
+    if (!out) return set_err(KMC_E_ARG, "null output pointer");
+    *out = nullptr;
+    if (n < 1 || n >= (int64_t)1 << 31) return set_err(KMC_E_ARG, "n out of range");
+    if (!(extent > 0)) return set_err(KMC_E_ARG, "extent must be positive");
+    if (r < 1 || r > 11) return set_err(KMC_E_ARG, "r out of range");
+    if (bc < 0 || bc > 3) return set_err(KMC_E_ARG, "unknown boundary condition");
+    kmc_ctx* c = new kmc_ctx();
+    if (!order_fns(L, &c->fns)) {
+        delete c;
+        return set_err(KMC_E_UNSUPPORTED, "expansion order L=" + std::to_string(L) + " not built (2..20)");
+    }
+    c->n = n;
+    c->E = extent;
+    c->r = r;
+    c->L = L;
+    c->bc = bc;
+    c->ncps = 1 << (r - 1);
+    c->ncells = (int64_t)c->ncps * c->ncps * c->ncps;
+    c->w = extent / c->ncps;
+    c->nh = nhalf(L);
+    c->max_move = max_move > 0 ? max_move : extent;
+    c->eu = energy_unit;
+    c->occ_headroom = occ_headroom >= 0 ? occ_headroom : 8;
+    // kmc_fmm_self_interaction.py:48-54
+    double s27 = 0.0;
+    for (int k = 0; k < 27; k++) {
+        const int ox = k % 3 - 1, oy = (k / 3) % 3 - 1, oz = k / 9 - 1;
+        if (ox == 0 && oy == 0 && oz == 0) continue;
+        const double dx = extent * ox, dy = extent * oy, dz = extent * oz;
+        s27 -= 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    c->self27 = s27;
+    int rc = KMC_OK;
+    if (c->energy.ensure(1) != hipSuccess || c->flags.ensure(1) != hipSuccess || c->err.ensure(1) != hipSuccess ||
+        c->rec.ensure(1) != hipSuccess || c->elr.ensure(1) != hipSuccess || c->dE1.ensure(1) != hipSuccess ||
+        c->one_pid.ensure(1) != hipSuccess || c->one_es.ensure(2) != hipSuccess || c->one_pos.ensure(3) != hipSuccess) {
+        rc = set_err(KMC_E_HIP, "device allocation failed");
+    }
+    if (rc == KMC_OK) {
+        hipMemset(c->energy.p, 0, sizeof(double));
+        hipMemset(c->flags.p, 0, sizeof(int));
+        hipMemset(c->err.p, 0, sizeof(int));
+        hipMemset(c->elr.p, 0, sizeof(double));
+        MoveRec z{};
+        hipMemcpy(c->rec.p, &z, sizeof(MoveRec), hipMemcpyHostToDevice);
+    }
+    if (rc != KMC_OK) {
+        kmc_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return KMC_OK;
+}
+
+int kmc_destroy(kmc_ctx* c) {
+    if (!c) return KMC_OK;
+    c->ppos4.release(); c->pcell.release(); c->occ.release(); c->slot_pos.release(); c->slot_id.release();
+    c->lexp.release(); c->M.release(); c->gvec.release(); c->gdat.release(); c->elr.release(); c->gptr.release();
+    c->gidx.release(); c->energy.release(); c->flags.release(); c->err.release(); c->rec.release();
+    c->epos4.release(); c->ecell.release(); c->eexcl.release(); c->owner.release(); c->UD.release();
+    c->UI.release(); c->lrd.release(); c->dE1.release(); c->one_pid.release(); c->one_es.release();
+    c->one_pos.release(); c->d_cnt.release(); c->d_has.release(); c->d_slot.release(); c->d_off.release();
+    c->d_pid.release(); c->d_es.release(); c->d_rate.release(); c->d_tot.release(); c->d_npos.release();
+    c->d_nq.release();
+    delete c;
+    return KMC_OK;
+}
+
+int kmc_set_tiles(kmc_ctx* c, int32_t near_lanes, int32_t sweep_block) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    if (near_lanes != 1 && near_lanes != 4 && near_lanes != 8 && near_lanes != 16 && near_lanes != 32 && near_lanes != 64)
+        return set_err(KMC_E_ARG, "near_lanes must be one of 1,4,8,16,32,64");
+    if (sweep_block < 64 || sweep_block > 1024 || sweep_block % 64)
+        return set_err(KMC_E_ARG, "sweep_block must be a multiple of 64 in [64,1024]");
+    c->near_lanes = near_lanes;
+    c->sweep_block = sweep_block;
+    return KMC_OK;
+}
+
+int kmc_set_particles(kmc_ctx* c, const double* h_pos, const double* h_q) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    if (!h_pos || !h_q) return set_err(KMC_E_ARG, "null input");
+    const int64_t n = c->n;
+    c->h_pos.assign(h_pos, h_pos + 3 * n);
+    c->h_q.assign(h_q, h_q + n);
+    c->h_cell.resize(n);
+    c->h_occ.assign(c->ncells, 0);
+    for (int64_t i = 0; i < n; i++) {
+        const int cx = bin_axis(h_pos[3 * i], c->E, c->w, c->ncps);
+        const int cy = bin_axis(h_pos[3 * i + 1], c->E, c->w, c->ncps);
+        const int cz = bin_axis(h_pos[3 * i + 2], c->E, c->w, c->ncps);
+        c->h_cell[i] = cx + c->ncps * (cy + c->ncps * cz);
+        c->h_occ[c->h_cell[i]]++;
+    }
+    int mx = 1;
+    for (int v : c->h_occ) mx = std::max(mx, v);
+    c->occmax = mx + c->occ_headroom;
+    // particles of a leaf in ascending index (kmc_local.py:606-615)
+    std::vector<int64_t> start(c->ncells + 1, 0);
+    for (int64_t k = 0; k < c->ncells; k++) start[k + 1] = start[k] + c->h_occ[k];
+    std::vector<double4> slots((size_t)c->ncells * c->occmax, make_double4(0, 0, 0, 0));
+    std::vector<int32_t> ids((size_t)c->ncells * c->occmax, -1);
+    std::vector<int> fill(c->ncells, 0);
+    std::vector<double4> p4(n);
+    for (int64_t i = 0; i < n; i++) {
+        const int k = c->h_cell[i];
+        const size_t sidx = (size_t)k * c->occmax + fill[k]++;
+        p4[i] = make_double4(h_pos[3 * i], h_pos[3 * i + 1], h_pos[3 * i + 2], h_q[i]);
+        slots[sidx] = p4[i];
+        ids[sidx] = (int32_t)i;
+    }
+    HIPCHK(c->ppos4.ensure(n));
+    HIPCHK(c->pcell.ensure(n));
+    HIPCHK(c->occ.ensure(c->ncells));
+    HIPCHK(c->slot_pos.ensure(slots.size()));
+    HIPCHK(c->slot_id.ensure(ids.size()));
+    HIPCHK(c->lexp.ensure((size_t)c->ncells * c->nh));
+    HIPCHK(hipMemcpy(c->ppos4.p, p4.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->pcell.p, c->h_cell.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->occ.p, c->h_occ.data(), c->ncells * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->slot_pos.p, slots.data(), slots.size() * sizeof(double4), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->slot_id.p, ids.data(), ids.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(c->lexp.p, 0, (size_t)c->ncells * c->nh * sizeof(double2)));
+    HIPCHK(c->M.ensure(2 * c->nh));
+    HIPCHK(c->gvec.ensure(2 * c->nh));
+    HIPCHK(hipMemset(c->M.p, 0, 2 * c->nh * sizeof(double)));
+    HIPCHK(hipMemset(c->gvec.p, 0, 2 * c->nh * sizeof(double)));
+    c->have_particles = true;
+    c->initialised = false;
+    return KMC_OK;
+}
+
+int kmc_set_lr_operator(kmc_ctx* c, int64_t nnz, const double* h_data, const int64_t* h_indptr,
+                        const int64_t* h_indices) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    const int nrow = 2 * c->nh;
+    if (nnz < 0 || !h_indptr || (nnz > 0 && (!h_data || !h_indices))) return set_err(KMC_E_ARG, "bad CSR");
+    if (h_indptr[0] != 0 || h_indptr[nrow] != nnz) return set_err(KMC_E_ARG, "CSR indptr does not match 2*L(L+1)/2 rows");
+    std::vector<int32_t> idx(std::max<int64_t>(nnz, 1), 0);
+    for (int64_t k = 0; k < nnz; k++) {
+        if (h_indices[k] < 0 || h_indices[k] >= nrow) return set_err(KMC_E_ARG, "CSR column out of range");
+        idx[k] = (int32_t)h_indices[k];
+    }
+    HIPCHK(c->gdat.ensure(std::max<int64_t>(nnz, 1)));
+    HIPCHK(c->gidx.ensure(std::max<int64_t>(nnz, 1)));
+    HIPCHK(c->gptr.ensure(nrow + 1));
+    if (nnz > 0) {
+        HIPCHK(hipMemcpy(c->gdat.p, h_data, nnz * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->gidx.p, idx.data(), nnz * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemcpy(c->gptr.p, h_indptr, (nrow + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+    c->gnnz = nnz;
+    c->have_lr = true;
+    return KMC_OK;
+}
+
+int kmc_initialise(kmc_ctx* c, void* stream, double* h_energy) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    if (!c->have_particles) return set_err(KMC_E_STATE, "set particles before initialise");
+    if ((c->bc == KMC_BC_PBC || c->bc == KMC_BC_FF_ONLY) && !c->have_lr)
+        return set_err(KMC_E_STATE, "set the long-range operator before initialise (pbc / ff-only)");
+    hipStream_t s = (hipStream_t)stream;
+    const int n = (int)c->n;
+    HIPCHK(hipMemsetAsync(c->lexp.p, 0, (size_t)c->ncells * c->nh * sizeof(double2), s));
+    if (c->bc != KMC_BC_FF_ONLY) {
+        // P2L of every charge into every well-separated leaf, in source chunks
+        const int chunk = 2048;
+        for (int b = 0; b < n; b += chunk) {
+            SweepArgs sa;
+            sa.g = c->geom();
+            sa.lexp = c->lexp.p;
+            sa.ncells = (int)c->ncells;
+            sa.mode = 1;
+            sa.rec = nullptr;
+            sa.src_pos4 = c->ppos4.p;
+            sa.src_cell = c->pcell.p;
+            sa.src_begin = b;
+            sa.src_end = std::min(n, b + chunk);
+            c->fns.sweep(sa, (int)c->ncells, s, c->sweep_block);
+            HIPCHK(hipGetLastError());
+        }
+    }
+    // level-0 multipole M
+    {
+        const int chunk = 1024;
+        const int nblk = (n + chunk - 1) / chunk;
+        DBuf<double> partial;
+        HIPCHK(partial.ensure((size_t)nblk * 2 * c->nh));
+        c->fns.mp(n, chunk, c->ppos4.p, partial.p, nblk, s);
+        hipLaunchKernelGGL(k_reduce_rows, dim3((2 * c->nh + 255) / 256), dim3(256), 0, s, nblk, 2 * c->nh, partial.p,
+                           c->M.p);
+        HIPCHK(hipStreamSynchronize(s));
+        partial.release();
+    }
+    if (c->have_lr)
+        hipLaunchKernelGGL(k_lr_refresh, dim3(1), dim3(256), 0, s, 2 * c->nh, c->gdat.p, c->gptr.p, c->gidx.p, c->M.p,
+                           c->gvec.p, c->elr.p);
+    // energy: 1/2 sum_i q_i phi(x_i) + long range
+    DBuf<int32_t> excl;
+    DBuf<double> ud, ui, esr;
+    HIPCHK(excl.ensure(n));
+    HIPCHK(ud.ensure(n));
+    HIPCHK(ui.ensure(n));
+    HIPCHK(esr.ensure(1));
+    {
+        std::vector<int32_t> ids(n);
+        for (int i = 0; i < n; i++) ids[i] = i;
+        HIPCHK(hipMemcpyAsync(excl.p, ids.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemsetAsync(esr.p, 0, sizeof(double), s));
+        if (c->bc != KMC_BC_FF_ONLY) {
+            EvalArgs an = eval_args(c, n, c->ppos4.p, c->pcell.p, excl.p, ud.p);
+            launch_near(an, c->near_lanes, false, s);
+            EvalArgs af = eval_args(c, n, c->ppos4.p, c->pcell.p, excl.p, ui.p);
+            c->fns.far(af, s);
+            hipLaunchKernelGGL(k_half_sum, dim3(1), dim3(256), 0, s, n, ud.p, ui.p, esr.p);
+        }
+        hipLaunchKernelGGL(k_finish_energy, dim3(1), dim3(1), 0, s, esr.p, c->elr.p, c->eu, c->bc, c->energy.p);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    excl.release();
+    ud.release();
+    ui.release();
+    esr.release();
+    c->initialised = true;
+    if (h_energy) HIPCHK(hipMemcpy(h_energy, c->energy.p, sizeof(double), hipMemcpyDeviceToHost));
+    return KMC_OK;
+}
+
+int kmc_get_expansions(kmc_ctx* c, double* h_out) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    const int L = c->L, L2 = L * L;
+    std::vector<double2> h((size_t)c->ncells * c->nh);
+    HIPCHK(hipMemcpy(h.data(), c->lexp.p, h.size() * sizeof(double2), hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < c->ncells; k++) {
+        double* o = h_out + (size_t)k * 2 * L2;
+        const double2* v = &h[(size_t)k * c->nh];
+        for (int l = 0; l < L; l++)
+            for (int m = -l; m <= l; m++) {
+                const double2 x = v[hidx(l, m < 0 ? -m : m)];
+                o[l * l + l + m] = x.x;
+                o[L2 + l * l + l + m] = m < 0 ? -x.y : x.y;
+            }
+    }
+    return KMC_OK;
+}
+
+int kmc_set_expansions(kmc_ctx* c, const double* h_in) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    if (!c->have_particles) return set_err(KMC_E_STATE, "set particles first");
+    const int L = c->L, L2 = L * L;
+    std::vector<double2> h((size_t)c->ncells * c->nh);
+    for (int64_t k = 0; k < c->ncells; k++) {
+        const double* o = h_in + (size_t)k * 2 * L2;
+        for (int l = 0; l < L; l++)
+            for (int m = 0; m <= l; m++) h[(size_t)k * c->nh + hidx(l, m)] = make_double2(o[l * l + l + m], o[L2 + l * l + l + m]);
+    }
+    HIPCHK(hipMemcpy(c->lexp.p, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice));
+    return KMC_OK;
+}
+
+int kmc_get_multipole(kmc_ctx* c, double* h_out) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    const int L = c->L, L2 = L * L;
+    std::vector<double> h(2 * c->nh);
+    HIPCHK(hipMemcpy(h.data(), c->M.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int l = 0; l < L; l++)
+        for (int m = -l; m <= l; m++) {
+            const int hh = hidx(l, m < 0 ? -m : m);
+            h_out[l * l + l + m] = h[2 * hh];
+            h_out[L2 + l * l + l + m] = m < 0 ? -h[2 * hh + 1] : h[2 * hh + 1];
+        }
+    return KMC_OK;
+}
+
+int kmc_store_occmax(kmc_ctx* c, int32_t* out) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    *out = c->occmax;
+    return KMC_OK;
+}
+
+int kmc_get_store(kmc_ctx* c, int32_t* h_occ, int32_t* h_ids, double* h_pos4, int32_t* h_occmax) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    if (h_occmax) *h_occmax = c->occmax;
+    if (h_occ) HIPCHK(hipMemcpy(h_occ, c->occ.p, c->ncells * sizeof(int), hipMemcpyDeviceToHost));
+    if (h_ids) HIPCHK(hipMemcpy(h_ids, c->slot_id.p, (size_t)c->ncells * c->occmax * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (h_pos4) HIPCHK(hipMemcpy(h_pos4, c->slot_pos.p, (size_t)c->ncells * c->occmax * sizeof(double4), hipMemcpyDeviceToHost));
+    return KMC_OK;
+}
+
+int kmc_get_particles(kmc_ctx* c, double* h_pos, int32_t* h_cells) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    std::vector<double4> p4(c->n);
+    HIPCHK(hipMemcpy(p4.data(), c->ppos4.p, c->n * sizeof(double4), hipMemcpyDeviceToHost));
+    if (h_pos)
+        for (int64_t i = 0; i < c->n; i++) {
+            h_pos[3 * i] = p4[i].x;
+            h_pos[3 * i + 1] = p4[i].y;
+            h_pos[3 * i + 2] = p4[i].z;
+        }
+    if (h_cells) HIPCHK(hipMemcpy(h_cells, c->pcell.p, c->n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return KMC_OK;
+}
+
+int kmc_get_energy(kmc_ctx* c, double* h_energy) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    HIPCHK(hipMemcpy(h_energy, c->energy.p, sizeof(double), hipMemcpyDeviceToHost));
+    return KMC_OK;
+}
+
+int kmc_set_energy(kmc_ctx* c, double e) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    HIPCHK(hipMemcpy(c->energy.p, &e, sizeof(double), hipMemcpyHostToDevice));
+    return KMC_OK;
+}
+
+int kmc_ncells(kmc_ctx* c, int64_t* out) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    *out = c->ncells;
+    return KMC_OK;
+}
+
+int kmc_propose(kmc_ctx* c, void* stream, int64_t P, int64_t T, const int64_t* d_pid, const int64_t* d_es,
+                const double* d_new_pos, const double* d_new_q, double* d_dE, int32_t precision) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    if (!c->initialised) return set_err(KMC_E_STATE, "Run initialise before this call");
+    if (P < 0 || T < 0 || T >= (int64_t)1 << 30) return set_err(KMC_E_ARG, "bad proposal counts");
+    if (P == 0 || T == 0) return KMC_OK;
+    if (!d_pid || !d_es || !d_new_pos || !d_dE) return set_err(KMC_E_ARG, "null device pointer");
+    hipStream_t s = (hipStream_t)stream;
+    int rc = ensure_eval(c, T, P);
+    if (rc) return rc;
+    HIPCHK(hipMemsetAsync(c->err.p, 0, sizeof(int), s));
+    rc = launch_setup(c, s, P, T, d_pid, d_es, d_new_pos, d_new_q);
+    if (rc) return rc;
+    int herr = 0;
+    HIPCHK(hipMemcpyAsync(&herr, c->err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (herr) return set_err(KMC_E_BAD_MOVE, "Bad proposed move was detected.");
+    return run_propose_core(c, s, P, T, d_dE, nullptr, precision == KMC_PREC_MIXED);
+}
+
+int kmc_propose_with_dats(kmc_ctx* c, void* stream, int64_t max_prop, const int64_t* d_smc, const int64_t* d_sites,
+                          const double* d_ppos, const int64_t* d_masks, const double* d_pq, double* d_out,
+                          int32_t precision, int64_t* h_total_moves) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    if (!c->initialised) return set_err(KMC_E_STATE, "Run initialise before this call");
+    if (max_prop < 1) return set_err(KMC_E_ARG, "max_prop must be >= 1");
+    hipStream_t s = (hipStream_t)stream;
+    const int n = (int)c->n;
+    HIPCHK(c->d_cnt.ensure(n));
+    HIPCHK(c->d_has.ensure(n));
+    HIPCHK(c->d_slot.ensure(n));
+    HIPCHK(c->d_off.ensure(n));
+    HIPCHK(c->d_tot.ensure(2));
+    hipLaunchKernelGGL(k_dats_count, dim3((n + 255) / 256), dim3(256), 0, s, n, max_prop, d_smc, d_sites, d_masks,
+                       c->d_cnt.p, c->d_has.p);
+    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, s, n, c->d_cnt.p, c->d_off.p, c->d_tot.p);
+    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, s, n, c->d_has.p, c->d_slot.p, c->d_tot.p + 1);
+    int64_t tot[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(tot, c->d_tot.p, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const int64_t T = tot[0], P = tot[1];
+    if (h_total_moves) *h_total_moves = T;
+    if (P == 0 || T == 0) return KMC_OK;
+    HIPCHK(c->d_pid.ensure(P));
+    HIPCHK(c->d_es.ensure(P + 1));
+    HIPCHK(c->d_npos.ensure(3 * T));
+    HIPCHK(c->d_nq.ensure(T));
+    HIPCHK(c->d_rate.ensure(T));
+    hipLaunchKernelGGL(k_dats_scatter, dim3((n + 255) / 256), dim3(256), 0, s, n, max_prop, d_smc, d_sites, d_masks,
+                       d_ppos, d_pq, c->d_cnt.p, c->d_slot.p, c->d_off.p, c->d_pid.p, c->d_es.p, c->d_npos.p,
+                       c->d_nq.p, c->d_rate.p, c->d_tot.p, c->d_tot.p + 1, c->ppos4.p);
+    HIPCHK(hipGetLastError());
+    int rc = ensure_eval(c, T, P);
+    if (rc) return rc;
+    HIPCHK(hipMemsetAsync(c->err.p, 0, sizeof(int), s));
+    rc = launch_setup(c, s, P, T, c->d_pid.p, c->d_es.p, c->d_npos.p, c->d_nq.p);
+    if (rc) return rc;
+    int herr = 0;
+    HIPCHK(hipMemcpyAsync(&herr, c->err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (herr) return set_err(KMC_E_BAD_MOVE, "Bad proposed move was detected.");
+    return run_propose_core(c, s, P, T, d_out, c->d_rate.p, precision == KMC_PREC_MIXED);
+}
+
+int kmc_accept(kmc_ctx* c, void* stream, int64_t pid, const double* h_new_pos, double* h_energy) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    if (!c->initialised) return set_err(KMC_E_STATE, "Run initialise before this call");
+    if (pid < 0 || pid >= c->n || !h_new_pos) return set_err(KMC_E_ARG, "bad move");
+    hipStream_t s = (hipStream_t)stream;
+    // host mirrors: leaf of the new position and capacity of its slots
+    const int n = c->ncps;
+    const int newc = bin_axis(h_new_pos[0], c->E, c->w, n) +
+                     n * (bin_axis(h_new_pos[1], c->E, c->w, n) + n * bin_axis(h_new_pos[2], c->E, c->w, n));
+    const int oldc = c->h_cell[pid];
+    if (newc != oldc && c->h_occ[newc] + 1 > c->occmax) {
+        int rc = grow_store(c, c->occmax + std::max(4, c->occ_headroom), s);
+        if (rc) return rc;
+    }
+    // new energy from a one-proposal FP64 propose (kmc_fmm.py:834-836)
+    const int64_t es[2] = {0, 1};
+    HIPCHK(hipMemcpyAsync(c->one_pid.p, &pid, sizeof(int64_t), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->one_es.p, es, 2 * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->one_pos.p, h_new_pos, 3 * sizeof(double), hipMemcpyHostToDevice, s));
+    int rc = kmc_propose(c, stream, 1, 1, c->one_pid.p, c->one_es.p, c->one_pos.p, nullptr, c->dE1.p, KMC_PREC_FP64);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_prepare_record, dim3(1), dim3(1), 0, s, c->geom(), (int)pid, h_new_pos[0], h_new_pos[1],
+                       h_new_pos[2], c->ppos4.p, c->pcell.p, c->dE1.p, c->energy.p, c->rec.p);
+    rc = run_accept_tail(c, s);
+    if (rc) return rc;
+    int fl = 0;
+    HIPCHK(hipMemcpyAsync(&fl, c->flags.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (fl) {
+        hipMemsetAsync(c->flags.p, 0, sizeof(int), s);
+        return set_err(KMC_E_STORE, "Index was not unique, this is an error.");
+    }
+    c->h_occ[oldc]--;
+    c->h_occ[newc]++;
+    c->h_cell[pid] = newc;
+    for (int d = 0; d < 3; d++) c->h_pos[3 * pid + d] = h_new_pos[d];
+    if (h_energy) HIPCHK(hipMemcpy(h_energy, c->energy.p, sizeof(double), hipMemcpyDeviceToHost));
+    return KMC_OK;
+}
+
+int kmc_step(kmc_ctx* c, void* stream, int64_t P, int64_t T, const int64_t* d_pid, const int64_t* d_es,
+             const double* d_new_pos, double* d_dE, int64_t nsteps, const double* d_uniform, double kT,
+             int64_t* d_counter, int64_t* d_selected, int32_t precision) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    if (!c->initialised) return set_err(KMC_E_STATE, "Run initialise before this call");
+    if (P < 1 || T < 1 || T >= (int64_t)1 << 30 || nsteps < 1) return set_err(KMC_E_ARG, "bad proposal counts");
+    if ((size_t)(T + P) > c->epos4.n || (size_t)T > c->owner.n)
+        return set_err(KMC_E_STATE, "work buffers not sized: call kmc_propose once with this T before kmc_step");
+    hipStream_t s = (hipStream_t)stream;
+    int rc = launch_setup(c, s, P, T, d_pid, d_es, d_new_pos, nullptr);
+    if (rc) return rc;
+    rc = run_propose_core(c, s, P, T, d_dE, nullptr, precision == KMC_PREC_MIXED);
+    if (rc) return rc;
+    SelectArgs sa;
+    sa.g = c->geom();
+    sa.T = (int)T;
+    sa.n = (int)c->n;
+    sa.nsteps = nsteps;
+    sa.kT = kT;
+    sa.dE = d_dE;
+    sa.uniform = d_uniform;
+    sa.counter = d_counter;
+    sa.selected = d_selected;
+    sa.epos4 = c->epos4.p;
+    sa.ecell = c->ecell.p;
+    sa.owner = c->owner.p;
+    sa.pid = d_pid;
+    sa.ppos4 = c->ppos4.p;
+    sa.pcell = c->pcell.p;
+    sa.occ = c->occ.p;
+    sa.occmax = c->occmax;
+    sa.energy = c->energy.p;
+    sa.rec = c->rec.p;
+    sa.flags = c->flags.p;
+    hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, s, sa);
+    rc = run_accept_tail(c, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_counter_inc, dim3(1), dim3(1), 0, s, d_counter);
+    HIPCHK(hipGetLastError());
+    return KMC_OK;
+}
+
+int kmc_materialize_moves(kmc_ctx* c, void* stream, int64_t nsteps, int64_t P, int64_t k,
+                          const int64_t* d_pids_all, const double* d_disp_all, const int64_t* d_counter,
+                          int64_t* d_pid, int64_t* d_es, double* d_new_pos) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    if (P < 1 || k < 1 || nsteps < 1) return set_err(KMC_E_ARG, "bad counts");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t T = P * k;
+    const int64_t threads = std::max(T, P + 1);
+    hipLaunchKernelGGL(k_materialize, dim3((threads + 255) / 256), dim3(256), 0, s, c->geom(), (int)c->n, nsteps,
+                       (int)P, (int)k, d_pids_all, d_disp_all, d_counter, c->ppos4.p, d_pid, d_es, d_new_pos,
+                       c->flags.p);
+    HIPCHK(hipGetLastError());
+    return KMC_OK;
+}
+
+int kmc_check_flags(kmc_ctx* c, void* stream, int32_t* h_flags) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    int fl = 0;
+    HIPCHK(hipMemcpyAsync(&fl, c->flags.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipMemset(c->flags.p, 0, sizeof(int)));
+    if (h_flags) *h_flags = fl;
+    return KMC_OK;
+}
+
+int kmc_sweep_hop(kmc_ctx* c, void* stream, double q, const double* h_old, const double* h_new) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    if (c->bc == KMC_BC_FF_ONLY) return KMC_OK;
+    hipStream_t s = (hipStream_t)stream;
+    MoveRec r{};
+    const int n = c->ncps;
+    for (int d = 0; d < 3; d++) {
+        r.old_pos[d] = h_old[d];
+        r.new_pos[d] = h_new[d];
+    }
+    r.q = q;
+    r.old_cell = bin_axis(h_old[0], c->E, c->w, n) + n * (bin_axis(h_old[1], c->E, c->w, n) + n * bin_axis(h_old[2], c->E, c->w, n));
+    r.new_cell = bin_axis(h_new[0], c->E, c->w, n) + n * (bin_axis(h_new[1], c->E, c->w, n) + n * bin_axis(h_new[2], c->E, c->w, n));
+    r.valid = 1;
+    HIPCHK(hipMemcpyAsync(c->rec.p, &r, sizeof(MoveRec), hipMemcpyHostToDevice, s));
+    SweepArgs sa;
+    sa.g = c->geom();
+    sa.lexp = c->lexp.p;
+    sa.ncells = (int)c->ncells;
+    sa.mode = 0;
+    sa.rec = c->rec.p;
+    sa.src_pos4 = nullptr;
+    sa.src_cell = nullptr;
+    sa.src_begin = sa.src_end = 0;
+    c->fns.sweep(sa, (int)c->ncells, s, c->sweep_block);
+    HIPCHK(hipGetLastError());
+    return KMC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,392 @@
+"""
+KMCFMM: drop-in replacement of coulomb_kmc.KMCFMM (coulomb_kmc/kmc_fmm.py:408-1125)
+for the FMM-KMC propose/accept path, backed by the HIP library (libkmc_amd.so).
+
+Same constructor, same method names, argument meaning, return values and error types:
+
+    kmc = KMCFMM(positions, charges, domain, r=..., l=..., boundary_condition='pbc')
+    kmc.initialise()                 # energy in kmc.energy
+    kmc.propose(((pid, new_positions), ...))          -> tuple of total energies
+    kmc.propose_with_dats(site_max_counts, current_sites, prop_positions,
+                          prop_masks, prop_energy_diffs)  # dE written in place
+    kmc.accept((pid, new_position))  # moves the charge in `positions`
+    kmc.free()
+
+Arrays take the place of ppmd's ParticleDat/PositionDat/ScalarArray (only their
+(npart, ncomp) contents are used on this path).  Extra keyword ``precision``
+selects 'fp64' (default, all FP64) or 'mixed' for the propose kernels.
+"""
+
+import ctypes
+import math
+import time
+from functools import lru_cache
+
+import numpy as np
+
+from . import _native
+from .common import BCType, Domain, leaf_of
+from . import lattice
+
+REAL = ctypes.c_double
+INT64 = ctypes.c_int64
+
+
+def _as_array(obj, ncomp, dtype):
+    """(n, ncomp) numpy view of a dat-like object or array (no copy when possible)."""
+    if isinstance(obj, np.ndarray):
+        a = obj
+    elif hasattr(obj, "view"):
+        a = np.asarray(obj.view)
+    elif hasattr(obj, "data") and not isinstance(obj.data, memoryview):
+        a = np.asarray(obj.data)
+    else:
+        a = np.asarray(obj)
+    if a.ndim == 1:
+        a = a.reshape(-1, ncomp)
+    return a
+
+
+@lru_cache(maxsize=16)
+def _lr_operator_csr(L, extent):
+    G = lattice.half_operator(L, extent)
+    return lattice.csr(G)
+
+
+@lru_cache(maxsize=16)
+def lr_operator_reference_layout(L, extent):
+    """(data, indptr, indices, dipole) of R in the reference layout (for the CPU restatement)."""
+    R = lattice.far_field_matrix(L, extent)
+    d, p, i = lattice.csr(R)
+    return d, p, i, lattice.dipole_correction(extent)
+
+
+def default_levels(N):
+    """Levels r (leaf grid 2^(r-1)) giving at most ~16 charges per leaf, r >= 3."""
+    r = 3
+    while N / float(8 ** (r - 1)) > 16.0 and r < 11:
+        r += 1
+    return r
+
+
+class KMCFMM:
+    """
+    Electrostatic energy differences for kinetic Monte Carlo with the FMM-KMC
+    algorithm, on one MI355X.
+
+    :arg positions: (N, 3) float64 charge positions (updated in place by accept).
+    :arg charges: (N, 1) or (N,) float64 charge values.
+    :arg domain: object with ``extent`` (cubic box centred on the origin), or a float.
+    :arg int N: number of charges (default: len(positions)).
+    :arg str boundary_condition: 'pbc', 'free_space', '27' or 'ff-only'.
+    :arg int r: number of FMM levels (leaf grid 2^(r-1) per side).
+    :arg float energy_unit: multiplies every energy.
+    :arg int l: number of expansion terms L (2..20).
+    :arg float max_move: maximum distance of a proposed move (default: the extent).
+    :arg str precision: 'fp64' or 'mixed' arithmetic for the propose kernels.
+    """
+
+    def __init__(self, positions, charges, domain, N=None, boundary_condition="pbc", r=None, shell_width=0.0,
+                 energy_unit=1.0, _debug=False, l=None, max_move=None, cuda_direct=False, mirror_direction=None,
+                 precision="fp64", occ_headroom=8):
+        if mirror_direction is not None:
+            raise NotImplementedError("mirror charges (Dirichlet boundaries) are not part of this implementation")
+        self._bc = BCType(boundary_condition)
+        self.boundary_condition = self._bc
+        if not hasattr(domain, "extent"):
+            domain = Domain(domain)
+        ext = np.asarray(domain.extent, dtype=np.float64)
+        if not (abs(ext[0] - ext[1]) < 1e-12 * ext[0] and abs(ext[0] - ext[2]) < 1e-12 * ext[0]):
+            raise AssertionError("only cubic domains are supported")
+        self.domain = domain
+        self.positions = positions
+        self.charges = charges
+        pos = _as_array(positions, 3, np.float64)
+        self.N = int(N) if N is not None else int(pos.shape[0])
+        self.extent = float(ext[0])
+        self.R = int(r) if r is not None else default_levels(self.N)
+        self.L = int(l) if l is not None else 12
+        self.energy_unit = float(energy_unit)
+        self.max_move = float(max_move) if max_move is not None else float(max(ext))
+        self.cuda_direct = cuda_direct
+        if precision not in _native.PREC:
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.precision = precision
+        self.energy = None
+        """Total system energy, valid after a call to **initialise**."""
+        self._ncps = 2 ** (self.R - 1)
+        self._lib = _native.lib()
+        self._ctx = ctypes.c_void_p()
+        if not (self._lib.kmc_min_order() <= self.L <= self._lib.kmc_max_order()):
+            raise ValueError("l must be in [{}, {}]".format(self._lib.kmc_min_order(), self._lib.kmc_max_order()))
+        _native.check(self._lib.kmc_create(ctypes.byref(self._ctx), self.N, self.extent, self.R, self.L,
+                                           _native.BC[self._bc.value], self.max_move, self.energy_unit,
+                                           int(occ_headroom)), "kmc_create")
+        self.profile = {}
+
+    # ------------------------------------------------------------------ helpers
+    def _torch(self):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("coulomb_kmc_amd needs a ROCm GPU (torch.cuda is not available)")
+        return torch
+
+    def _stream(self):
+        torch = self._torch()
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def _dev(self, arr, dtype):
+        torch = self._torch()
+        return torch.as_tensor(np.ascontiguousarray(arr), dtype=dtype).cuda()
+
+    def _pos(self):
+        return _as_array(self.positions, 3, np.float64)
+
+    def _q(self):
+        return _as_array(self.charges, 1, np.float64).reshape(-1)
+
+    def _assert_init(self):
+        if self.energy is None:
+            raise RuntimeError("Run initialise before this call")
+
+    def _profile_inc(self, key, inc):
+        self.profile[key] = self.profile.get(key, 0.0) + inc
+
+    @property
+    def handle(self):
+        """The C-ABI context (kmc_ctx*)."""
+        return self._ctx
+
+    def set_tiles(self, near_lanes=8, sweep_block=256):
+        _native.check(self._lib.kmc_set_tiles(self._ctx, int(near_lanes), int(sweep_block)), "kmc_set_tiles")
+
+    # ------------------------------------------------------------------ API
+    def initialise(self):
+        """Initial solve (coulomb_kmc/kmc_fmm.py:947-993); sets ``energy``."""
+        t0 = time.time()
+        pos = np.ascontiguousarray(self._pos()[:self.N], dtype=np.float64)
+        q = np.ascontiguousarray(self._q()[:self.N], dtype=np.float64)
+        _native.check(self._lib.kmc_set_particles(self._ctx, pos.ctypes.data_as(ctypes.c_void_p),
+                                                  q.ctypes.data_as(ctypes.c_void_p)), "kmc_set_particles")
+        if self._bc in (BCType.PBC, BCType.FF_ONLY):
+            d, p, i = _lr_operator_csr(self.L, self.extent)
+            _native.check(self._lib.kmc_set_lr_operator(
+                self._ctx, len(d), d.ctypes.data_as(ctypes.c_void_p), p.ctypes.data_as(ctypes.c_void_p),
+                i.ctypes.data_as(ctypes.c_void_p)), "kmc_set_lr_operator")
+        e = ctypes.c_double(0.0)
+        self._torch()
+        _native.check(self._lib.kmc_initialise(self._ctx, self._stream(), ctypes.byref(e)), "kmc_initialise")
+        self.energy = e.value
+        self._profile_inc("initialise", time.time() - t0)
+
+    def _check_move_valid(self, old_pos, new_pos):
+        # coulomb_kmc/kmc_mpi_decomp.py:312-345
+        e = self.domain.extent
+        for dimx in range(3):
+            if np.any(new_pos[:, dimx] < -0.5 * e[dimx]) or np.any(new_pos[:, dimx] > 0.5 * e[dimx]):
+                raise RuntimeError(
+                    "Proposed position is outside simulation domain. Extent {}, dimension {}, value {}.".format(
+                        e, dimx, new_pos[:, dimx]))
+        rvec = new_pos - old_pos[None, :]
+        if self._bc in (BCType.PBC, BCType.NEAREST, BCType.FF_ONLY):
+            rvec = np.where(rvec < -0.5 * e, rvec + e, rvec)
+            rvec = np.where(rvec > 0.5 * e, rvec - e, rvec)
+        if np.any(np.abs(rvec) > self.max_move):
+            raise RuntimeError("Proposed move violates max_move. Old position {}, new position {}.".format(
+                old_pos, new_pos))
+
+    def propose(self, moves):
+        """Total energies of proposed moves ((pid, new_positions), ...); kmc_fmm.py:774-787."""
+        self._assert_init()
+        t0 = time.time()
+        torch = self._torch()
+        pos = self._pos()
+        pids, es, allpos = [], [0], []
+        for pid, mv in moves:
+            mv = np.atleast_2d(np.asarray(mv, dtype=np.float64))
+            self._check_move_valid(pos[pid], mv)
+            pids.append(int(pid))
+            es.append(es[-1] + mv.shape[0])
+            allpos.append(mv)
+        if not pids:
+            return tuple()
+        T = es[-1]
+        d_pid = self._dev(np.array(pids, np.int64), torch.int64)
+        d_es = self._dev(np.array(es, np.int64), torch.int64)
+        d_pos = self._dev(np.concatenate(allpos), torch.float64)
+        d_dE = torch.empty(T, dtype=torch.float64, device="cuda")
+        _native.check(self._lib.kmc_propose(self._ctx, self._stream(), len(pids), T, d_pid.data_ptr(), d_es.data_ptr(),
+                                            d_pos.data_ptr(), None, d_dE.data_ptr(), _native.PREC[self.precision]),
+                      "kmc_propose")
+        dE = d_dE.cpu().numpy()
+        out = []
+        for i, (pid, mv) in enumerate(moves):
+            mv = np.atleast_2d(np.asarray(mv, dtype=np.float64))
+            vals = self.energy + dE[es[i]:es[i + 1]]
+            same = np.linalg.norm(mv - pos[pid][None, :], axis=1) < 1e-14
+            vals[same] = self.energy
+            out.append(vals)
+        self._profile_inc("propose", time.time() - t0)
+        self._profile_inc("propose_count", T)
+        return tuple(out)
+
+    def propose_with_dats(self, site_max_counts, current_sites, prop_positions, prop_masks, prop_energy_diffs,
+                          diff=True, prop_charges=None):
+        """Energy differences of masked proposals, written into prop_energy_diffs (kmc_fmm.py:665-762)."""
+        self._assert_init()
+        t0 = time.time()
+        if not diff:
+            raise NotImplementedError()
+        torch = self._torch()
+        masks = _as_array(prop_masks, 1, np.int64)
+        diffs = _as_array(prop_energy_diffs, 1, np.float64)
+        ppos = _as_array(prop_positions, 3, np.float64)
+        assert diffs.shape[1] == masks.shape[1]
+        assert diffs.shape[1] * 3 == ppos.shape[1]
+        assert diffs.dtype == np.float64
+        assert masks.dtype == np.int64
+        M = masks.shape[1]
+        n = self.N
+        smc = np.ascontiguousarray(np.asarray(site_max_counts, dtype=np.int64).ravel())
+        sites = np.ascontiguousarray(_as_array(current_sites, 1, np.int64)[:n].ravel())
+        d_smc = self._dev(smc, torch.int64)
+        d_sites = self._dev(sites, torch.int64)
+        d_ppos = self._dev(ppos[:n], torch.float64)
+        d_masks = self._dev(masks[:n], torch.int64)
+        d_out = self._dev(diffs[:n], torch.float64)
+        d_pq = None
+        if prop_charges is not None:
+            d_pq = self._dev(_as_array(prop_charges, 1, np.float64)[:n], torch.float64)
+        tm = ctypes.c_int64(0)
+        _native.check(self._lib.kmc_propose_with_dats(
+            self._ctx, self._stream(), M, d_smc.data_ptr(), d_sites.data_ptr(), d_ppos.data_ptr(), d_masks.data_ptr(),
+            None if d_pq is None else d_pq.data_ptr(), d_out.data_ptr(), _native.PREC[self.precision],
+            ctypes.byref(tm)), "kmc_propose_with_dats")
+        diffs[:n] = d_out.cpu().numpy()
+        self._profile_inc("propose_with_dats", time.time() - t0)
+        self._profile_inc("propose_count", tm.value)
+
+    def get_old_energy_with_dats(self, masks, energy):
+        raise NotImplementedError()
+
+    def accept(self, move=None):
+        """Accept (pid, new_position); moves the charge in ``positions`` (kmc_fmm.py:789-912)."""
+        self._assert_init()
+        if move is None:
+            raise RuntimeError("No moves to accept were passed.")
+        t0 = time.time()
+        pid = int(move[0])
+        newp = np.ascontiguousarray(np.asarray(move[1], dtype=np.float64).reshape(3))
+        pos = self._pos()
+        self._check_move_valid(pos[pid], newp[None, :])
+        e = ctypes.c_double(0.0)
+        self._torch()
+        _native.check(self._lib.kmc_accept(self._ctx, self._stream(), pid, newp.ctypes.data_as(ctypes.c_void_p),
+                                           ctypes.byref(e)), "kmc_accept")
+        self.energy = e.value
+        pos[pid, :] = newp
+        self._profile_inc("accept", time.time() - t0)
+
+    def eval_field(self, points):
+        raise NotImplementedError("eval_field is not part of the propose/accept path built here")
+
+    def free(self):
+        if getattr(self, "_ctx", None) is not None and self._ctx.value:
+            self._lib.kmc_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------ state access
+    @property
+    def ncells(self):
+        return self._ncps ** 3
+
+    def get_expansions(self):
+        """Leaf local expansions in the reference layout [ncells, 2 L^2]."""
+        out = np.zeros((self.ncells, 2 * self.L * self.L))
+        _native.check(self._lib.kmc_get_expansions(self._ctx, out.ctypes.data_as(ctypes.c_void_p)), "get_expansions")
+        return out
+
+    def set_expansions(self, full):
+        full = np.ascontiguousarray(full, dtype=np.float64)
+        _native.check(self._lib.kmc_set_expansions(self._ctx, full.ctypes.data_as(ctypes.c_void_p)), "set_expansions")
+
+    def get_multipole(self):
+        out = np.zeros(2 * self.L * self.L)
+        _native.check(self._lib.kmc_get_multipole(self._ctx, out.ctypes.data_as(ctypes.c_void_p)), "get_multipole")
+        return out
+
+    def get_store(self):
+        """(occ[ncells], ids[ncells, occmax], pos4[ncells, occmax, 4]) of the cell tables."""
+        om = ctypes.c_int32(0)
+        _native.check(self._lib.kmc_store_occmax(self._ctx, ctypes.byref(om)), "occmax")
+        occ = np.zeros(self.ncells, np.int32)
+        ids = np.zeros((self.ncells, om.value), np.int32)
+        p4 = np.zeros((self.ncells, om.value, 4))
+        _native.check(self._lib.kmc_get_store(self._ctx, occ.ctypes.data_as(ctypes.c_void_p),
+                                              ids.ctypes.data_as(ctypes.c_void_p), p4.ctypes.data_as(ctypes.c_void_p),
+                                              None), "get_store")
+        return occ, ids, p4
+
+    def get_device_particles(self):
+        pos = np.zeros((self.N, 3))
+        cells = np.zeros(self.N, np.int32)
+        _native.check(self._lib.kmc_get_particles(self._ctx, pos.ctypes.data_as(ctypes.c_void_p),
+                                                  cells.ctypes.data_as(ctypes.c_void_p)), "get_particles")
+        return pos, cells
+
+    def device_energy(self):
+        e = ctypes.c_double(0.0)
+        _native.check(self._lib.kmc_get_energy(self._ctx, ctypes.byref(e)), "get_energy")
+        return e.value
+
+    def leaf_of(self, pos):
+        return leaf_of(pos, self.extent, self._ncps)
+
+    # ------------------------------------------------------------------ device-resident path
+    def propose_device(self, d_pid, d_es, d_new_pos, d_dE, precision=None, d_new_q=None, stream=None):
+        """kmc_propose on torch device tensors (int64 pid[P], es[P+1], float64 new_pos[T,3], dE[T])."""
+        P = d_pid.numel()
+        T = d_new_pos.numel() // 3
+        prec = _native.PREC[precision or self.precision]
+        s = stream if stream is not None else self._stream()
+        _native.check(self._lib.kmc_propose(self._ctx, s, P, T, d_pid.data_ptr(), d_es.data_ptr(),
+                                            d_new_pos.data_ptr(), None if d_new_q is None else d_new_q.data_ptr(),
+                                            d_dE.data_ptr(), prec), "kmc_propose")
+
+    def step_device(self, d_pid, d_es, d_new_pos, d_dE, d_uniform, kT, d_counter, d_selected, precision=None,
+                    stream=None):
+        P = d_pid.numel()
+        T = d_new_pos.numel() // 3
+        prec = _native.PREC[precision or self.precision]
+        s = stream if stream is not None else self._stream()
+        nsteps = d_uniform.numel()
+        assert d_selected.numel() >= nsteps
+        _native.check(self._lib.kmc_step(self._ctx, s, P, T, d_pid.data_ptr(), d_es.data_ptr(), d_new_pos.data_ptr(),
+                                         d_dE.data_ptr(), nsteps, d_uniform.data_ptr(), float(kT),
+                                         d_counter.data_ptr(), d_selected.data_ptr(), prec), "kmc_step")
+
+    def materialize_moves(self, P, k, d_pids_all, d_disp_all, d_counter, d_pid, d_es, d_new_pos, stream=None):
+        s = stream if stream is not None else self._stream()
+        nsteps = d_pids_all.numel() // int(P)
+        assert d_disp_all.numel() >= nsteps * int(P) * int(k) * 3
+        _native.check(self._lib.kmc_materialize_moves(self._ctx, s, nsteps, int(P), int(k), d_pids_all.data_ptr(),
+                                                      d_disp_all.data_ptr(), d_counter.data_ptr(), d_pid.data_ptr(),
+                                                      d_es.data_ptr(), d_new_pos.data_ptr()), "materialize")
+
+    def check_flags(self):
+        f = ctypes.c_int32(0)
+        _native.check(self._lib.kmc_check_flags(self._ctx, self._stream(), ctypes.byref(f)), "check_flags")
+        return f.value
+
+    def sweep_hop(self, q, old_pos, new_pos, stream=None):
+        o = np.ascontiguousarray(old_pos, dtype=np.float64)
+        n = np.ascontiguousarray(new_pos, dtype=np.float64)
+        s = stream if stream is not None else self._stream()
+        _native.check(self._lib.kmc_sweep_hop(self._ctx, s, float(q), o.ctypes.data_as(ctypes.c_void_p),
+                                              n.ctypes.data_as(ctypes.c_void_p)), "sweep_hop")

@@ -1,0 +1,174 @@
+/*
+ * kmc_amd.h - C ABI of the MI355X (gfx950) FMM-KMC propose/accept library.
+ *
+ * One opaque context per system.  All functions return 0 on success and a
+ * non-zero status otherwise (KMC_E_*); kmc_last_error() gives a message.
+ * Pointers named d_* are device pointers (HBM), h_* are host pointers.  The
+ * stream argument is a hipStream_t passed as void* (NULL = default stream).
+ * A context is used from one host thread at a time.
+ *
+ * Each entry point cites the reference interface it replaces (paths relative to
+ * the ppmd/coulomb_kmc tree).
+ */
+#ifndef KMC_AMD_H
+#define KMC_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* boundary conditions: coulomb_kmc/common.py:36-49 (BCType) */
+#define KMC_BC_FREE_SPACE 0
+#define KMC_BC_NEAREST    1   /* '27' */
+#define KMC_BC_PBC        2
+#define KMC_BC_FF_ONLY    3
+
+/* arithmetic of the propose kernels */
+#define KMC_PREC_FP64  0      /* all FP64, library sqrt/division            */
+#define KMC_PREC_MIXED 1      /* FP32 reciprocal-sqrt seeds, one FP64 Newton */
+
+/* status codes */
+#define KMC_OK            0
+#define KMC_E_ARG         1   /* bad argument / shape                       */
+#define KMC_E_HIP         2   /* HIP runtime error                          */
+#define KMC_E_STATE       3   /* call out of order (e.g. before initialise) */
+#define KMC_E_BAD_MOVE    4   /* proposal outside domain or beyond max_move */
+#define KMC_E_STORE       5   /* particle store inconsistent / overflow     */
+#define KMC_E_UNSUPPORTED 6   /* expansion order or option not built        */
+
+typedef struct kmc_ctx kmc_ctx;
+
+const char* kmc_last_error(void);
+
+/* Smallest and largest expansion order L compiled into the library. */
+int32_t kmc_min_order(void);
+int32_t kmc_max_order(void);
+
+/*
+ * KMCFMM.__init__  (coulomb_kmc/kmc_fmm.py:436-542).
+ * n charges in a cubic box of side `extent` centred on the origin, r = number of
+ * FMM levels (leaf grid 2^(r-1) per side, coulomb_kmc/kmc_fmm.py:225), L = number
+ * of expansion terms, bc = KMC_BC_*, max_move as the reference (<=0: extent),
+ * energy_unit multiplies every energy.  occ_headroom: extra particle slots per
+ * cell above the initial maximum occupancy (the store grows when needed).
+ */
+int kmc_create(kmc_ctx** out, int64_t n, double extent, int32_t r, int32_t L, int32_t bc,
+               double max_move, double energy_unit, int32_t occ_headroom);
+int kmc_destroy(kmc_ctx* ctx);
+
+/*
+ * Positions (n x 3) and charges (n) of the charges; the index is the charge id.
+ * Bins the charges into leaf cells and builds the cell occupancy / particle tables
+ * (coulomb_kmc/kmc_local.py:582-824, single rank).
+ */
+int kmc_set_particles(kmc_ctx* ctx, const double* h_pos, const double* h_q);
+
+/*
+ * Periodic far-field operator for bc = PBC / FF_ONLY: a symmetric sparse matrix on
+ * the packed (l, m>=0) (re, im) multipole vector (2*L*(L+1)/2 rows, CSR) such that
+ * E_lr(M) = 0.5 * M . (G M).  Replaces ppmd's LongRangeMTL operator and dipole
+ * correction (coulomb_kmc/kmc_full_long_range.py:40, 516-560, 641-692).
+ */
+int kmc_set_lr_operator(kmc_ctx* ctx, int64_t nnz, const double* h_data, const int64_t* h_indptr,
+                        const int64_t* h_indices);
+
+/*
+ * KMCFMM.initialise (coulomb_kmc/kmc_fmm.py:947-993): builds every leaf local
+ * expansion by direct P2L of all charges (and their 26 images for '27'/PBC) on
+ * the GPU, the level-0 multipole M, and the energy.  *h_energy receives the energy.
+ */
+int kmc_initialise(kmc_ctx* ctx, void* stream, double* h_energy);
+
+/* Leaf expansions in the reference layout [ncells][2*L*L] (re block then im block,
+ * index l*l+l+m; coulomb_kmc/kmc_octal.py:61-62). */
+int kmc_get_expansions(kmc_ctx* ctx, double* h_out);
+int kmc_set_expansions(kmc_ctx* ctx, const double* h_in);
+/* M (= E) in the reference layout [2*L*L] (coulomb_kmc/kmc_full_long_range.py:42-43). */
+int kmc_get_multipole(kmc_ctx* ctx, double* h_out);
+/* Cell tables: occ[ncells], ids[ncells*occmax], pos4[ncells*occmax*4]; *occmax out. */
+int kmc_get_store(kmc_ctx* ctx, int32_t* h_occ, int32_t* h_ids, double* h_pos4, int32_t* h_occmax);
+int kmc_store_occmax(kmc_ctx* ctx, int32_t* out);
+int kmc_get_particles(kmc_ctx* ctx, double* h_pos, int32_t* h_cells);
+int kmc_get_energy(kmc_ctx* ctx, double* h_energy);
+int kmc_set_energy(kmc_ctx* ctx, double energy);
+int kmc_ncells(kmc_ctx* ctx, int64_t* out);
+
+/*
+ * Proposal energies, tuple form (KMCFMM.propose -> _propose, coulomb_kmc/kmc_fmm.py:
+ * 774-787, 1001-1105, and the diff_lib combine :583-663).  P moving charges d_pid[P];
+ * their proposals are d_new_pos[d_es[p] .. d_es[p+1]) (T = es[P] rows of 3 doubles);
+ * d_new_q optional new charges (NULL: unchanged).  Writes d_dE[T] = energy_unit *
+ * (U1D + U1I - U0D - U0I - USI).  T must equal es[P].  Returns KMC_E_BAD_MOVE when a
+ * proposal lies outside the domain or beyond max_move (kmc_mpi_decomp.py:312-345, 730).
+ */
+int kmc_propose(kmc_ctx* ctx, void* stream, int64_t P, int64_t T, const int64_t* d_pid,
+                const int64_t* d_es, const double* d_new_pos, const double* d_new_q, double* d_dE,
+                int32_t precision);
+
+/*
+ * KMCFMM.propose_with_dats (coulomb_kmc/kmc_fmm.py:665-762 with setup_move,
+ * coulomb_kmc/kmc_mpi_decomp.py:501-616).  Device arrays: site_max_counts[nsites],
+ * current_sites[n], prop_positions[n][max_prop][3], prop_masks[n][max_prop],
+ * prop_charges[n][max_prop] or NULL; writes prop_energy_diffs[n][max_prop] at every
+ * masked slot taken (mask > 0, at most site_max_counts[site] per charge).
+ * *h_total_moves (optional) receives the number of proposals evaluated.
+ */
+int kmc_propose_with_dats(kmc_ctx* ctx, void* stream, int64_t max_prop, const int64_t* d_site_max_counts,
+                          const int64_t* d_current_sites, const double* d_prop_positions,
+                          const int64_t* d_prop_masks, const double* d_prop_charges,
+                          double* d_prop_energy_diffs, int32_t precision, int64_t* h_total_moves);
+
+/*
+ * KMCFMM.accept (coulomb_kmc/kmc_fmm.py:789-912): moves charge `pid` to new_pos.
+ * Computes the new energy with a one-proposal FP64 propose, updates the cell tables
+ * (extract/insert, coulomb_kmc/kmc_local.py:180-370), sweeps every leaf local
+ * expansion (accept_local_exp, coulomb_kmc/kmc_octal.py:977-1227) and the level-0
+ * multipole (coulomb_kmc/kmc_full_long_range.py:270-316).  *h_energy (optional)
+ * receives the new energy.
+ */
+int kmc_accept(kmc_ctx* ctx, void* stream, int64_t pid, const double* h_new_pos, double* h_energy);
+
+/*
+ * One whole KMC step on the device, no host synchronisation (graph capturable
+ * once the context's work buffers have been sized by a first call):
+ * proposals T from d_pid/d_es/d_new_pos as kmc_propose, then Metropolis selection
+ * with rates min(1, exp(-dE/kT)) and cumulative sum in proposal order against
+ * uniform d_uniform[*d_counter % nsteps] (the example harness's rule,
+ * examples/propose_with_dats_example.py:316-411), then the accept of the selected
+ * proposal.  d_selected[*d_counter % nsteps] receives the selected proposal index
+ * and *d_counter is incremented.  d_dE (T) receives the proposal energies.
+ */
+int kmc_step(kmc_ctx* ctx, void* stream, int64_t P, int64_t T, const int64_t* d_pid, const int64_t* d_es,
+             const double* d_new_pos, double* d_dE, int64_t nsteps, const double* d_uniform, double kT,
+             int64_t* d_counter, int64_t* d_selected, int32_t precision);
+
+/*
+ * Synthetic move stream for benchmarks and tests (no reference counterpart: the
+ * reference's harness builds proposals on the host).  For step = *d_counter % nsteps:
+ * pid = d_pids_all[step*P + p], new_pos[t] = wrap/clip(pos[pid] + d_disp_all[step*T + t])
+ * with t in p*k .. p*k+k-1.  Fills d_pid[P], d_es[P+1] (k per charge), d_new_pos[T].
+ * d_pids_all holds nsteps*P ids, d_disp_all nsteps*T*3 doubles.
+ */
+int kmc_materialize_moves(kmc_ctx* ctx, void* stream, int64_t nsteps, int64_t P, int64_t k,
+                          const int64_t* d_pids_all, const double* d_disp_all, const int64_t* d_counter,
+                          int64_t* d_pid, int64_t* d_es, double* d_new_pos);
+
+/* Flags raised by device-side steps (store overflow, bad moves); cleared by the call. */
+int kmc_check_flags(kmc_ctx* ctx, void* stream, int32_t* h_flags);
+
+/*
+ * Tile choices of the hot kernels (filled by the autotuner in coulomb_kmc_amd/tune.py):
+ * near_lanes = lanes per evaluation point in the near-field kernel (1,4,8,16,32,64),
+ * sweep_block = threads per block of the accept sweep (64..1024, multiple of 64).
+ */
+int kmc_set_tiles(kmc_ctx* ctx, int32_t near_lanes, int32_t sweep_block);
+
+/* Accept sweep only, for a hop of charge q from h_old to h_new (timing / tests). */
+int kmc_sweep_hop(kmc_ctx* ctx, void* stream, double q, const double* h_old_pos, const double* h_new_pos);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,273 @@
+"""
+GPU parity: the HIP path (through the C ABI via KMCFMM) against the oracle
+(oracle/kmc_oracle.cpp, the CPU restatement of the reference) on the same seeded
+inputs.  Bars: FP64 dE within 1e-10 relative, mixed within 1e-8 relative, cell
+tables and accepted-move sequences bit-exact, expansions after accepts within
+1e-11 of their scale.
+"""
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import neutral_system, ball_moves, rel_err, metropolis_pick
+
+pytestmark = pytest.mark.gpu
+
+BCS = ("free_space", "27", "pbc", "ff-only")
+
+
+def make_pair(bc, N=300, E=6.0, r=4, L=12, max_move=1.0, seed=11):
+    from coulomb_kmc_amd import KMCFMM
+    from coulomb_kmc_amd.kmc_fmm import lr_operator_reference_layout
+    rng = np.random.default_rng(seed)
+    pos, q = neutral_system(rng, N, E)
+    kmc = KMCFMM(pos.copy(), q.reshape(-1, 1), E, r=r, l=L, boundary_condition=bc, max_move=max_move)
+    kmc.initialise()
+    lr = lr_operator_reference_layout(L, E) if bc in ("pbc", "ff-only") else None
+    orc = O.Oracle(pos, q, E, r, L, bc=bc, max_move=max_move, lr=lr)
+    orc.initialise()
+    return kmc, orc, pos, q, rng
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+@pytest.mark.parametrize("bc", BCS)
+@pytest.mark.parametrize("L", (8, 12))
+def test_initialise_matches_oracle(bc, L):
+    kmc, orc, pos, q, rng = make_pair(bc, L=L)
+    _print("init", bc, L, kmc.energy, orc.energy)
+    assert abs(kmc.energy - orc.energy) <= 1e-11 * max(1.0, abs(orc.energy))
+    if bc != "ff-only":
+        eg, ec = kmc.get_expansions(), orc.get_expansions()
+        assert np.max(np.abs(eg - ec)) <= 1e-12 * np.max(np.abs(ec))
+    if bc in ("pbc", "ff-only"):
+        Mg = kmc.get_multipole()
+        Mc, Ec = orc.get_ME()
+        assert np.max(np.abs(Mg - Mc)) <= 1e-12 * np.max(np.abs(Mc))
+        assert np.max(np.abs(Mg - Ec)) <= 1e-12 * np.max(np.abs(Ec))
+    kmc.free()
+
+
+@pytest.mark.parametrize("bc", BCS)
+@pytest.mark.parametrize("prec", ("fp64", "mixed"))
+def test_propose_matches_oracle(bc, prec):
+    kmc, orc, pos, q, rng = make_pair(bc)
+    kmc.precision = prec
+    pids = rng.choice(len(q), 40, replace=False)
+    moves = ball_moves(rng, pos, pids, 4, 1.0, 6.0, bc)
+    g = np.concatenate(kmc.propose(moves)) - kmc.energy
+    c = np.concatenate(orc.propose(moves)) - orc.energy
+    err = rel_err(g, c)
+    _print("propose", bc, prec, "max rel", err.max(), "median", np.median(err))
+    assert err.max() <= (1e-10 if prec == "fp64" else 1e-8)
+    kmc.free()
+
+
+@pytest.mark.parametrize("L", (4, 10, 16, 20))
+def test_propose_orders(L):
+    kmc, orc, pos, q, rng = make_pair("pbc", N=200, E=5.0, r=3, L=L)
+    pids = rng.choice(len(q), 20, replace=False)
+    moves = ball_moves(rng, pos, pids, 3, 1.0, 5.0, "pbc")
+    g = np.concatenate(kmc.propose(moves)) - kmc.energy
+    c = np.concatenate(orc.propose(moves)) - orc.energy
+    err = rel_err(g, c)
+    _print("orders", L, err.max())
+    assert err.max() <= 1e-10
+    kmc.free()
+
+
+@pytest.mark.parametrize("bc", ("free_space", "pbc"))
+def test_propose_with_dats_matches_oracle(bc):
+    kmc, orc, pos, q, rng = make_pair(bc)
+    N, M = len(q), 6
+    site_max = np.array([2, 4, 6], dtype=np.int64)
+    sites = rng.integers(0, 3, size=(N, 1)).astype(np.int64)
+    masks = (rng.uniform(size=(N, M)) < 0.3).astype(np.int64)
+    ppos = np.zeros((N, 3 * M))
+    for i in range(N):
+        mv = ball_moves(rng, pos, [i], M, 1.0, 6.0, bc)[0][1]
+        ppos[i] = mv.reshape(-1)
+    dg = np.full((N, M), -7.0)
+    dc = np.full((N, M), -7.0)
+    kmc.propose_with_dats(site_max, sites, ppos, masks, dg)
+    orc.propose_with_dats(site_max, sites, ppos, masks, dc)
+    touched = dc != -7.0
+    assert np.array_equal(touched, dg != -7.0)
+    err = rel_err(dg[touched], dc[touched])
+    _print("dats", bc, touched.sum(), err.max())
+    assert err.max() <= 1e-10
+    # propose_with_dats == propose on the same moves (tests/host/test_propose_dat_setup.py:447-)
+    moves = []
+    for i in range(N):
+        taken = np.nonzero(masks[i])[0][: site_max[sites[i, 0]]]
+        if len(taken):
+            moves.append((i, ppos[i].reshape(M, 3)[taken]))
+    e = kmc.propose(tuple(moves))
+    for (i, mv), vals in zip(moves, e):
+        taken = np.nonzero(masks[i])[0][: site_max[sites[i, 0]]]
+        assert np.allclose(vals - kmc.energy, dg[i, taken], rtol=1e-12, atol=1e-12)
+    kmc.free()
+
+
+def _store_equal(kmc, orc):
+    occ_g, ids_g, p4_g = kmc.get_store()
+    occ_c, ids_c, data_c = orc.get_store()
+    assert np.array_equal(occ_g.astype(np.int64), occ_c)
+    for c in np.nonzero(occ_c)[0]:
+        n = occ_c[c]
+        assert np.array_equal(ids_g[c, :n].astype(np.int64), ids_c[c, :n])
+        assert np.array_equal(p4_g[c, :n, :3], data_c[c, :n, :3])
+        assert np.array_equal(p4_g[c, :n, 3], data_c[c, :n, 3])
+
+
+@pytest.mark.parametrize("bc", BCS)
+def test_accept_sequence_matches_oracle(bc):
+    kmc, orc, pos, q, rng = make_pair(bc, N=240, E=6.0, r=4, L=10)
+    picks_g, picks_c = [], []
+    for step in range(12):
+        pg = kmc._pos()
+        pids = rng.choice(len(q), 16, replace=False)
+        moves = ball_moves(rng, pg, pids, 4, 1.0, 6.0, bc)
+        dg = np.concatenate(kmc.propose(moves)) - kmc.energy
+        dc = np.concatenate(orc.propose(moves)) - orc.energy
+        u = rng.uniform()
+        kg, kc = metropolis_pick(dg, u), metropolis_pick(dc, u)
+        picks_g.append(kg)
+        picks_c.append(kc)
+        flat = [(pid, p) for pid, mv in moves for p in mv]
+        pid, npos = flat[kg]
+        kmc.accept((pid, npos))
+        orc.accept(pid, npos)
+        assert abs(kmc.energy - orc.energy) <= 1e-10 * max(1.0, abs(orc.energy))
+    assert picks_g == picks_c
+    pos_g, cells_g = kmc.get_device_particles()
+    assert np.array_equal(pos_g, orc.positions())
+    assert np.array_equal(cells_g.astype(np.int64), orc.cells())
+    assert np.array_equal(kmc._pos(), orc.positions())
+    if bc != "ff-only":
+        _store_equal(kmc, orc)
+        eg, ec = kmc.get_expansions(), orc.get_expansions()
+        err = np.max(np.abs(eg - ec)) / np.max(np.abs(ec))
+        _print("accept", bc, "expansion err", err)
+        assert err <= 1e-11
+    if bc in ("pbc", "ff-only"):
+        Mg = kmc.get_multipole()
+        Mc, _ = orc.get_ME()
+        assert np.max(np.abs(Mg - Mc)) <= 1e-12 * np.max(np.abs(Mc))
+    kmc.free()
+
+
+def test_accept_equals_reinitialise():
+    """Sweep after a move == fresh initialise of the moved system (direct P2L, exact)."""
+    from coulomb_kmc_amd import KMCFMM
+    kmc, orc, pos, q, rng = make_pair("pbc", N=200, E=5.0, r=4, L=12)
+    pid = 17
+    newp = kmc._pos()[pid] + np.array([0.3, -0.2, 0.25])
+    kmc.accept((pid, newp))
+    pos2 = pos.copy()
+    pos2[pid] = newp
+    k2 = KMCFMM(pos2.copy(), q.reshape(-1, 1), 5.0, r=4, l=12, boundary_condition="pbc", max_move=1.0)
+    k2.initialise()
+    a, b = kmc.get_expansions(), k2.get_expansions()
+    assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b))
+    # accumulated energy vs fresh state energy: equal up to expansion truncation
+    assert abs(kmc.energy - k2.energy) <= 1e-4 * abs(k2.energy)
+    kmc.free()
+    k2.free()
+
+
+def test_device_step_matches_host_path():
+    """kmc_step (select + accept on the device) == host propose/select/accept."""
+    import torch
+    from coulomb_kmc_amd import KMCFMM
+    bc, N, E, r, L = "pbc", 300, 6.0, 4, 12
+    rng = np.random.default_rng(5)
+    pos, q = neutral_system(rng, N, E)
+    a = KMCFMM(pos.copy(), q.reshape(-1, 1), E, r=r, l=L, boundary_condition=bc, max_move=1.0)
+    b = KMCFMM(pos.copy(), q.reshape(-1, 1), E, r=r, l=L, boundary_condition=bc, max_move=1.0)
+    a.initialise()
+    b.initialise()
+    steps, P, k = 10, 24, 4
+    T = P * k
+    pids_all = np.stack([rng.choice(N, P, replace=False) for _ in range(steps)]).astype(np.int64)
+    d = rng.normal(size=(steps, T, 3))
+    d /= np.linalg.norm(d, axis=2, keepdims=True)
+    d *= rng.uniform(size=(steps, T, 1)) ** (1.0 / 3.0)
+    us = rng.uniform(size=steps)
+    dev = lambda x, t: torch.as_tensor(x, dtype=t).cuda()
+    t_pids, t_disp, t_u = dev(pids_all, torch.int64), dev(d, torch.float64), dev(us, torch.float64)
+    counter = torch.zeros(1, dtype=torch.int64, device="cuda")
+    sel = torch.full((steps,), -1, dtype=torch.int64, device="cuda")
+    d_pid = torch.zeros(P, dtype=torch.int64, device="cuda")
+    d_es = torch.zeros(P + 1, dtype=torch.int64, device="cuda")
+    d_np = torch.zeros(T * 3, dtype=torch.float64, device="cuda")
+    d_dE = torch.zeros(T, dtype=torch.float64, device="cuda")
+    a.materialize_moves(P, k, t_pids, t_disp, counter, d_pid, d_es, d_np)
+    a.propose_device(d_pid, d_es, d_np, d_dE)    # sizes the work buffers
+    for s in range(steps):
+        a.materialize_moves(P, k, t_pids, t_disp, counter, d_pid, d_es, d_np)
+        a.step_device(d_pid, d_es, d_np, d_dE, t_u, 1.0, counter, sel)
+        # host path on b
+        pb = b._pos()
+        moves = []
+        for j in range(P):
+            pid = int(pids_all[s, j])
+            npos = pb[pid] + d[s, j * k:(j + 1) * k]
+            npos = np.where(npos < -0.5 * E, npos + E, npos)
+            npos = np.where(npos >= 0.5 * E, npos - E, npos)
+            moves.append((pid, npos))
+        db = np.concatenate(b.propose(tuple(moves))) - b.energy
+        kb = metropolis_pick(db, us[s])
+        flat = [(pid, p) for pid, mv in moves for p in mv]
+        b.accept(flat[kb])
+        torch.cuda.synchronize()
+        assert int(sel[s].item()) == kb, (s, int(sel[s].item()), kb)
+    assert a.check_flags() == 0
+    assert abs(a.device_energy() - b.energy) <= 1e-10 * abs(b.energy)
+    pa, ca = a.get_device_particles()
+    pb, cb = b.get_device_particles()
+    assert np.array_equal(pa, pb) and np.array_equal(ca, cb)
+    assert np.max(np.abs(a.get_expansions() - b.get_expansions())) == 0.0
+    a.free()
+    b.free()
+
+
+def test_free_space_against_direct_sum():
+    """dE vs brute-force Coulomb differences at the reference's own tolerance
+    (tests/host/test_free_space_1.py:194-230: 1e-5 at l=12)."""
+    from ewald import free_space_energy
+    kmc, orc, pos, q, rng = make_pair("free_space", N=200, E=4.0, r=3, L=12, max_move=4.0)
+    e0 = free_space_energy(pos, q)
+    assert abs(kmc.energy - e0) <= 1e-5 * abs(e0)
+    for _ in range(5):
+        pid = int(rng.integers(0, len(q)))
+        newp = rng.uniform(-2.0, 2.0, (3, 3))
+        vals = kmc.propose(((pid, newp),))[0]
+        for j in range(3):
+            p2 = pos.copy()
+            p2[pid] = newp[j]
+            ref = free_space_energy(p2, q)
+            assert abs(vals[j] - ref) <= 1e-5 * abs(ref)
+    kmc.free()
+
+
+def test_bad_move_raises():
+    kmc, orc, pos, q, rng = make_pair("pbc")
+    with pytest.raises(RuntimeError):
+        kmc.propose(((0, np.array([[10.0, 0.0, 0.0]])),))
+    with pytest.raises(RuntimeError):
+        kmc.propose(((0, (pos[0] + np.array([1.5, 0, 0]))[None, :]),))
+    kmc.free()
+
+
+def test_use_before_initialise_raises():
+    from coulomb_kmc_amd import KMCFMM
+    rng = np.random.default_rng(0)
+    pos, q = neutral_system(rng, 50, 4.0)
+    kmc = KMCFMM(pos, q, 4.0, r=3, l=8, boundary_condition="free_space")
+    with pytest.raises(RuntimeError):
+        kmc.propose(((0, pos[0:1]),))
+    kmc.free()
