@@ -1,0 +1,341 @@
+#!/usr/bin/env python3
+"""
+FMM-KMC propose/accept throughput on one MI355X.
+
+A step is one pass of the hot path over one batch of synthetic proposals: P charges
+each propose k moves (displacement uniform in a ball, wrapped into the periodic box
+or clipped to a free-space box), the energy differences of all T = P*k proposals
+are computed, one proposal is selected by Metropolis (rates min(1, exp(-dE/kT)),
+cumulative sum in proposal order, seeded uniform) and accepted (cell tables, every
+leaf local expansion, the level-0 multipole).  Each step runs entirely on the
+device; the steps are replayed from a captured HIP graph.
+
+Prints one JSON line (see the "value" field: mixed-precision proposals/s of the
+whole step; accepts/s, the FP64 path, the dE error of mixed vs FP64, the roofline of
+the dominant kernel and the host-CPU baseline are in the same line).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1|0|4]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "ΔE proposals/s + accepts/s on 1 GPU, mixed vs FP64, at ΔE rel err ≤1e-8"
+
+# leaf grids as named by the workload (r = levels, leaf grid 2^(r-1) per side)
+CONFIGS = {
+    0: dict(workload="free-space N=1000 box 10 leaves 8^3 L=12, 64x4 proposals", N=1000, E=10.0, r=4, L=12,
+            bc="free_space", P=64, k=4, radius=1.0, seed=1000),
+    1: dict(workload="pbc N=100000 box 100 leaves 32^3 L=12, 1024x8 proposals", N=100000, E=100.0, r=6, L=12,
+            bc="pbc", P=1024, k=8, radius=1.5, seed=1001),
+    4: dict(workload="pbc N=2000000 box 271.4 leaves 64^3 L=16, 32768x8 proposals", N=2000000, E=271.4, r=7,
+            L=16, bc="pbc", P=32768, k=8, radius=1.5, seed=1004),
+}
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def sweep_flops_per_source(L):
+    """FP64 flops of one P2L of one source into one leaf in k_sweep: 9 per packed
+    coefficient (recurrence 5, complex accumulate 4), the power chain 6 per order,
+    30 for displacement, 1/r and set-up."""
+    return 9 * L * (L + 1) // 2 + 6 * (L - 1) + 30
+
+
+def make_system(cfg):
+    rng = np.random.default_rng(cfg["seed"])
+    N, E = cfg["N"], cfg["E"]
+    pos = rng.uniform(-0.5 * E, 0.5 * E, (N, 3))
+    q = np.ones(N)
+    q[: N // 2] = -1.0
+    rng.shuffle(q)
+    return pos, q, rng
+
+
+def move_stream(rng, cfg, nsteps):
+    N, P, k, rad = cfg["N"], cfg["P"], cfg["k"], cfg["radius"]
+    T = P * k
+    pids = np.empty((nsteps, P), np.int64)
+    for s in range(nsteps):
+        pids[s] = rng.choice(N, size=P, replace=False)
+    d = rng.normal(size=(nsteps, T, 3))
+    d /= np.linalg.norm(d, axis=2, keepdims=True)
+    d *= rad * rng.uniform(size=(nsteps, T, 1)) ** (1.0 / 3.0)
+    u = rng.uniform(size=nsteps)
+    return pids, d, u
+
+
+def host_moves(pos, pids_s, disp_s, k, E, bc):
+    moves = []
+    for j, pid in enumerate(pids_s):
+        npos = pos[pid] + disp_s[j * k:(j + 1) * k]
+        if bc == "free_space":
+            npos = np.clip(npos, -0.5 * E, 0.5 * E)
+        else:
+            npos = np.where(npos < -0.5 * E, npos + E, npos)
+            npos = np.where(npos >= 0.5 * E, npos - E, npos)
+        moves.append((int(pid), npos))
+    return moves
+
+
+def metropolis_pick(dE, u, kT=1.0):
+    w = np.where(np.isfinite(dE), np.minimum(1.0, np.exp(-dE / kT)), 0.0)
+    c = np.cumsum(w)
+    return min(int(np.searchsorted(c, u * c[-1], side="right")), len(dE) - 1)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--config", type=int, default=1, choices=sorted(CONFIGS))
+    ap.add_argument("--graph-steps", type=int, default=10, help="steps captured per HIP graph")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--near-lanes", type=int, default=0, help="0: autotuned")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (one MI355X per box); --gpus {} is not supported"
+              .format(args.gpus), file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from coulomb_kmc_amd import KMCFMM
+    from coulomb_kmc_amd import tune
+
+    cfg = CONFIGS[args.config]
+    N, E, r, L, bc, P, k = cfg["N"], cfg["E"], cfg["r"], cfg["L"], cfg["bc"], cfg["P"], cfg["k"]
+    T = P * k
+    G = max(1, args.graph_steps)
+    steps = max(G, (args.steps // G) * G)
+    warm = max(G, (args.warmup // G) * G)
+    nstream = warm + 2 * steps + 16
+    log("config", args.config, cfg["workload"], "steps", steps, "warmup", warm)
+
+    pos0, q, rng = make_system(cfg)
+    t0 = time.time()
+    pids_all, disp_all, us = move_stream(rng, cfg, nstream)
+    log("move stream", round(time.time() - t0, 2), "s")
+
+    kmc = KMCFMM(pos0.copy(), q.reshape(-1, 1), E, r=r, l=L, boundary_condition=bc, max_move=cfg["radius"])
+    tiles = tune.choose_tiles(L, T, near_lanes=args.near_lanes or None)
+    kmc.set_tiles(**tiles)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    kmc.initialise()
+    torch.cuda.synchronize()
+    t_init = time.time() - t0
+    e_init = kmc.energy
+    log("initialise", round(t_init, 2), "s, energy", e_init)
+
+    dev = lambda x, t: torch.as_tensor(np.ascontiguousarray(x), dtype=t).cuda()
+    t_pids = dev(pids_all, torch.int64)
+    t_disp = dev(disp_all, torch.float64)
+    t_u = dev(us, torch.float64)
+    counter = torch.zeros(1, dtype=torch.int64, device="cuda")
+    sel = torch.full((nstream,), -1, dtype=torch.int64, device="cuda")
+    d_pid = torch.zeros(P, dtype=torch.int64, device="cuda")
+    d_es = torch.zeros(P + 1, dtype=torch.int64, device="cuda")
+    d_np = torch.zeros(T * 3, dtype=torch.float64, device="cuda")
+    d_dE = torch.zeros(T, dtype=torch.float64, device="cuda")
+    d_dE2 = torch.zeros(T, dtype=torch.float64, device="cuda")
+
+    # ---- dE error of mixed vs FP64 on the initial state, 65536 proposals (or fewer)
+    nerr = max(1, 65536 // T)
+    errs_abs, errs_rel, ref_vals = [], [], []
+    for b in range(nerr):
+        counter.fill_(nstream - 1 - b)
+        kmc.materialize_moves(P, k, t_pids, t_disp, counter, d_pid, d_es, d_np)
+        kmc.propose_device(d_pid, d_es, d_np, d_dE, precision="fp64")
+        kmc.propose_device(d_pid, d_es, d_np, d_dE2, precision="mixed")
+        a, m = d_dE.cpu().numpy(), d_dE2.cpu().numpy()
+        errs_abs.append(np.abs(m - a))
+        ref_vals.append(np.abs(a))
+    ea = np.concatenate(errs_abs)
+    rv = np.concatenate(ref_vals)
+    er = ea / np.maximum(rv, 1e-300)
+    big = rv >= 1e-6 * np.median(rv)
+    derr = {"proposals": int(ea.size), "max_abs": float(ea.max()), "median_abs": float(np.median(ea)),
+            "max_rel": float(er[big].max()), "median_rel": float(np.median(er[big])),
+            "rel_floor": "proposals with |dE| >= 1e-6 x median |dE| ({} of {})".format(int(big.sum()), ea.size)}
+    log("mixed vs fp64 dE error", derr)
+
+    # ---- CPU baseline (oracle, same state and move stream), bounded sample
+    cpu = None
+    cpu_parity = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        from coulomb_kmc_amd.kmc_fmm import lr_operator_reference_layout
+        nth = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        O.set_threads(nth)
+        orc = O.Oracle(pos0, q, E, r, L, bc=bc, max_move=cfg["radius"],
+                       lr=lr_operator_reference_layout(L, E) if bc in ("pbc", "ff-only") else None)
+        orc.init_from_expansions(kmc.get_expansions(), kmc.energy)
+        t_cpu, n_cpu, cmax = 0.0, 0, 0.0
+        gpos = pos0.copy()
+        picks = []
+        while n_cpu < 1 or (t_cpu < args.cpu_seconds and n_cpu < 50):
+            s = n_cpu
+            moves = host_moves(orc.positions(), pids_all[s], disp_all[s], k, E, bc)
+            es = np.arange(P + 1, dtype=np.int64) * k
+            npos = np.concatenate([m[1] for m in moves])
+            tc = time.time()
+            dc = orc.propose_arrays(pids_all[s], es, npos)
+            kc = metropolis_pick(dc, us[s])
+            pid, p_new = moves[kc // k][0], moves[kc // k][1][kc % k]
+            orc.accept(pid, p_new)
+            t_cpu += time.time() - tc
+            picks.append(kc)
+            if s == 0:
+                counter.fill_(0)
+                kmc.materialize_moves(P, k, t_pids, t_disp, counter, d_pid, d_es, d_np)
+                kmc.propose_device(d_pid, d_es, d_np, d_dE, precision="fp64")
+                g = d_dE.cpu().numpy()
+                cmax = float(np.max(np.abs(g - dc) / np.maximum(np.abs(dc), 1e-3)))
+            n_cpu += 1
+        cpu = {"value": n_cpu * T / t_cpu, "unit": "proposals/s", "cores": nth, "kind": "port",
+               "accepts_per_s": n_cpu / t_cpu,
+               "sample": "{} steps ({} proposals + 1 accept each) of this workload from the same state and move "
+                         "stream, {:.1f} s, oracle/kmc_oracle.cpp (FP64, OpenMP, reference data structures)"
+                         .format(n_cpu, T, t_cpu)}
+        cpu_parity = {"fp64_dE_max_rel_err_step0": cmax, "cpu_picks": picks}
+        log("cpu baseline", cpu)
+
+    # ---- timed runs: graph of G steps, replayed
+    stream = torch.cuda.Stream()
+    results = {}
+    for prec in ("fp64", "mixed"):
+        counter.fill_(0)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(stream):
+            kmc.materialize_moves(P, k, t_pids, t_disp, counter, d_pid, d_es, d_np)
+            kmc.propose_device(d_pid, d_es, d_np, d_dE, precision=prec)   # sizes work buffers
+        stream.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=stream):
+            for _ in range(G):
+                kmc.materialize_moves(P, k, t_pids, t_disp, counter, d_pid, d_es, d_np)
+                kmc.step_device(d_pid, d_es, d_np, d_dE, t_u, 1.0, counter, sel, precision=prec)
+        counter.fill_(0)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(stream):
+            for _ in range(warm // G):
+                graph.replay()
+        stream.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        tw0 = time.time()
+        with torch.cuda.stream(stream):
+            ev0.record(stream)
+            for _ in range(steps // G):
+                graph.replay()
+            ev1.record(stream)
+        ev1.synchronize()
+        wall = time.time() - tw0
+        ms = ev0.elapsed_time(ev1)
+        flags = kmc.check_flags()
+        results[prec] = {"ms_per_step": ms / steps, "proposals_per_s": T * steps / (ms * 1e-3),
+                         "accepts_per_s": steps / (ms * 1e-3), "wall_s": wall, "flags": flags,
+                         "energy": kmc.device_energy()}
+        log(prec, results[prec])
+        del graph
+
+    # ---- CPU/GPU accept-sequence agreement on the sampled steps (same state, same stream)
+    if cpu_parity is not None:
+        gpu_sel = []
+        kchk = KMCFMM(pos0.copy(), q.reshape(-1, 1), E, r=r, l=L, boundary_condition=bc, max_move=cfg["radius"])
+        kchk.set_tiles(**tiles)
+        kchk.initialise()
+        counter.fill_(0)
+        sel.fill_(-1)
+        for s in range(len(cpu_parity["cpu_picks"])):
+            kchk.materialize_moves(P, k, t_pids, t_disp, counter, d_pid, d_es, d_np)
+            if s == 0:
+                kchk.propose_device(d_pid, d_es, d_np, d_dE, precision="fp64")
+            kchk.step_device(d_pid, d_es, d_np, d_dE, t_u, 1.0, counter, sel, precision="fp64")
+        torch.cuda.synchronize()
+        gpu_sel = sel[:len(cpu_parity["cpu_picks"])].cpu().numpy().tolist()
+        cpu_parity["gpu_picks"] = gpu_sel
+        cpu_parity["accept_sequence_equal"] = gpu_sel == cpu_parity["cpu_picks"]
+        kchk.free()
+        log("accept sequence cpu == gpu:", cpu_parity["accept_sequence_equal"])
+
+    # ---- dominant kernel: the accept sweep, timed alone with HIP events (changes state: last)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(stream):
+        counter.fill_(0)
+        kmc.materialize_moves(P, k, t_pids, t_disp, counter, d_pid, d_es, d_np)
+        kmc.propose_device(d_pid, d_es, d_np, d_dE, precision="mixed", stream=torch.cuda.current_stream().cuda_stream)
+    stream.synchronize()
+    kt = {}
+    for name in ("near", "far", "lr"):
+        kt[name] = kmc.time_kernel(name, reps=20, precision="mixed")
+    kt["sweep"] = kmc.time_kernel("sweep", reps=20)
+    log("kernel ms", kt)
+    ncells = (2 ** (r - 1)) ** 3
+    nimg = 54 if bc != "free_space" else 2
+    sweep_flops = ncells * nimg * sweep_flops_per_source(L)
+    sweep_bytes = 2 * ncells * (L * (L + 1) // 2) * 16
+    ach_tflops = sweep_flops / (kt["sweep"] * 1e-3) / 1e12
+    ach_gbs = sweep_bytes / (kt["sweep"] * 1e-3) / 1e9
+    roof = {"bound": "valu", "kernel": "k_sweep (accept sweep)", "achieved": ach_tflops,
+            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach_tflops / FP64_PEAK_TFLOPS,
+            "traffic": None, "launch_ms": kt["sweep"], "algorithmic_flops": sweep_flops,
+            "algorithmic_bytes": sweep_bytes, "hbm_GBs": ach_gbs, "hbm_frac": ach_gbs / HBM_PEAK_GBS,
+            "note": "FP64 vector ALU bound (no dense contraction: MFMA unused); peak = MI355X FP64 "
+                    "vector rate; flops = leaves x 54 images x (9 L(L+1)/2 + 6(L-1) + 30)"}
+    traffic = tune.read_traffic(ROOT, "k_sweep", cfg)
+    if traffic is not None:
+        roof["traffic"] = traffic
+
+    mixed, fp64 = results["mixed"], results["fp64"]
+    out = {
+        "metric": METRIC,
+        "value": mixed["proposals_per_s"],
+        "unit": "proposals/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warm,
+        "ms_per_step": mixed["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded uniform charges +-1, ball-displacement proposals, seeded Metropolis uniforms)",
+        "config": {"workload": cfg["workload"], "N": N, "box": E, "leaves_per_side": 2 ** (r - 1), "r": r,
+                   "L": L, "bc": bc, "proposals_per_step": T, "moving_charges_per_step": P,
+                   "ball_radius": cfg["radius"], "kT": 1.0, "graph_steps": G, "tiles": tiles},
+        "accepts_per_s": mixed["accepts_per_s"],
+        "mixed": mixed,
+        "fp64": fp64,
+        "mixed_vs_fp64": mixed["proposals_per_s"] / fp64["proposals_per_s"],
+        "dE_error_mixed_vs_fp64": derr,
+        "kernel_ms": kt,
+        "initialise_s": t_init,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "gpu_vs_cpu": (mixed["proposals_per_s"] / cpu["value"]) if cpu else None,
+        "parity": cpu_parity,
+    }
+    print(json.dumps(out), flush=True)
+    kmc.free()
+
+
+if __name__ == "__main__":
+    main()

@@ -38,7 +38,7 @@ EXPORTS = (
     "kmc_set_expansions", "kmc_get_multipole", "kmc_get_store", "kmc_store_occmax",
     "kmc_get_particles", "kmc_get_energy", "kmc_set_energy", "kmc_ncells", "kmc_propose",
     "kmc_propose_with_dats", "kmc_accept", "kmc_step", "kmc_materialize_moves",
-    "kmc_check_flags", "kmc_set_tiles", "kmc_sweep_hop",
+    "kmc_check_flags", "kmc_set_tiles", "kmc_sweep_hop", "kmc_time_kernel",
 )
 
 
@@ -107,6 +107,7 @@ def lib():
         "kmc_check_flags": (ctypes.c_int, [P, P, P]),
         "kmc_set_tiles": (ctypes.c_int, [P, I32, I32]),
         "kmc_sweep_hop": (ctypes.c_int, [P, P, D, P, P]),
+        "kmc_time_kernel": (ctypes.c_int, [P, P, I32, I32, I32, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -1038,6 +1038,7 @@ struct kmc_ctx {
     int occmax = 0;
     int near_lanes = 8;
     int sweep_block = 256;
+    int64_t last_P = 0, last_T = 0;   // sizes of the last propose (kernel timing)
     bool have_particles = false, have_lr = false, initialised = false;
     OrderFns fns{};
 
@@ -1112,6 +1113,8 @@ static EvalArgs eval_args(kmc_ctx* c, int n, const double4* pos4, const int32_t*
 static int run_propose_core(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, double* d_out, const int64_t* rate_loc,
                             bool mixed) {
     const int ne = (int)(T + P);
+    c->last_P = P;
+    c->last_T = T;
     if (c->bc != KMC_BC_FF_ONLY) {
         EvalArgs an = eval_args(c, ne, c->epos4.p, c->ecell.p, c->eexcl.p, c->UD.p);
         launch_near(an, c->near_lanes, mixed, s);
@@ -1731,6 +1734,62 @@ int kmc_check_flags(kmc_ctx* c, void* stream, int32_t* h_flags) {
     HIPCHK(hipStreamSynchronize(s));
     HIPCHK(hipMemset(c->flags.p, 0, sizeof(int)));
     if (h_flags) *h_flags = fl;
+    return KMC_OK;
+}
+
+int kmc_time_kernel(kmc_ctx* c, void* stream, int32_t which, int32_t reps, int32_t precision, double* h_ms) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    if (reps < 1 || !h_ms) return set_err(KMC_E_ARG, "bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t P = c->last_P, T = c->last_T;
+    if ((which != 0 && (P < 1 || T < 1)) || !c->initialised) return set_err(KMC_E_STATE, "run a propose first");
+    const bool mixed = precision == KMC_PREC_MIXED;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, s));
+    for (int i = 0; i < reps; i++) {
+        if (which == 0) {
+            SweepArgs sa;
+            sa.g = c->geom();
+            sa.lexp = c->lexp.p;
+            sa.ncells = (int)c->ncells;
+            sa.mode = 0;
+            sa.rec = c->rec.p;
+            sa.src_pos4 = nullptr;
+            sa.src_cell = nullptr;
+            sa.src_begin = sa.src_end = 0;
+            c->fns.sweep(sa, (int)c->ncells, s, c->sweep_block);
+        } else if (which == 1) {
+            EvalArgs an = eval_args(c, (int)(T + P), c->epos4.p, c->ecell.p, c->eexcl.p, c->UD.p);
+            launch_near(an, c->near_lanes, mixed, s);
+        } else if (which == 2) {
+            EvalArgs af = eval_args(c, (int)(T + P), c->epos4.p, c->ecell.p, c->eexcl.p, c->UI.p);
+            c->fns.far(af, s);
+        } else if (which == 3) {
+            LRArgs la;
+            la.T = (int)T;
+            la.nrow = 2 * c->nh;
+            la.gdat = c->gdat.p;
+            la.gptr = c->gptr.p;
+            la.gidx = c->gidx.p;
+            la.gvec = c->gvec.p;
+            la.epos4 = c->epos4.p;
+            la.owner = c->owner.p;
+            la.lrd = c->lrd.p;
+            if (c->have_lr) c->fns.lr(la, s);
+        } else {
+            return set_err(KMC_E_ARG, "unknown kernel id");
+        }
+    }
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    *h_ms = (double)ms / reps;
+    HIPCHK(hipGetLastError());
     return KMC_OK;
 }
 

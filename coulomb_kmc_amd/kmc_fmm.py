@@ -384,6 +384,17 @@ class KMCFMM:
         _native.check(self._lib.kmc_check_flags(self._ctx, self._stream(), ctypes.byref(f)), "check_flags")
         return f.value
 
+    KERNELS = {"sweep": 0, "near": 1, "far": 2, "lr": 3}
+
+    def time_kernel(self, which, reps=20, precision=None, stream=None):
+        """Average ms per launch of one kernel (HIP events on the stream); 'sweep' changes the state."""
+        ms = ctypes.c_double(0.0)
+        s = stream if stream is not None else self._stream()
+        _native.check(self._lib.kmc_time_kernel(self._ctx, s, self.KERNELS[which], int(reps),
+                                                _native.PREC[precision or self.precision], ctypes.byref(ms)),
+                      "time_kernel")
+        return ms.value
+
     def sweep_hop(self, q, old_pos, new_pos, stream=None):
         o = np.ascontiguousarray(old_pos, dtype=np.float64)
         n = np.ascontiguousarray(new_pos, dtype=np.float64)

@@ -168,6 +168,15 @@ int kmc_set_tiles(kmc_ctx* ctx, int32_t near_lanes, int32_t sweep_block);
 /* Accept sweep only, for a hop of charge q from h_old to h_new (timing / tests). */
 int kmc_sweep_hop(kmc_ctx* ctx, void* stream, double q, const double* h_old_pos, const double* h_new_pos);
 
+/*
+ * Measurement hook (no reference counterpart): average duration in ms of `reps`
+ * back-to-back launches of one kernel, timed with HIP events on `stream`:
+ * which = 0 accept sweep (re-applies the last accepted move: changes the state,
+ * call last), 1 near field, 2 far field, 3 long-range proposal term (on the
+ * evaluation arrays of the last propose).
+ */
+int kmc_time_kernel(kmc_ctx* ctx, void* stream, int32_t which, int32_t reps, int32_t precision, double* h_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -701,6 +701,9 @@ void orc_initialise(void* h, double* energy) {
     if (energy) *energy = o->energy;
 }
 
+// level-0 multipole M and dot vector E only (the expansions come from orc_set_expansions)
+void orc_init_lr(void* h) { ((Oracle*)h)->init_lr(); }
+
 // use given primary-cell expansions (reference layout [ncps^3][2L^2]) instead of P2L
 void orc_set_expansions(void* h, const double* prim) {
     Oracle* o = (Oracle*)h;

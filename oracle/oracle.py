@@ -38,6 +38,7 @@ def lib():
         L.orc_set_particles.argtypes = [P, P, P]
         L.orc_set_lr.argtypes = [P, I64, P, P, P, P]
         L.orc_initialise.argtypes = [P, P]
+        L.orc_init_lr.argtypes = [P]
         L.orc_set_expansions.argtypes = [P, P]
         L.orc_get_expansions.argtypes = [P, P]
         L.orc_get_store_expansions.argtypes = [P, P]
@@ -122,6 +123,12 @@ class Oracle:
         e = np.zeros(1)
         lib().orc_initialise(self.h, _p(e))
         return float(e[0])
+
+    def init_from_expansions(self, full, energy):
+        """Take leaf expansions (reference layout) and the energy from elsewhere; build M/E."""
+        self.set_expansions(full)
+        lib().orc_init_lr(self.h)
+        self.energy = energy
 
     @property
     def energy(self):
