@@ -13,10 +13,11 @@
 //   k_far         indirect_interactions          coulomb_kmc/kmc_octal.py:495-707
 //   k_combine     self_interaction + diff_lib    kmc_fmm_self_interaction.py:190-242,
 //                                                kmc_fmm.py:583-663
-//   k_lr_prop     long_range_energy              kmc_full_long_range.py:710-858
+//   k_lr_part,    long_range_energy              kmc_full_long_range.py:710-858
+//   k_lr_prop2
 //   k_sweep       accept_local_exp (hop)         coulomb_kmc/kmc_octal.py:1008-1089
-//   k_accept_store extract + inject              coulomb_kmc/kmc_local.py:180-370
-//   k_multipole_delta long range accept          kmc_full_long_range.py:270-316
+//   k_accept_tail extract + inject,              coulomb_kmc/kmc_local.py:180-370,
+//                 long range accept              kmc_full_long_range.py:270-316
 //   k_dats_*      setup_move                     coulomb_kmc/kmc_mpi_decomp.py:501-616
 //
 // Solid harmonics are evaluated from Cartesian coordinates with the column
@@ -192,16 +193,102 @@ struct SweepArgs {
     int src_begin, src_end;
 };
 
-template <int L, int M1, int M2>
-__device__ __forceinline__ void sweep_columns(const SweepArgs& a, int cell) {
-    constexpr int N1 = L - M1;
-    constexpr int N2 = (M2 != M1) ? (L - M2) : 0;
-    constexpr int NA = N2 > 0 ? N2 : 1;
-    double a1r[N1], a1i[N1], a2r[NA], a2i[NA];
+// Rescaled column recurrence of the sweep: e_l = c_l / Gam_l obeys
+//   e_l = a2_lm zr e_{l-1} - ir2 e_{l-2},  Gam_l = beta_lm Gam_{l-2},  a2_lm = alpha_lm Gam_{l-1} / Gam_l,
+// one multiply less per coefficient; accumulators are multiplied by Gam_l once per leaf.
+struct RecTab2 {
+    double a2[KMAX_L + 1][KMAX_L + 1];
+    double gam[KMAX_L + 1][KMAX_L + 1];
+};
+constexpr RecTab2 make_rectab2() {
+    RecTab2 t{};
+    for (int m = 0; m <= KMAX_L; m++) {
+        t.gam[m][m] = 1.0;
+        if (m + 1 <= KMAX_L) {
+            t.gam[m + 1][m] = 1.0;
+            t.a2[m + 1][m] = kRec.a[m + 1][m];
+        }
+        for (int l = m + 2; l <= KMAX_L; l++) {
+            t.gam[l][m] = kRec.b[l][m] * t.gam[l - 2][m];
+            t.a2[l][m] = kRec.a[l][m] * t.gam[l - 1][m] / t.gam[l][m];
+        }
+    }
+    return t;
+}
+static constexpr RecTab2 kRec2 = make_rectab2();
+
+// one column m of one source-image: acc[k] += e_{m+k} * W, k = 0 .. L-m-1
+template <int L, int M>
+__device__ __forceinline__ void sweep_column(double zr, double ir2, Cplx W, double* ar, double* ai) {
+    double e2 = 0.0, e1 = 1.0;
+    ar[0] += W.re;
+    ai[0] += W.im;
 #pragma unroll
-    for (int k = 0; k < N1; k++) { a1r[k] = 0.0; a1i[k] = 0.0; }
+    for (int k = 1; k < L - M; k++) {
+        const int l = M + k;
+        const double e = fma(kRec2.a2[l][M] * zr, e1, -ir2 * e2);
+        e2 = e1;
+        e1 = e;
+        ar[k] = fma(e, W.re, ar[k]);
+        ai[k] = fma(e, W.im, ai[k]);
+    }
+}
+
+template <int L, int M>
+__device__ __forceinline__ void sweep_store_column(double2* le, const double* ar, const double* ai) {
 #pragma unroll
-    for (int k = 0; k < NA; k++) { a2r[k] = 0.0; a2i[k] = 0.0; }
+    for (int k = 0; k < L - M; k++) {
+        const double gm = kRec2.gam[M + k][M];
+        double2 v = le[hidx(M + k, M)];
+        v.x = fma(gm, ar[k], v.x);
+        v.y = fma(gm, ai[k], v.y);
+        le[hidx(M + k, M)] = v;
+    }
+}
+
+
+// columns (P, L-1-P) of pair P into accumulator row ar/ai (first L-P entries: column P)
+template <int L, int P>
+__device__ __forceinline__ void sweep_pair_cols(double zr, double ir2, Cplx u, double base, double* ar, double* ai) {
+    constexpr int M1 = P, M2 = L - 1 - P;
+    const Cplx U1 = cpow<M1>(u);
+    const double f1 = base * kRec.s[M1];
+    sweep_column<L, M1>(zr, ir2, Cplx{f1 * U1.re, f1 * U1.im}, ar, ai);
+    if constexpr (M2 != M1) {
+        const Cplx U2 = cpow<M2>(u);
+        const double f2 = base * kRec.s[M2];
+        sweep_column<L, M2>(zr, ir2, Cplx{f2 * U2.re, f2 * U2.im}, ar + (L - M1), ai + (L - M1));
+    }
+}
+template <int L, int P0, int NP, int J = 0>
+__device__ __forceinline__ void sweep_group_cols(double zr, double ir2, Cplx u, double base, double (*ar)[L + 1],
+                                                 double (*ai)[L + 1]) {
+    if constexpr (J < NP && P0 + J < (L + 1) / 2) {
+        sweep_pair_cols<L, P0 + J>(zr, ir2, u, base, ar[J], ai[J]);
+        sweep_group_cols<L, P0, NP, J + 1>(zr, ir2, u, base, ar, ai);
+    }
+}
+template <int L, int P0, int NP, int J = 0>
+__device__ __forceinline__ void sweep_group_store(double2* le, double (*ar)[L + 1], double (*ai)[L + 1]) {
+    if constexpr (J < NP && P0 + J < (L + 1) / 2) {
+        constexpr int M1 = P0 + J, M2 = L - 1 - M1;
+        sweep_store_column<L, M1>(le, ar[J], ai[J]);
+        if constexpr (M2 != M1) sweep_store_column<L, M2>(le, ar[J] + (L - M1), ai[J] + (L - M1));
+        sweep_group_store<L, P0, NP, J + 1>(le, ar, ai);
+    }
+}
+
+// pairs P = G*NP .. G*NP+NP-1 of columns (P, L-1-P) for one leaf (one lane)
+template <int L, int G, int NP>
+__device__ __forceinline__ void sweep_group(const SweepArgs& a, int cell) {
+    constexpr int NPAIR = (L + 1) / 2;
+    constexpr int P0 = G * NP;
+    constexpr int W = L + 1;                 // accumulator entries per pair
+    double ar[NP][W], ai[NP][W];
+#pragma unroll
+    for (int j = 0; j < NP; j++)
+#pragma unroll
+        for (int k = 0; k < W; k++) { ar[j][k] = 0.0; ai[j][k] = 0.0; }
 
     const int n = a.g.ncps;
     const int tx = cell % n, ty = (cell / n) % n, tz = cell / (n * n);
@@ -210,7 +297,6 @@ __device__ __forceinline__ void sweep_columns(const SweepArgs& a, int cell) {
     const double cz = a.g.c0 + tz * a.g.w;
     const bool periodic = a.g.bc != KMC_BC_FREE_SPACE;
     const int nimg = periodic ? 27 : 1;
-
     int nsrc;
     if (a.mode == 0) nsrc = a.rec->valid ? 2 : 0;
     else nsrc = a.src_end - a.src_begin;
@@ -228,15 +314,16 @@ __device__ __forceinline__ void sweep_columns(const SweepArgs& a, int cell) {
             sx = p.x; sy = p.y; sz = p.z; sq = p.w;
             sc = a.src_cell[a.src_begin + s];
         }
-        const int scx = sc % n, scy = (sc / n) % n, scz = sc / (n * n);
+        const int dcx = sc % n - tx, dcy = (sc / n) % n - ty, dcz = sc / (n * n) - tz;
+        const double dx0 = sx - cx, dy0 = sy - cy, dz0 = sz - cz;
         for (int k = 0; k < nimg; k++) {
             int ox = 0, oy = 0, oz = 0;
             if (periodic) stencil(k, ox, oy, oz);
-            const int ddx = scx + ox * n - tx, ddy = scy + oy * n - ty, ddz = scz + oz * n - tz;
+            const int ddx = dcx + ox * n, ddy = dcy + oy * n, ddz = dcz + oz * n;
             const bool ws = !(ddx < 2 && ddx > -2 && ddy < 2 && ddy > -2 && ddz < 2 && ddz > -2);
-            const double dx = (sx + ox * a.g.E) - cx;
-            const double dy = (sy + oy * a.g.E) - cy;
-            const double dz = (sz + oz * a.g.E) - cz;
+            const double dx = fma((double)ox, a.g.E, dx0);
+            const double dy = fma((double)oy, a.g.E, dy0);
+            const double dz = fma((double)oz, a.g.E, dz0);
             double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
             r2 = ws ? r2 : 1.0;
             const double qe = ws ? sq : 0.0;
@@ -245,83 +332,28 @@ __device__ __forceinline__ void sweep_columns(const SweepArgs& a, int cell) {
             const double zr = dz * ir2;
             const Cplx u = {dx * ir2, -dy * ir2};     // conj((x+iy)/r^2)
             const double base = qe * ir;
-            const Cplx U1 = cpow<M1>(u);
-            {
-                const double f = base * kRec.s[M1];
-                const double wr = f * U1.re, wi = f * U1.im;
-                double c2 = 0.0, c1 = 1.0;
-                a1r[0] += wr;
-                a1i[0] += wi;
-#pragma unroll
-                for (int k2 = 1; k2 < N1; k2++) {
-                    const int l = M1 + k2;
-                    const double c = fma(kRec.a[l][M1] * zr, c1, -(kRec.b[l][M1] * ir2) * c2);
-                    c2 = c1;
-                    c1 = c;
-                    a1r[k2] = fma(c, wr, a1r[k2]);
-                    a1i[k2] = fma(c, wi, a1i[k2]);
-                }
-            }
-            if constexpr (N2 > 0) {
-                const Cplx U2 = cmul(U1, cpow<M2 - M1>(u));
-                const double f = base * kRec.s[M2];
-                const double wr = f * U2.re, wi = f * U2.im;
-                double c2 = 0.0, c1 = 1.0;
-                a2r[0] += wr;
-                a2i[0] += wi;
-#pragma unroll
-                for (int k2 = 1; k2 < N2; k2++) {
-                    const int l = M2 + k2;
-                    const double c = fma(kRec.a[l][M2] * zr, c1, -(kRec.b[l][M2] * ir2) * c2);
-                    c2 = c1;
-                    c1 = c;
-                    a2r[k2] = fma(c, wr, a2r[k2]);
-                    a2i[k2] = fma(c, wi, a2i[k2]);
-                }
-            }
+            sweep_group_cols<L, P0, NP>(zr, ir2, u, base, ar, ai);
         }
     }
-    // one FP64 add per coefficient into the stored expansion (kmc_octal.py:1084)
     double2* le = a.lexp + (size_t)cell * nhalf(L);
-#pragma unroll
-    for (int k = 0; k < N1; k++) {
-        double2 v = le[hidx(M1 + k, M1)];
-        v.x += a1r[k];
-        v.y += a1i[k];
-        le[hidx(M1 + k, M1)] = v;
-    }
-    if constexpr (N2 > 0) {
-#pragma unroll
-        for (int k = 0; k < N2; k++) {
-            double2 v = le[hidx(M2 + k, M2)];
-            v.x += a2r[k];
-            v.y += a2i[k];
-            le[hidx(M2 + k, M2)] = v;
-        }
+    sweep_group_store<L, P0, NP>(le, ar, ai);
+}
+
+template <int L, int NP, int G0>
+__device__ __forceinline__ void sweep_dispatch(const SweepArgs& a, int cell, int group) {
+    constexpr int NG = ((L + 1) / 2 + NP - 1) / NP;
+    if constexpr (G0 < NG) {
+        if (group == G0) sweep_group<L, G0, NP>(a, cell);
+        else sweep_dispatch<L, NP, G0 + 1>(a, cell, group);
     }
 }
 
-template <int L, int P>
-__device__ __forceinline__ void sweep_pair(const SweepArgs& a, int cell) {
-    if constexpr (P < (L + 1) / 2) {
-        sweep_columns<L, P, L - 1 - P>(a, cell);
-    }
-}
-
-template <int L, int P0>
-__device__ __forceinline__ void sweep_dispatch(const SweepArgs& a, int cell, int pair) {
-    if constexpr (P0 < (L + 1) / 2) {
-        if (pair == P0) sweep_pair<L, P0>(a, cell);
-        else sweep_dispatch<L, P0 + 1>(a, cell, pair);
-    }
-}
-
-// grid: x = leaf blocks (lane = leaf), y = column pair (m, L-1-m)
-template <int L>
-__global__ void __launch_bounds__(1024) k_sweep(SweepArgs a) {
+// grid: x = leaf blocks (lane = leaf), y = group of NP column pairs (m, L-1-m)
+template <int L, int NP>
+__global__ void __launch_bounds__(256) k_sweep(SweepArgs a) {
     const int cell = blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= a.ncells) return;
-    sweep_dispatch<L, 0>(a, cell, (int)blockIdx.y);
+    sweep_dispatch<L, NP, 0>(a, cell, (int)blockIdx.y);
 }
 
 // ============================================================================
@@ -341,6 +373,48 @@ struct EvalArgs {
     int occmax;
 };
 
+// columns M1 and M2 (M2 == M1: one column) of  Re sum_l L_lm R_l^m(d)  with weight
+// 2 for m > 0 (the m < 0 terms are the complex conjugates)
+template <int L, int M1, int M2>
+__device__ __forceinline__ double far_columns(double dx, double dy, double dz, double r2, const double2* le) {
+    double phi = 0.0;
+    const Cplx w = {dx, dy};
+    const Cplx C1 = cpow<M1>(w);
+#pragma unroll
+    for (int col = 0; col < 2; col++) {
+        if (col == 1 && M2 == M1) break;
+        const int m = col == 0 ? M1 : M2;
+        const Cplx C = col == 0 ? C1 : cmul(C1, cpow<M2 - M1>(w));
+        double b2 = 0.0, b1 = kRec.s[m];
+        double2 v = le[hidx(m, m)];
+        double sr = b1 * v.x, si = b1 * v.y;
+#pragma unroll
+        for (int l = m + 1; l < L; l++) {
+            const double bb = fma(kRec.a[l][m] * dz, b1, -(kRec.b[l][m] * r2) * b2);
+            b2 = b1;
+            b1 = bb;
+            v = le[hidx(l, m)];
+            sr = fma(bb, v.x, sr);
+            si = fma(bb, v.y, si);
+        }
+        const double re = fma(C.re, sr, -C.im * si);
+        phi = fma(m == 0 ? 1.0 : 2.0, re, phi);
+    }
+    return phi;
+}
+
+template <int L, int P0>
+__device__ __forceinline__ double far_dispatch(int pair, double dx, double dy, double dz, double r2,
+                                               const double2* le) {
+    if constexpr (P0 < (L + 1) / 2) {
+        if (pair == P0) return far_columns<L, P0, L - 1 - P0>(dx, dy, dz, r2, le);
+        return far_dispatch<L, P0 + 1>(pair, dx, dy, dz, r2, le);
+    } else {
+        return 0.0;
+    }
+}
+
+// one lane per evaluation point, grid.y = column pair (m, L-1-m); out[pair * n + i]
 template <int L>
 __global__ void __launch_bounds__(256) k_far(EvalArgs a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -354,29 +428,8 @@ __global__ void __launch_bounds__(256) k_far(EvalArgs a) {
     const double dz = p.z - (a.g.c0 + tz * a.g.w);
     const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
     const double2* le = a.lexp + (size_t)c * nhalf(L);
-    const Cplx w = {dx, dy};
-    Cplx C = {1.0, 0.0};
-    double phi = 0.0;
-#pragma unroll
-    for (int m = 0; m < L; m++) {
-        double b2 = 0.0, b1 = kRec.s[m];
-        double2 v = le[hidx(m, m)];
-        double sr = b1 * v.x, si = b1 * v.y;
-#pragma unroll
-        for (int l = m + 1; l < L; l++) {
-            const double b = fma(kRec.a[l][m] * dz, b1, -(kRec.b[l][m] * r2) * b2);
-            b2 = b1;
-            b1 = b;
-            v = le[hidx(l, m)];
-            sr = fma(b, v.x, sr);
-            si = fma(b, v.y, si);
-        }
-        // Re(C_m * S_m), weight 2 for m > 0 (the m < 0 terms are the conjugates)
-        const double re = fma(C.re, sr, -C.im * si);
-        phi = (m == 0) ? re : fma(2.0, re, phi);
-        C = cmul(C, w);
-    }
-    a.out[i] = p.w * phi;
+    const int pair = blockIdx.y;
+    a.out[(size_t)pair * a.n + i] = p.w * far_dispatch<L, 0>(pair, dx, dy, dz, r2, le);
 }
 
 // ============================================================================
@@ -429,6 +482,53 @@ __global__ void __launch_bounds__(256) k_near(EvalArgs a) {
         for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, G);
     }
     if (active && lane == 0) a.out[i] = acc * a.pos4[i].w;
+}
+
+// near field, one lane per neighbour leaf: 32 lanes per evaluation point, lanes
+// 0..26 each walk one leaf of the stencil, shuffle reduction over the 32 lanes.
+template <bool MIXED>
+__global__ void __launch_bounds__(256) k_near_cells(EvalArgs a) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = gid >> 5;
+    const int k = gid & 31;
+    double acc = 0.0;
+    if (i < a.n && k < 27) {
+        const double4 p = a.pos4[i];
+        const int c = a.cell[i];
+        const int ex = a.excl[i];
+        const int n = a.g.ncps;
+        const int tx = c % n, ty = (c / n) % n, tz = c / (n * n);
+        int ox, oy, oz;
+        stencil(k, ox, oy, oz);
+        int nx = tx + ox, ny = ty + oy, nz = tz + oz;
+        double shx = 0.0, shy = 0.0, shz = 0.0;
+        bool ok = true;
+        if (a.g.bc != KMC_BC_FREE_SPACE) {
+            if (nx < 0) { nx += n; shx = -a.g.E; } else if (nx >= n) { nx -= n; shx = a.g.E; }
+            if (ny < 0) { ny += n; shy = -a.g.E; } else if (ny >= n) { ny -= n; shy = a.g.E; }
+            if (nz < 0) { nz += n; shz = -a.g.E; } else if (nz >= n) { nz -= n; shz = a.g.E; }
+        } else if (nx < 0 || nx >= n || ny < 0 || ny >= n || nz < 0 || nz >= n) {
+            ok = false;
+        }
+        if (ok) {
+            const int nc = nx + n * (ny + n * nz);
+            const int occ = min(a.occ[nc], a.occmax);
+            const double px = p.x - shx, py = p.y - shy, pz = p.z - shz;
+            const size_t base = (size_t)nc * a.occmax;
+#pragma unroll 2
+            for (int s = 0; s < occ; s++) {
+                const double4 q = a.slot_pos[base + s];
+                const double dx = px - q.x, dy = py - q.y, dz = pz - q.z;
+                const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                const double contrib = q.w * rsqrt_sel<MIXED>(r2);
+                if (ex >= 0) acc += (a.slot_id[base + s] != ex) ? contrib : 0.0;
+                else acc += contrib;
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 32);
+    if (i < a.n && k == 0) a.out[i] = acc * a.pos4[i].w;
 }
 
 // ============================================================================
@@ -516,17 +616,27 @@ struct CombineArgs {
     const double4* epos4;
     const int32_t* owner;
     const double* UD;
-    const double* UI;
-    const double* lrd;           // old - new long-range energy, or null
+    const double* UI;            // far-field partials [nfar][T+P]
+    int nfar;
+    const double* lpart;         // long range per proposal partials [12][T], or null
+    const double* cpart;         // long range per moving charge partials [12][P]
+    int P;
     const int64_t* rate_loc;     // output index, or null (identity)
     double* out;
 };
+
+__device__ __forceinline__ double sum_parts(const double* v, int nparts, size_t stride, size_t i) {
+    double s = 0.0;
+    for (int j = 0; j < nparts; j++) s += v[(size_t)j * stride + i];
+    return s;
+}
 
 template <bool MIXED>
 __global__ void k_combine(CombineArgs a) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.T) return;
-    const int po = a.T + a.owner[t];
+    const int ow = a.owner[t];
+    const int po = a.T + ow;
     const double4 xo = a.epos4[po];
     const double4 xn = a.epos4[t];
     double e27;
@@ -547,10 +657,15 @@ __global__ void k_combine(CombineArgs a) {
         }
     }
     double usi = xo.w * xo.w * e27;
-    if (a.lrd) usi += a.lrd[t];
+    if (a.lpart) usi -= sum_parts(a.lpart, 12, a.T, t) + sum_parts(a.cpart, 12, a.P, ow);
     double de;
-    if (a.g.bc == KMC_BC_FF_ONLY) de = a.eu * (-1.0 * usi);
-    else de = a.eu * (((a.UD[t] + a.UI[t]) - a.UD[po] - a.UI[po]) - usi);
+    if (a.g.bc == KMC_BC_FF_ONLY) {
+        de = a.eu * (-1.0 * usi);
+    } else {
+        const size_t ne = (size_t)a.T + a.P;
+        const double ui_n = sum_parts(a.UI, a.nfar, ne, t), ui_o = sum_parts(a.UI, a.nfar, ne, po);
+        de = a.eu * (((a.UD[t] + ui_n) - a.UD[po] - ui_o) - usi);
+    }
     const int64_t o = a.rate_loc ? a.rate_loc[t] : (int64_t)t;
     a.out[o] = de;
 }
@@ -588,40 +703,174 @@ __device__ __forceinline__ void multipole_column(int m, double x, double y, doub
     }
 }
 
-struct LRArgs {
-    int T;
-    int nrow;                    // 2 * nh
-    const double* gdat;
-    const int64_t* gptr;
-    const int32_t* gidx;
-    const double* gvec;          // G M
-    const double4* epos4;
-    const int32_t* owner;
-    double* lrd;                 // old - new
+// ---------------------------------------------------------------------------
+// Long range, blocked form.  Under the cubic lattice's symmetry G is exactly
+// block diagonal over 12 blocks: (real | imaginary part) x (m mod 4 class:
+// {0}, {2}, odd) x (parity of l) - only lattice sums S_p^q with p even and
+// q = 0 mod 4 survive.  Entries of a block are ordered by m, then l.  The host
+// gathers the dense blocks (kmc_set_lr_operator); the kernels below keep a
+// block's multipole entries in registers and read the block coefficients as
+// wave-uniform (scalar) operands.
+//
+//   dLR(old - new) = -( v_n . a_o + 0.5 v_n G v_n + c_o ),
+//   a_o = g - G v_o,  c_o = 0.5 v_o G v_o - g . v_o,  g = G M,
+// with v = q conj(R(x)) the packed multipole of the moved charge.
+// ---------------------------------------------------------------------------
+constexpr int LRB_MAX = 64;
+struct LRBlocks {
+    int n[6];
+    int off[6];            // offset of the block pair (real block, then imaginary block) in gblk
+    int l[6][LRB_MAX];
+    int m[6][LRB_MAX];
+};
+__host__ __device__ constexpr int m_class(int m) { return (m % 4 == 0) ? 0 : ((m % 4 == 2) ? 1 : 2); }
+__host__ __device__ constexpr LRBlocks make_lr_blocks(int L) {
+    LRBlocks t{};
+    int off = 0;
+    for (int b = 0; b < 6; b++) {
+        const int c = b / 2, par = b % 2;
+        int k = 0;
+        for (int m = 0; m < L; m++) {
+            if (m_class(m) != c) continue;
+            for (int l = m; l < L; l++)
+                if ((l & 1) == par) {
+                    t.l[b][k] = l;
+                    t.m[b][k] = m;
+                    k++;
+                }
+        }
+        t.n[b] = k;
+        t.off[b] = off;
+        off += 2 * k * k;
+    }
+    return t;
+}
+template <int L>
+struct LRTab {
+    static constexpr LRBlocks T = make_lr_blocks(L);
 };
 
-// one wave per proposal: delta = m(new) - m(old); d = g.delta + 0.5 delta.G.delta
+// entries of block B of v = q conj(R(x)): real parts (COMP 0) or imaginary parts (COMP 1)
+template <int L, int B, int COMP>
+__device__ __forceinline__ void lr_block_values(double x, double y, double z, double q, double* v) {
+    constexpr int C = B / 2, PAR = B % 2;
+    const double r2 = fma(x, x, fma(y, y, z * z));
+    const Cplx w = {x, -y};
+    const Cplx w2 = cmul(w, w);
+    const Cplx w4 = cmul(w2, w2);
+    Cplx Cm = (C == 0) ? Cplx{1.0, 0.0} : ((C == 1) ? w2 : w);
+    const Cplx step = (C == 2) ? w2 : w4;
+    int k = 0;
+#pragma unroll
+    for (int m = 0; m < L; m++) {
+        if (m_class(m) != C) continue;
+        const double qc = q * (COMP == 0 ? Cm.re : Cm.im);
+        double b2 = 0.0, b1 = 0.0;
+#pragma unroll
+        for (int l = m; l < L; l++) {
+            const double bb = (l == m) ? kRec.s[m] : fma(kRec.a[l][m] * z, b1, -(kRec.b[l][m] * r2) * b2);
+            b2 = b1;
+            b1 = bb;
+            if ((l & 1) == PAR) v[k++] = bb * qc;
+        }
+        Cm = cmul(Cm, step);
+    }
+}
+
+// per proposal: a_o . v and 0.5 v G v over one component of block B
+template <int L, int B, int COMP>
+__device__ __forceinline__ void lr_block_prop(double x, double y, double z, double q, const double* __restrict__ ao,
+                                              const double* __restrict__ gblk, double& acc) {
+    constexpr LRBlocks T = LRTab<L>::T;
+    constexpr int NB = T.n[B];
+    if constexpr (NB > 0) {
+        double v[NB];
+        lr_block_values<L, B, COMP>(x, y, z, q, v);
+        const double* G = gblk + T.off[B] + COMP * NB * NB;
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            double s = fma(0.5 * G[i * NB + i], v[i], ao[2 * hidx(T.l[B][i], T.m[B][i]) + COMP]);
+#pragma unroll
+            for (int j = 0; j < i; j++) s = fma(G[i * NB + j], v[j], s);
+            acc = fma(v[i], s, acc);
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+// per moving charge: a_o = g - G v_o over one component of block B, and its share of c_o
+template <int L, int B, int COMP>
+__device__ __forceinline__ void lr_block_part(double x, double y, double z, double q, const double* __restrict__ g,
+                                              const double* __restrict__ gblk, double* __restrict__ ao, double& c) {
+    constexpr LRBlocks T = LRTab<L>::T;
+    constexpr int NB = T.n[B];
+    if constexpr (NB > 0) {
+        double v[NB];
+        lr_block_values<L, B, COMP>(x, y, z, q, v);
+        const double* G = gblk + T.off[B] + COMP * NB * NB;
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            const int h = 2 * hidx(T.l[B][i], T.m[B][i]) + COMP;
+            double yy = 0.0;
+#pragma unroll
+            for (int j = 0; j < NB; j++) yy = fma(G[i * NB + j], v[j], yy);
+            ao[h] = g[h] - yy;
+            c += v[i] * fma(0.5, yy, -g[h]);
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+struct LR2Args {
+    int P, T;
+    const double* gblk;
+    const double* gvec;
+    const double4* epos4;
+    const int32_t* owner;
+    double* aold;              // [P][2 nh]
+    double* cpart;             // [12][P]   c_o per (block, component)
+    double* lpart;             // [12][T]   v.a_o + 0.5 v G v per (block, component)
+};
+
+template <int L, int Y>
+__device__ __forceinline__ void lr_part_dispatch(int y, double x, double yy, double z, double q, const double* g,
+                                                 const double* gblk, double* ao, double& c) {
+    if constexpr (Y < 12) {
+        if (y == Y) lr_block_part<L, Y / 2, Y % 2>(x, yy, z, q, g, gblk, ao, c);
+        else lr_part_dispatch<L, Y + 1>(y, x, yy, z, q, g, gblk, ao, c);
+    }
+}
+template <int L, int Y>
+__device__ __forceinline__ void lr_prop_dispatch(int y, double x, double yy, double z, double q, const double* ao,
+                                                 const double* gblk, double& acc) {
+    if constexpr (Y < 12) {
+        if (y == Y) lr_block_prop<L, Y / 2, Y % 2>(x, yy, z, q, ao, gblk, acc);
+        else lr_prop_dispatch<L, Y + 1>(y, x, yy, z, q, ao, gblk, acc);
+    }
+}
+
 template <int L>
-__global__ void __launch_bounds__(64) k_lr_prop(LRArgs a) {
-    constexpr int NR = 2 * nhalf(L);
-    __shared__ double dl[NR];
-    const int t = blockIdx.x;
-    const int lane = threadIdx.x;
+__global__ void __launch_bounds__(64) k_lr_part(LR2Args a, const double* __restrict__ gblk,
+                                                const double* __restrict__ gvec) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.P) return;
+    const double4 xo = a.epos4[a.T + p];
+    double* ao = a.aold + (size_t)p * 2 * nhalf(L);
+    double c = 0.0;
+    lr_part_dispatch<L, 0>((int)blockIdx.y, xo.x, xo.y, xo.z, xo.w, gvec, gblk, ao, c);
+    a.cpart[(size_t)blockIdx.y * a.P + p] = c;
+}
+
+template <int L>
+__global__ void __launch_bounds__(64) k_lr_prop2(LR2Args a, const double* __restrict__ gblk) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.T) return;
+    const int p = a.owner[t];
     const double4 xn = a.epos4[t];
-    const double4 xo = a.epos4[a.T + a.owner[t]];
-    if (lane < L) {
-        multipole_column<L>(lane, xn.x, xn.y, xn.z, xn.w, dl, 1.0, false);
-        multipole_column<L>(lane, xo.x, xo.y, xo.z, xo.w, dl, -1.0, true);
-    }
-    __syncthreads();
+    const double* ao = a.aold + (size_t)p * 2 * nhalf(L);
     double acc = 0.0;
-    for (int row = lane; row < NR; row += 64) {
-        double y = 0.0;
-        for (int64_t k = a.gptr[row]; k < a.gptr[row + 1]; k++) y = fma(a.gdat[k], dl[a.gidx[k]], y);
-        acc = fma(dl[row], fma(0.5, y, a.gvec[row]), acc);
-    }
-    acc = wave_sum(acc);
-    if (lane == 0) a.lrd[t] = -acc;
+    lr_prop_dispatch<L, 0>((int)blockIdx.y, xn.x, xn.y, xn.z, xn.w, ao, gblk, acc);
+    a.lpart[(size_t)blockIdx.y * a.T + t] = acc;
 }
 
 // g = G M and E_lr = 0.5 M.g  (one block)
@@ -672,16 +921,6 @@ __global__ void k_reduce_rows(int nblk, int nrow, const double* partial, double*
     out[r] = s;
 }
 
-// accept: M += m(old, -q); M += m(new, +q) (kmc_full_long_range.py:270-316)
-template <int L>
-__global__ void __launch_bounds__(64) k_multipole_delta(const MoveRec* rec, double* M) {
-    if (!rec->valid) return;
-    const int m = threadIdx.x;
-    if (m >= L) return;
-    multipole_column<L>(m, rec->old_pos[0], rec->old_pos[1], rec->old_pos[2], rec->q, M, -1.0, true);
-    multipole_column<L>(m, rec->new_pos[0], rec->new_pos[1], rec->new_pos[2], rec->q, M, 1.0, true);
-}
-
 // ============================================================================
 // accept: move record, cell tables
 // ============================================================================
@@ -702,9 +941,8 @@ __global__ void k_prepare_record(Geom g, int pid, double nx, double ny, double n
 
 // kmc_local.py:180-271 (extract: swap the last occupant into the freed slot) then
 // kmc_local.py:274-359 (inject: append), then the particle arrays.
-__global__ void k_accept_store(const MoveRec* rec, int* occ, double4* slot_pos, int32_t* slot_id, int occmax,
-                               double4* ppos4, int32_t* pcell, int* flags) {
-    if (!rec->valid) return;
+__device__ void store_update(const MoveRec* rec, int* occ, double4* slot_pos, int32_t* slot_id, int occmax,
+                             double4* ppos4, int32_t* pcell, int* flags) {
     const int pid = rec->pid;
     const int oc = rec->old_cell, nc = rec->new_cell;
     const size_t ob = (size_t)oc * occmax, nb = (size_t)nc * occmax;
@@ -723,6 +961,63 @@ __global__ void k_accept_store(const MoveRec* rec, int* occ, double4* slot_pos, 
     occ[nc] = no + 1;
     ppos4[pid] = make_double4(rec->new_pos[0], rec->new_pos[1], rec->new_pos[2], rec->q);
     pcell[pid] = nc;
+}
+
+struct TailArgs {
+    const MoveRec* rec;
+    int* occ;
+    double4* slot_pos;
+    int32_t* slot_id;
+    int occmax;
+    double4* ppos4;
+    int32_t* pcell;
+    int* flags;
+    int lr;                      // update M, g = G M and E_lr
+    double* M;
+    const double* gdat;
+    const int64_t* gptr;
+    const int32_t* gidx;
+    double* g;
+    double* elr;
+    int64_t* counter;            // step counter to advance, or null
+};
+
+// everything of an accept except the leaf sweep, in one block: cell tables and
+// particle arrays (thread 0), M += m(old, -q) then += m(new, +q)
+// (kmc_full_long_range.py:270-316), g = G M, E_lr = 0.5 M.g, step counter.
+template <int L>
+__global__ void __launch_bounds__(256) k_accept_tail(TailArgs a) {
+    constexpr int NR = 2 * nhalf(L);
+    __shared__ double dold[NR], dnew[NR];
+    __shared__ double part[256];
+    const int tid = threadIdx.x;
+    const bool valid = a.rec->valid != 0;
+    if (valid && tid == 0) store_update(a.rec, a.occ, a.slot_pos, a.slot_id, a.occmax, a.ppos4, a.pcell, a.flags);
+    if (valid && a.lr) {
+        const MoveRec* r = a.rec;
+        if (tid < L) {
+            multipole_column<L>(tid, r->old_pos[0], r->old_pos[1], r->old_pos[2], r->q, dold, -1.0, false);
+            multipole_column<L>(tid, r->new_pos[0], r->new_pos[1], r->new_pos[2], r->q, dnew, 1.0, false);
+        }
+        __syncthreads();
+        for (int k = tid; k < NR; k += blockDim.x) a.M[k] = (a.M[k] + dold[k]) + dnew[k];
+        __syncthreads();
+        double acc = 0.0;
+        for (int row = tid; row < NR; row += blockDim.x) {
+            double y = 0.0;
+            for (int64_t k = a.gptr[row]; k < a.gptr[row + 1]; k++) y = fma(a.gdat[k], a.M[a.gidx[k]], y);
+            a.g[row] = y;
+            acc = fma(a.M[row], y, acc);
+        }
+        part[tid] = acc;
+        __syncthreads();
+        for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+            if (tid < o) part[tid] += part[tid + o];
+            __syncthreads();
+        }
+        if (tid == 0) *a.elr = 0.5 * part[0];
+    }
+    if (tid == 0 && a.counter) *a.counter += 1;
 }
 
 // ============================================================================
@@ -818,7 +1113,6 @@ __global__ void __launch_bounds__(1024) k_select(SelectArgs a) {
     }
 }
 
-__global__ void k_counter_inc(int64_t* c) { *c += 1; }
 
 // ============================================================================
 // propose_with_dats set-up (mask compaction, kmc_mpi_decomp.py:548-614)
@@ -910,11 +1204,15 @@ __global__ void k_materialize(Geom g, int n, int64_t nsteps, int P, int k, const
     }
 }
 
-__global__ void k_half_sum(int n, const double* a, const double* b, double* out) {
-    // out = 0.5 * sum (a + b)  (one block, deterministic)
+__global__ void k_half_sum(int n, const double* a, const double* b, int nb, double* out) {
+    // out = 0.5 * sum_i (a_i + sum_j b[j*n + i])  (one block, deterministic)
     __shared__ double part[256];
     double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s += a[i] + b[i];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        double bi = 0.0;
+        for (int j = 0; j < nb; j++) bi += b[(size_t)j * n + i];
+        s += a[i] + bi;
+    }
     part[threadIdx.x] = s;
     __syncthreads();
     for (int o = blockDim.x / 2; o > 0; o >>= 1) {
@@ -934,36 +1232,44 @@ __global__ void k_finish_energy(const double* e_sr, const double* elr, double eu
 // ============================================================================
 // order dispatch tables
 // ============================================================================
-typedef void (*SweepLaunch)(SweepArgs, int, hipStream_t, int);
+typedef void (*SweepLaunch)(SweepArgs, int, hipStream_t, int, int);
 typedef void (*EvalLaunch)(EvalArgs, hipStream_t);
-typedef void (*LRLaunch)(LRArgs, hipStream_t);
+typedef void (*LRLaunch)(LR2Args, hipStream_t);
 typedef void (*MpLaunch)(int, int, const double4*, double*, int, hipStream_t);
-typedef void (*MdLaunch)(const MoveRec*, double*, hipStream_t);
+typedef void (*MdLaunch)(TailArgs, hipStream_t);
 
-template <int L>
-static void launch_sweep(SweepArgs a, int ncells, hipStream_t s, int block) {
-    const int npairs = (L + 1) / 2;
+template <int L, int NP>
+static void launch_sweep_np(SweepArgs a, int ncells, hipStream_t s, int block) {
+    const int ng = ((L + 1) / 2 + NP - 1) / NP;
     int bx = (ncells + block - 1) / block;
-    if (bx % 8) bx += 8 - bx % 8;   // all column pairs of a leaf block on one XCD
-    hipLaunchKernelGGL(k_sweep<L>, dim3(bx, npairs), dim3(block), 0, s, a);
+    if (bx % 8) bx += 8 - bx % 8;   // all column groups of a leaf block on one XCD
+    hipLaunchKernelGGL((k_sweep<L, NP>), dim3(bx, ng), dim3(block), 0, s, a);
 }
+template <int L>
+static void launch_sweep(SweepArgs a, int ncells, hipStream_t s, int block, int np) {
+    if (np == 1) launch_sweep_np<L, 1>(a, ncells, s, block);
+    else if (np == 3 && L <= 12) launch_sweep_np<L, 3>(a, ncells, s, block);
+    else launch_sweep_np<L, 2>(a, ncells, s, block);
+}
+
 template <int L>
 static void launch_far(EvalArgs a, hipStream_t s) {
     if (a.n <= 0) return;
-    hipLaunchKernelGGL(k_far<L>, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_far<L>, dim3((a.n + 255) / 256, (L + 1) / 2), dim3(256), 0, s, a);
 }
 template <int L>
-static void launch_lr(LRArgs a, hipStream_t s) {
-    if (a.T <= 0) return;
-    hipLaunchKernelGGL(k_lr_prop<L>, dim3(a.T), dim3(64), 0, s, a);
+static void launch_lr(LR2Args a, hipStream_t s) {
+    if (a.T <= 0 || a.P <= 0) return;
+    hipLaunchKernelGGL(k_lr_part<L>, dim3((a.P + 63) / 64, 12), dim3(64), 0, s, a, a.gblk, a.gvec);
+    hipLaunchKernelGGL(k_lr_prop2<L>, dim3((a.T + 63) / 64, 12), dim3(64), 0, s, a, a.gblk);
 }
 template <int L>
 static void launch_mp(int n, int chunk, const double4* p, double* partial, int nblk, hipStream_t s) {
     hipLaunchKernelGGL(k_multipole_partial<L>, dim3(nblk), dim3(64), 0, s, n, chunk, p, partial);
 }
 template <int L>
-static void launch_md(const MoveRec* r, double* M, hipStream_t s) {
-    hipLaunchKernelGGL(k_multipole_delta<L>, dim3(1), dim3(64), 0, s, r, M);
+static void launch_md(TailArgs a, hipStream_t s) {
+    hipLaunchKernelGGL(k_accept_tail<L>, dim3(1), dim3(256), 0, s, a);
 }
 
 struct OrderFns {
@@ -991,6 +1297,11 @@ static bool order_fns(int L, OrderFns* f) {
 template <bool MIXED>
 static void launch_near_t(EvalArgs a, int G, hipStream_t s) {
     if (a.n <= 0) return;
+    if (G == 27) {
+        const long long threads = (long long)a.n * 32;
+        hipLaunchKernelGGL((k_near_cells<MIXED>), dim3((int)((threads + 255) / 256)), dim3(256), 0, s, a);
+        return;
+    }
     const long long threads = (long long)a.n * G;
     const int blocks = (int)((threads + 255) / 256);
     switch (G) {
@@ -1038,6 +1349,7 @@ struct kmc_ctx {
     int occmax = 0;
     int near_lanes = 8;
     int sweep_block = 256;
+    int sweep_pairs = 2;
     int64_t last_P = 0, last_T = 0;   // sizes of the last propose (kernel timing)
     bool have_particles = false, have_lr = false, initialised = false;
     OrderFns fns{};
@@ -1055,7 +1367,7 @@ struct kmc_ctx {
     // expansions
     DBuf<double2> lexp;
     // long range
-    DBuf<double> M, gvec, gdat, elr;
+    DBuf<double> M, gvec, gdat, elr, gblk, aold, cold;
     DBuf<int64_t> gptr;
     DBuf<int32_t> gidx;
     int64_t gnnz = 0;
@@ -1088,8 +1400,8 @@ static int ensure_eval(kmc_ctx* c, int64_t T, int64_t P) {
     HIPCHK(c->eexcl.ensure(ne));
     HIPCHK(c->owner.ensure((size_t)std::max<int64_t>(T, 1)));
     HIPCHK(c->UD.ensure(ne));
-    HIPCHK(c->UI.ensure(ne));
-    HIPCHK(c->lrd.ensure((size_t)std::max<int64_t>(T, 1)));
+    HIPCHK(c->UI.ensure(ne * (size_t)((c->L + 1) / 2)));
+    HIPCHK(c->lrd.ensure((size_t)12 * std::max<int64_t>(T, 1)));
     return KMC_OK;
 }
 
@@ -1109,6 +1421,26 @@ static EvalArgs eval_args(kmc_ctx* c, int n, const double4* pos4, const int32_t*
     return a;
 }
 
+static int ensure_lr_work(kmc_ctx* c, int64_t P) {
+    HIPCHK(c->aold.ensure((size_t)P * 2 * c->nh));
+    HIPCHK(c->cold.ensure((size_t)12 * P));
+    return KMC_OK;
+}
+
+static LR2Args lr_args(kmc_ctx* c, int64_t P, int64_t T) {
+    LR2Args la;
+    la.P = (int)P;
+    la.T = (int)T;
+    la.gblk = c->gblk.p;
+    la.gvec = c->gvec.p;
+    la.epos4 = c->epos4.p;
+    la.owner = c->owner.p;
+    la.aold = c->aold.p;
+    la.cpart = c->cold.p;
+    la.lpart = c->lrd.p;
+    return la;
+}
+
 // propose kernels after the evaluation arrays are filled (T new points, then P old)
 static int run_propose_core(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, double* d_out, const int64_t* rate_loc,
                             bool mixed) {
@@ -1123,17 +1455,9 @@ static int run_propose_core(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, dou
     }
     const bool lr = (c->bc == KMC_BC_PBC || c->bc == KMC_BC_FF_ONLY) && c->have_lr;
     if (lr) {
-        LRArgs la;
-        la.T = (int)T;
-        la.nrow = 2 * c->nh;
-        la.gdat = c->gdat.p;
-        la.gptr = c->gptr.p;
-        la.gidx = c->gidx.p;
-        la.gvec = c->gvec.p;
-        la.epos4 = c->epos4.p;
-        la.owner = c->owner.p;
-        la.lrd = c->lrd.p;
-        c->fns.lr(la, s);
+        int rc = ensure_lr_work(c, P);
+        if (rc) return rc;
+        c->fns.lr(lr_args(c, P, T), s);
     }
     CombineArgs ca;
     ca.g = c->geom();
@@ -1144,7 +1468,10 @@ static int run_propose_core(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, dou
     ca.owner = c->owner.p;
     ca.UD = c->UD.p;
     ca.UI = c->UI.p;
-    ca.lrd = lr ? c->lrd.p : nullptr;
+    ca.nfar = (c->L + 1) / 2;
+    ca.lpart = lr ? c->lrd.p : nullptr;
+    ca.cpart = lr ? c->cold.p : nullptr;
+    ca.P = (int)P;
     ca.rate_loc = rate_loc;
     ca.out = d_out;
     if (T > 0) {
@@ -1180,7 +1507,7 @@ static int launch_setup(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, const i
     return KMC_OK;
 }
 
-static int run_accept_tail(kmc_ctx* c, hipStream_t s) {
+static int run_accept_tail(kmc_ctx* c, hipStream_t s, int64_t* d_counter) {
     if (c->bc != KMC_BC_FF_ONLY) {
         SweepArgs sa;
         sa.g = c->geom();
@@ -1191,19 +1518,26 @@ static int run_accept_tail(kmc_ctx* c, hipStream_t s) {
         sa.src_pos4 = nullptr;
         sa.src_cell = nullptr;
         sa.src_begin = sa.src_end = 0;
-        c->fns.sweep(sa, (int)c->ncells, s, c->sweep_block);
-        hipLaunchKernelGGL(k_accept_store, dim3(1), dim3(1), 0, s, c->rec.p, c->occ.p, c->slot_pos.p, c->slot_id.p,
-                           c->occmax, c->ppos4.p, c->pcell.p, c->flags.p);
-    } else {
-        // no cell tables in 'ff-only': only the particle arrays move
-        hipLaunchKernelGGL(k_accept_store, dim3(1), dim3(1), 0, s, c->rec.p, c->occ.p, c->slot_pos.p, c->slot_id.p,
-                           c->occmax, c->ppos4.p, c->pcell.p, c->flags.p);
+        c->fns.sweep(sa, (int)c->ncells, s, c->sweep_block, c->sweep_pairs);
     }
-    if ((c->bc == KMC_BC_PBC || c->bc == KMC_BC_FF_ONLY) && c->have_lr) {
-        c->fns.md(c->rec.p, c->M.p, s);
-        hipLaunchKernelGGL(k_lr_refresh, dim3(1), dim3(256), 0, s, 2 * c->nh, c->gdat.p, c->gptr.p, c->gidx.p, c->M.p,
-                           c->gvec.p, c->elr.p);
-    }
+    TailArgs ta;
+    ta.rec = c->rec.p;
+    ta.occ = c->occ.p;
+    ta.slot_pos = c->slot_pos.p;
+    ta.slot_id = c->slot_id.p;
+    ta.occmax = c->occmax;
+    ta.ppos4 = c->ppos4.p;
+    ta.pcell = c->pcell.p;
+    ta.flags = c->flags.p;
+    ta.lr = ((c->bc == KMC_BC_PBC || c->bc == KMC_BC_FF_ONLY) && c->have_lr) ? 1 : 0;
+    ta.M = c->M.p;
+    ta.gdat = c->gdat.p;
+    ta.gptr = c->gptr.p;
+    ta.gidx = c->gidx.p;
+    ta.g = c->gvec.p;
+    ta.elr = c->elr.p;
+    ta.counter = d_counter;
+    c->fns.md(ta, s);
     HIPCHK(hipGetLastError());
     return KMC_OK;
 }
@@ -1301,6 +1635,7 @@ int kmc_destroy(kmc_ctx* c) {
     if (!c) return KMC_OK;
     c->ppos4.release(); c->pcell.release(); c->occ.release(); c->slot_pos.release(); c->slot_id.release();
     c->lexp.release(); c->M.release(); c->gvec.release(); c->gdat.release(); c->elr.release(); c->gptr.release();
+    c->gblk.release(); c->aold.release(); c->cold.release();
     c->gidx.release(); c->energy.release(); c->flags.release(); c->err.release(); c->rec.release();
     c->epos4.release(); c->ecell.release(); c->eexcl.release(); c->owner.release(); c->UD.release();
     c->UI.release(); c->lrd.release(); c->dE1.release(); c->one_pid.release(); c->one_es.release();
@@ -1311,14 +1646,18 @@ int kmc_destroy(kmc_ctx* c) {
     return KMC_OK;
 }
 
-int kmc_set_tiles(kmc_ctx* c, int32_t near_lanes, int32_t sweep_block) {
+int kmc_set_tiles(kmc_ctx* c, int32_t near_lanes, int32_t sweep_block, int32_t sweep_pairs) {
     if (check_ctx(c)) return KMC_E_ARG;
-    if (near_lanes != 1 && near_lanes != 4 && near_lanes != 8 && near_lanes != 16 && near_lanes != 32 && near_lanes != 64)
-        return set_err(KMC_E_ARG, "near_lanes must be one of 1,4,8,16,32,64");
-    if (sweep_block < 64 || sweep_block > 1024 || sweep_block % 64)
-        return set_err(KMC_E_ARG, "sweep_block must be a multiple of 64 in [64,1024]");
+    if (near_lanes != 1 && near_lanes != 4 && near_lanes != 8 && near_lanes != 16 && near_lanes != 27 &&
+        near_lanes != 32 && near_lanes != 64)
+        return set_err(KMC_E_ARG, "near_lanes must be one of 1,4,8,16,27,32,64 (27: one lane per neighbour leaf)");
+    if (sweep_block < 64 || sweep_block > 256 || sweep_block % 64)
+        return set_err(KMC_E_ARG, "sweep_block must be 64, 128, 192 or 256");
+    if (sweep_pairs < 1 || sweep_pairs > 3 || (sweep_pairs == 3 && c->L > 12))
+        return set_err(KMC_E_ARG, "sweep_pairs must be 1 or 2 (3 for L <= 12)");
     c->near_lanes = near_lanes;
     c->sweep_block = sweep_block;
+    c->sweep_pairs = sweep_pairs;
     return KMC_OK;
 }
 
@@ -1394,6 +1733,37 @@ int kmc_set_lr_operator(kmc_ctx* c, int64_t nnz, const double* h_data, const int
         HIPCHK(hipMemcpy(c->gidx.p, idx.data(), nnz * sizeof(int32_t), hipMemcpyHostToDevice));
     }
     HIPCHK(hipMemcpy(c->gptr.p, h_indptr, (nrow + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+    // dense diagonal blocks in the kernels' order; everything outside them must vanish
+    {
+        std::vector<double> dense((size_t)nrow * nrow, 0.0);
+        for (int r = 0; r < nrow; r++)
+            for (int64_t k = h_indptr[r]; k < h_indptr[r + 1]; k++) dense[(size_t)r * nrow + h_indices[k]] += h_data[k];
+        const LRBlocks T = make_lr_blocks(c->L);
+        std::vector<int> blk(nrow, -1);
+        size_t total = 0;
+        for (int b = 0; b < 6; b++) {
+            for (int i = 0; i < T.n[b]; i++)
+                for (int comp = 0; comp < 2; comp++) blk[2 * hidx(T.l[b][i], T.m[b][i]) + comp] = 2 * b + comp;
+            total += 2 * (size_t)T.n[b] * T.n[b];
+        }
+        for (int r = 0; r < nrow; r++)
+            for (int q = 0; q < nrow; q++)
+                if (dense[(size_t)r * nrow + q] != 0.0 && blk[r] != blk[q])
+                    return set_err(KMC_E_ARG, "long-range operator couples entries outside the cubic-symmetry blocks");
+        std::vector<double> gb(std::max<size_t>(total, 1), 0.0);
+        for (int b = 0; b < 6; b++) {
+            const int nb = T.n[b];
+            for (int comp = 0; comp < 2; comp++)
+                for (int i = 0; i < nb; i++)
+                    for (int j = 0; j < nb; j++) {
+                        const int ri = 2 * hidx(T.l[b][i], T.m[b][i]) + comp;
+                        const int rj = 2 * hidx(T.l[b][j], T.m[b][j]) + comp;
+                        gb[T.off[b] + (size_t)comp * nb * nb + (size_t)i * nb + j] = dense[(size_t)ri * nrow + rj];
+                    }
+        }
+        HIPCHK(c->gblk.ensure(gb.size()));
+        HIPCHK(hipMemcpy(c->gblk.p, gb.data(), gb.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     c->gnnz = nnz;
     c->have_lr = true;
     return KMC_OK;
@@ -1421,7 +1791,7 @@ int kmc_initialise(kmc_ctx* c, void* stream, double* h_energy) {
             sa.src_cell = c->pcell.p;
             sa.src_begin = b;
             sa.src_end = std::min(n, b + chunk);
-            c->fns.sweep(sa, (int)c->ncells, s, c->sweep_block);
+            c->fns.sweep(sa, (int)c->ncells, s, c->sweep_block, c->sweep_pairs);
             HIPCHK(hipGetLastError());
         }
     }
@@ -1445,7 +1815,7 @@ int kmc_initialise(kmc_ctx* c, void* stream, double* h_energy) {
     DBuf<double> ud, ui, esr;
     HIPCHK(excl.ensure(n));
     HIPCHK(ud.ensure(n));
-    HIPCHK(ui.ensure(n));
+    HIPCHK(ui.ensure((size_t)n * ((c->L + 1) / 2)));
     HIPCHK(esr.ensure(1));
     {
         std::vector<int32_t> ids(n);
@@ -1457,7 +1827,7 @@ int kmc_initialise(kmc_ctx* c, void* stream, double* h_energy) {
             launch_near(an, c->near_lanes, false, s);
             EvalArgs af = eval_args(c, n, c->ppos4.p, c->pcell.p, excl.p, ui.p);
             c->fns.far(af, s);
-            hipLaunchKernelGGL(k_half_sum, dim3(1), dim3(256), 0, s, n, ud.p, ui.p, esr.p);
+            hipLaunchKernelGGL(k_half_sum, dim3(1), dim3(256), 0, s, n, ud.p, ui.p, (c->L + 1) / 2, esr.p);
         }
         hipLaunchKernelGGL(k_finish_energy, dim3(1), dim3(1), 0, s, esr.p, c->elr.p, c->eu, c->bc, c->energy.p);
         HIPCHK(hipGetLastError());
@@ -1652,7 +2022,7 @@ int kmc_accept(kmc_ctx* c, void* stream, int64_t pid, const double* h_new_pos, d
     if (rc) return rc;
     hipLaunchKernelGGL(k_prepare_record, dim3(1), dim3(1), 0, s, c->geom(), (int)pid, h_new_pos[0], h_new_pos[1],
                        h_new_pos[2], c->ppos4.p, c->pcell.p, c->dE1.p, c->energy.p, c->rec.p);
-    rc = run_accept_tail(c, s);
+    rc = run_accept_tail(c, s, nullptr);
     if (rc) return rc;
     int fl = 0;
     HIPCHK(hipMemcpyAsync(&fl, c->flags.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1704,9 +2074,8 @@ int kmc_step(kmc_ctx* c, void* stream, int64_t P, int64_t T, const int64_t* d_pi
     sa.rec = c->rec.p;
     sa.flags = c->flags.p;
     hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, s, sa);
-    rc = run_accept_tail(c, s);
+    rc = run_accept_tail(c, s, d_counter);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_counter_inc, dim3(1), dim3(1), 0, s, d_counter);
     HIPCHK(hipGetLastError());
     return KMC_OK;
 }
@@ -1759,7 +2128,7 @@ int kmc_time_kernel(kmc_ctx* c, void* stream, int32_t which, int32_t reps, int32
             sa.src_pos4 = nullptr;
             sa.src_cell = nullptr;
             sa.src_begin = sa.src_end = 0;
-            c->fns.sweep(sa, (int)c->ncells, s, c->sweep_block);
+            c->fns.sweep(sa, (int)c->ncells, s, c->sweep_block, c->sweep_pairs);
         } else if (which == 1) {
             EvalArgs an = eval_args(c, (int)(T + P), c->epos4.p, c->ecell.p, c->eexcl.p, c->UD.p);
             launch_near(an, c->near_lanes, mixed, s);
@@ -1767,17 +2136,7 @@ int kmc_time_kernel(kmc_ctx* c, void* stream, int32_t which, int32_t reps, int32
             EvalArgs af = eval_args(c, (int)(T + P), c->epos4.p, c->ecell.p, c->eexcl.p, c->UI.p);
             c->fns.far(af, s);
         } else if (which == 3) {
-            LRArgs la;
-            la.T = (int)T;
-            la.nrow = 2 * c->nh;
-            la.gdat = c->gdat.p;
-            la.gptr = c->gptr.p;
-            la.gidx = c->gidx.p;
-            la.gvec = c->gvec.p;
-            la.epos4 = c->epos4.p;
-            la.owner = c->owner.p;
-            la.lrd = c->lrd.p;
-            if (c->have_lr) c->fns.lr(la, s);
+            if (c->have_lr && (size_t)(12 * P) <= c->cold.n) c->fns.lr(lr_args(c, P, T), s);
         } else {
             return set_err(KMC_E_ARG, "unknown kernel id");
         }
@@ -1817,7 +2176,7 @@ int kmc_sweep_hop(kmc_ctx* c, void* stream, double q, const double* h_old, const
     sa.src_pos4 = nullptr;
     sa.src_cell = nullptr;
     sa.src_begin = sa.src_end = 0;
-    c->fns.sweep(sa, (int)c->ncells, s, c->sweep_block);
+    c->fns.sweep(sa, (int)c->ncells, s, c->sweep_block, c->sweep_pairs);
     HIPCHK(hipGetLastError());
     return KMC_OK;
 }

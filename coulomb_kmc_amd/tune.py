@@ -2,8 +2,10 @@
 Tile autotuner for the propose/accept kernels.
 
 Tunable shapes (set on a context with KMCFMM.set_tiles / kmc_set_tiles):
-  near_lanes   lanes per evaluation point in the near-field kernel (1, 4, 8, 16, 32, 64)
-  sweep_block  threads per block of the accept sweep (multiple of 64, 64..1024)
+  near_lanes   lanes per evaluation point in the near-field kernel (1, 4, 8, 16, 32, 64;
+               27 = one lane per neighbour leaf)
+  sweep_block  threads per block of the accept sweep (64, 128, 192, 256)
+  sweep_pairs  column pairs per lane of the accept sweep (1, 2; 3 for L <= 12)
 
 ``autotune(kmc)`` times every candidate with HIP events (kmc_time_kernel) on the
 context's current state and returns the fastest; ``choose_tiles`` reads the table of
@@ -18,9 +20,9 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(_HERE, "tiles.json")
 
-NEAR_LANES = (1, 4, 8, 16, 32, 64)
-SWEEP_BLOCKS = (64, 128, 256, 512, 1024)
-DEFAULT = {"near_lanes": 8, "sweep_block": 256}
+NEAR_LANES = (1, 4, 8, 16, 27, 32, 64)
+SWEEP_BLOCKS = (64, 128, 256)
+DEFAULT = {"near_lanes": 27, "sweep_block": 256, "sweep_pairs": 2}
 
 
 def _key(L, T):
@@ -38,14 +40,17 @@ def load_table():
     return {}
 
 
-def choose_tiles(L, T, near_lanes=None, sweep_block=None):
+def choose_tiles(L, T, near_lanes=None, sweep_block=None, sweep_pairs=None):
     t = dict(DEFAULT)
     t.update(load_table().get(_key(L, T), {}))
     if near_lanes:
         t["near_lanes"] = int(near_lanes)
     if sweep_block:
         t["sweep_block"] = int(sweep_block)
-    return {"near_lanes": int(t["near_lanes"]), "sweep_block": int(t["sweep_block"])}
+    if sweep_pairs:
+        t["sweep_pairs"] = int(sweep_pairs)
+    return {"near_lanes": int(t["near_lanes"]), "sweep_block": int(t["sweep_block"]),
+            "sweep_pairs": int(t["sweep_pairs"])}
 
 
 def autotune(kmc, T, reps=10, precision="mixed", save=False, log=None):
@@ -53,16 +58,18 @@ def autotune(kmc, T, reps=10, precision="mixed", save=False, log=None):
     timings = {"near": {}, "sweep": {}}
     cur = choose_tiles(kmc.L, T)
     for g in NEAR_LANES:
-        kmc.set_tiles(near_lanes=g, sweep_block=cur["sweep_block"])
+        kmc.set_tiles(near_lanes=g, sweep_block=cur["sweep_block"], sweep_pairs=cur["sweep_pairs"])
         kmc.time_kernel("near", reps=2, precision=precision)
         timings["near"][g] = kmc.time_kernel("near", reps=reps, precision=precision)
     best_g = min(timings["near"], key=timings["near"].get)
-    for b in SWEEP_BLOCKS:
-        kmc.set_tiles(near_lanes=best_g, sweep_block=b)
-        kmc.time_kernel("sweep", reps=1)
-        timings["sweep"][b] = kmc.time_kernel("sweep", reps=reps)
-    best_b = min(timings["sweep"], key=timings["sweep"].get)
-    choice = {"near_lanes": int(best_g), "sweep_block": int(best_b)}
+    for npair in ((1, 2, 3) if kmc.L <= 12 else (1, 2)):
+        for b in SWEEP_BLOCKS:
+            kmc.set_tiles(near_lanes=best_g, sweep_block=b, sweep_pairs=npair)
+            kmc.time_kernel("sweep", reps=1)
+            timings["sweep"]["{}x{}".format(npair, b)] = kmc.time_kernel("sweep", reps=reps)
+    best = min(timings["sweep"], key=timings["sweep"].get)
+    best_np, best_b = (int(x) for x in best.split("x"))
+    choice = {"near_lanes": int(best_g), "sweep_block": best_b, "sweep_pairs": best_np}
     kmc.set_tiles(**choice)
     if log:
         log("autotune", _key(kmc.L, T), choice, timings)

@@ -160,10 +160,12 @@ int kmc_check_flags(kmc_ctx* ctx, void* stream, int32_t* h_flags);
 
 /*
  * Tile choices of the hot kernels (filled by the autotuner in coulomb_kmc_amd/tune.py):
- * near_lanes = lanes per evaluation point in the near-field kernel (1,4,8,16,32,64),
- * sweep_block = threads per block of the accept sweep (64..1024, multiple of 64).
+ * near_lanes = lanes per evaluation point in the near-field kernel (1,4,8,16,32,64, or
+ * 27 = one lane per neighbour leaf), sweep_block = threads per block of the accept
+ * sweep (64, 128, 192, 256), sweep_pairs = column pairs (m, L-1-m) per lane of
+ * the accept sweep (1, 2; 3 for L <= 12).
  */
-int kmc_set_tiles(kmc_ctx* ctx, int32_t near_lanes, int32_t sweep_block);
+int kmc_set_tiles(kmc_ctx* ctx, int32_t near_lanes, int32_t sweep_block, int32_t sweep_pairs);
 
 /* Accept sweep only, for a hop of charge q from h_old to h_new (timing / tests). */
 int kmc_sweep_hop(kmc_ctx* ctx, void* stream, double q, const double* h_old_pos, const double* h_new_pos);
