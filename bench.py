@@ -97,6 +97,93 @@ def metropolis_pick(dE, u, kT=1.0):
     return min(int(np.searchsorted(c, u * c[-1], side="right")), len(dE) - 1)
 
 
+COPY_PROBE_BYTES = 512 * 2 ** 20
+
+
+def probe_traffic(kmc, cfg, pids_all, disp_all):
+    """Launches for the HBM-traffic counters: one propose (sizes the evaluation arrays),
+    5 accept sweeps, then 5 device copies of COPY_PROBE_BYTES (calibration of the
+    counters against a kernel whose traffic is known: read + write of that size)."""
+    import torch
+    P, k = cfg["P"], cfg["k"]
+    T = P * k
+    counter = torch.zeros(1, dtype=torch.int64, device="cuda")
+    t_pids = torch.as_tensor(pids_all, dtype=torch.int64).cuda()
+    t_disp = torch.as_tensor(disp_all, dtype=torch.float64).cuda()
+    d_pid = torch.zeros(P, dtype=torch.int64, device="cuda")
+    d_es = torch.zeros(P + 1, dtype=torch.int64, device="cuda")
+    d_np = torch.zeros(T * 3, dtype=torch.float64, device="cuda")
+    d_dE = torch.zeros(T, dtype=torch.float64, device="cuda")
+    kmc.materialize_moves(P, k, t_pids, t_disp, counter, d_pid, d_es, d_np)
+    kmc.propose_device(d_pid, d_es, d_np, d_dE, precision="mixed")    # sizes the work buffers
+    kmc.step_device(d_pid, d_es, d_np, d_dE, torch.full((1,), 0.5, dtype=torch.float64, device="cuda"),
+                    1.0, counter, torch.zeros(1, dtype=torch.int64, device="cuda"), precision="mixed")
+    torch.cuda.synchronize()
+    ms = kmc.time_kernel("sweep", reps=5)
+    a = torch.ones(COPY_PROBE_BYTES // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log("probe: sweep", ms, "ms per launch; copy probe", COPY_PROBE_BYTES, "bytes x 5")
+
+
+def _pmc_rows(d, counter):
+    import csv
+    import glob
+    rows = []
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                if r.get("Counter_Name") == counter:
+                    rows.append((r["Kernel_Name"], r.get("Dispatch_Id"), float(r["Counter_Value"])))
+    if not rows:
+        raise SystemExit("no {} rows under {}".format(counter, d))
+    return rows
+
+
+def _per_dispatch(rows):
+    acc = {}
+    for name, disp, v in rows:   # one row per dispatch (values summed over instances)
+        acc.setdefault((name, disp), 0.0)
+        acc[(name, disp)] += v
+    return acc
+
+
+def traffic_from_pmc(config, fetch_dir, write_dir):
+    """HBM bytes per accept-sweep launch from FETCH_SIZE / WRITE_SIZE (KiB per dispatch),
+    each scaled by bytes / counter of the copy probe (which reads and writes
+    COPY_PROBE_BYTES)."""
+    cfg = CONFIGS[config]
+    res = {}
+    for which, d, counter in (("fetch", fetch_dir, "FETCH_SIZE"), ("write", write_dir, "WRITE_SIZE")):
+        per = _per_dispatch(_pmc_rows(d, counter))
+        # the last 5 sweeps are the timed accept sweeps (initialise also launches k_sweep)
+        sweep = [v for (n, i), v in sorted(per.items(), key=lambda kv: int(kv[0][1] or 0))
+                 if n.startswith("void k_sweep")][-5:]
+        big = sorted(((v, n) for (n, _), v in per.items() if not n.startswith("void k_")), reverse=True)
+        copies = [v for v, n in big if v * 1024 > 0.5 * COPY_PROBE_BYTES]
+        if not sweep or not copies:
+            raise SystemExit("missing k_sweep or copy probe dispatches under " + d)
+        kib = float(np.mean(sweep))
+        cal = COPY_PROBE_BYTES / (float(np.median(copies)) * 1024.0)
+        res[which] = {"counter": counter, "k_sweep_KiB_per_dispatch": kib, "dispatches": len(sweep),
+                      "copy_probe_KiB": float(np.median(copies)), "scale": cal,
+                      "bytes_per_launch": kib * 1024.0 * cal}
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    tab = {}
+    if os.path.exists(path):
+        with open(path) as f:
+            tab = json.load(f)
+    key = "k_sweep|" + cfg["workload"]
+    tab[key] = {"bytes_per_launch": res["fetch"]["bytes_per_launch"] + res["write"]["bytes_per_launch"],
+                "fetch": res["fetch"], "write": res["write"],
+                "copy_probe_bytes": COPY_PROBE_BYTES}
+    with open(path, "w") as f:
+        json.dump(tab, f, indent=1, sort_keys=True)
+    print(json.dumps({key: tab[key]}, indent=1))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,7 +194,16 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--near-lanes", type=int, default=0, help="0: autotuned")
+    ap.add_argument("--probe-traffic", action="store_true",
+                    help="only launch the accept sweep and a copy of known size (for rocprofv3 --pmc runs)")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="read two rocprofv3 --pmc outputs (FETCH_SIZE, WRITE_SIZE) of --probe-traffic, "
+                         "write profiles/traffic.json and exit (no GPU)")
     args = ap.parse_args()
+
+    if args.traffic_from:
+        traffic_from_pmc(args.config, *args.traffic_from)
+        return
 
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (one MI355X per box); --gpus {} is not supported"
@@ -133,7 +229,7 @@ def main():
     log("move stream", round(time.time() - t0, 2), "s")
 
     kmc = KMCFMM(pos0.copy(), q.reshape(-1, 1), E, r=r, l=L, boundary_condition=bc, max_move=cfg["radius"])
-    tiles = tune.choose_tiles(L, T, near_lanes=args.near_lanes or None)
+    tiles = tune.choose_tiles(L, T, (2 ** (r - 1)) ** 3, near_lanes=args.near_lanes or None)
     kmc.set_tiles(**tiles)
     torch.cuda.synchronize()
     t0 = time.time()
@@ -142,6 +238,9 @@ def main():
     t_init = time.time() - t0
     e_init = kmc.energy
     log("initialise", round(t_init, 2), "s, energy", e_init)
+    if args.probe_traffic:
+        probe_traffic(kmc, cfg, pids_all, disp_all)
+        return
 
     dev = lambda x, t: torch.as_tensor(np.ascontiguousarray(x), dtype=t).cuda()
     t_pids = dev(pids_all, torch.int64)

@@ -105,7 +105,7 @@ def lib():
         "kmc_step": (ctypes.c_int, [P, P, I64, I64, P, P, P, P, I64, P, D, P, P, I32]),
         "kmc_materialize_moves": (ctypes.c_int, [P, P, I64, I64, I64, P, P, P, P, P, P]),
         "kmc_check_flags": (ctypes.c_int, [P, P, P]),
-        "kmc_set_tiles": (ctypes.c_int, [P, I32, I32, I32]),
+        "kmc_set_tiles": (ctypes.c_int, [P, I32, I32, I32, I32]),
         "kmc_sweep_hop": (ctypes.c_int, [P, P, D, P, P]),
         "kmc_time_kernel": (ctypes.c_int, [P, P, I32, I32, I32, P]),
     }

@@ -217,20 +217,34 @@ constexpr RecTab2 make_rectab2() {
 }
 static constexpr RecTab2 kRec2 = make_rectab2();
 
-// one column m of one source-image: acc[k] += e_{m+k} * W, k = 0 .. L-m-1
-template <int L, int M>
-__device__ __forceinline__ void sweep_column(double zr, double ir2, Cplx W, double* ar, double* ai) {
-    double e2 = 0.0, e1 = 1.0;
-    ar[0] += W.re;
-    ai[0] += W.im;
+// one column m of NI source-images: acc[k] += e_{m+k} * W_j, k = 0 .. L-m-1, j = 0 .. NI-1.
+// The NI recurrences are independent dependency chains (instruction-level
+// parallelism); each accumulator still receives the images in order j = 0, 1, ...
+template <int L, int M, int NI>
+__device__ __forceinline__ void sweep_column(const double* zr, const double* ir2, const Cplx* W, double* ar,
+                                             double* ai) {
+    double e2[NI], e1[NI];
+#pragma unroll
+    for (int j = 0; j < NI; j++) {
+        e2[j] = 0.0;
+        e1[j] = 1.0;
+        ar[0] += W[j].re;
+        ai[0] += W[j].im;
+    }
 #pragma unroll
     for (int k = 1; k < L - M; k++) {
         const int l = M + k;
-        const double e = fma(kRec2.a2[l][M] * zr, e1, -ir2 * e2);
-        e2 = e1;
-        e1 = e;
-        ar[k] = fma(e, W.re, ar[k]);
-        ai[k] = fma(e, W.im, ai[k]);
+#pragma unroll
+        for (int j = 0; j < NI; j++) {
+            const double e = fma(kRec2.a2[l][M] * zr[j], e1[j], -ir2[j] * e2[j]);
+            e2[j] = e1[j];
+            e1[j] = e;
+        }
+#pragma unroll
+        for (int j = 0; j < NI; j++) {
+            ar[k] = fma(e1[j], W[j].re, ar[k]);
+            ai[k] = fma(e1[j], W[j].im, ai[k]);
+        }
     }
 }
 
@@ -246,26 +260,35 @@ __device__ __forceinline__ void sweep_store_column(double2* le, const double* ar
     }
 }
 
-
-// columns (P, L-1-P) of pair P into accumulator row ar/ai (first L-P entries: column P)
-template <int L, int P>
-__device__ __forceinline__ void sweep_pair_cols(double zr, double ir2, Cplx u, double base, double* ar, double* ai) {
+// columns (P, L-1-P) of pair P, NI images, into accumulator row ar/ai (first L-P entries: column P)
+template <int L, int P, int NI>
+__device__ __forceinline__ void sweep_pair_cols(const double* zr, const double* ir2, const Cplx* u,
+                                                const double* base, double* ar, double* ai) {
     constexpr int M1 = P, M2 = L - 1 - P;
-    const Cplx U1 = cpow<M1>(u);
-    const double f1 = base * kRec.s[M1];
-    sweep_column<L, M1>(zr, ir2, Cplx{f1 * U1.re, f1 * U1.im}, ar, ai);
+    Cplx U1[NI], W[NI];
+#pragma unroll
+    for (int j = 0; j < NI; j++) {
+        U1[j] = cpow<M1>(u[j]);
+        const double f = base[j] * kRec.s[M1];
+        W[j] = Cplx{f * U1[j].re, f * U1[j].im};
+    }
+    sweep_column<L, M1, NI>(zr, ir2, W, ar, ai);
     if constexpr (M2 != M1) {
-        const Cplx U2 = cpow<M2>(u);
-        const double f2 = base * kRec.s[M2];
-        sweep_column<L, M2>(zr, ir2, Cplx{f2 * U2.re, f2 * U2.im}, ar + (L - M1), ai + (L - M1));
+#pragma unroll
+        for (int j = 0; j < NI; j++) {
+            const Cplx U2 = cmul(U1[j], cpow<M2 - M1>(u[j]));
+            const double f = base[j] * kRec.s[M2];
+            W[j] = Cplx{f * U2.re, f * U2.im};
+        }
+        sweep_column<L, M2, NI>(zr, ir2, W, ar + (L - M1), ai + (L - M1));
     }
 }
-template <int L, int P0, int NP, int J = 0>
-__device__ __forceinline__ void sweep_group_cols(double zr, double ir2, Cplx u, double base, double (*ar)[L + 1],
-                                                 double (*ai)[L + 1]) {
+template <int L, int P0, int NP, int NI, int J = 0>
+__device__ __forceinline__ void sweep_group_cols(const double* zr, const double* ir2, const Cplx* u,
+                                                 const double* base, double (*ar)[L + 1], double (*ai)[L + 1]) {
     if constexpr (J < NP && P0 + J < (L + 1) / 2) {
-        sweep_pair_cols<L, P0 + J>(zr, ir2, u, base, ar[J], ai[J]);
-        sweep_group_cols<L, P0, NP, J + 1>(zr, ir2, u, base, ar, ai);
+        sweep_pair_cols<L, P0 + J, NI>(zr, ir2, u, base, ar[J], ai[J]);
+        sweep_group_cols<L, P0, NP, NI, J + 1>(zr, ir2, u, base, ar, ai);
     }
 }
 template <int L, int P0, int NP, int J = 0>
@@ -278,25 +301,41 @@ __device__ __forceinline__ void sweep_group_store(double2* le, double (*ar)[L + 
     }
 }
 
-// pairs P = G*NP .. G*NP+NP-1 of columns (P, L-1-P) for one leaf (one lane)
-template <int L, int G, int NP>
-__device__ __forceinline__ void sweep_group(const SweepArgs& a, int cell) {
-    constexpr int NPAIR = (L + 1) / 2;
-    constexpr int P0 = G * NP;
-    constexpr int W = L + 1;                 // accumulator entries per pair
-    double ar[NP][W], ai[NP][W];
+// NI source images (displacement leaf centre -> image, charge or 0 when not well separated)
+template <int L, int P0, int NP, int NI>
+__device__ __forceinline__ void sweep_images(const double* dx, double dy, double dz, const double* qe,
+                                             double (*ar)[L + 1], double (*ai)[L + 1]) {
+    double zr[NI], ir2[NI], base[NI];
+    Cplx u[NI];
 #pragma unroll
-    for (int j = 0; j < NP; j++)
-#pragma unroll
-        for (int k = 0; k < W; k++) { ar[j][k] = 0.0; ai[j][k] = 0.0; }
+    for (int j = 0; j < NI; j++) {
+        double r2 = fma(dx[j], dx[j], fma(dy, dy, dz * dz));
+        r2 = qe[j] != 0.0 ? r2 : 1.0;
+        const double ir = rsqrt_fp64(r2);
+        ir2[j] = ir * ir;
+        zr[j] = dz * ir2[j];
+        u[j] = Cplx{dx[j] * ir2[j], -dy * ir2[j]};     // conj((x+iy)/r^2)
+        base[j] = qe[j] * ir;
+    }
+    sweep_group_cols<L, P0, NP, NI>(zr, ir2, u, base, ar, ai);
+}
 
+__device__ __forceinline__ bool near1(int d) { return d < 2 && d > -2; }
+
+// pairs P = G*NP .. G*NP+NP-1 of columns (P, L-1-P) for one leaf (one lane).
+// PER ('27'/PBC): the 27 images of a source in stencil order (x fastest), of which
+// this lane takes the z-layers oz = z0 .. z1 (all three unless the images are split
+// over the waves of the workgroup); free space: the source itself.
+template <int L, int G, int NP, bool PER>
+__device__ __forceinline__ void sweep_group(const SweepArgs& a, int cell, int z0, int z1, double (*ar)[L + 1],
+                                            double (*ai)[L + 1]) {
+    constexpr int P0 = G * NP;
     const int n = a.g.ncps;
+    const double E = a.g.E;
     const int tx = cell % n, ty = (cell / n) % n, tz = cell / (n * n);
     const double cx = a.g.c0 + tx * a.g.w;
     const double cy = a.g.c0 + ty * a.g.w;
     const double cz = a.g.c0 + tz * a.g.w;
-    const bool periodic = a.g.bc != KMC_BC_FREE_SPACE;
-    const int nimg = periodic ? 27 : 1;
     int nsrc;
     if (a.mode == 0) nsrc = a.rec->valid ? 2 : 0;
     else nsrc = a.src_end - a.src_begin;
@@ -316,44 +355,98 @@ __device__ __forceinline__ void sweep_group(const SweepArgs& a, int cell) {
         }
         const int dcx = sc % n - tx, dcy = (sc / n) % n - ty, dcz = sc / (n * n) - tz;
         const double dx0 = sx - cx, dy0 = sy - cy, dz0 = sz - cz;
-        for (int k = 0; k < nimg; k++) {
-            int ox = 0, oy = 0, oz = 0;
-            if (periodic) stencil(k, ox, oy, oz);
-            const int ddx = dcx + ox * n, ddy = dcy + oy * n, ddz = dcz + oz * n;
-            const bool ws = !(ddx < 2 && ddx > -2 && ddy < 2 && ddy > -2 && ddz < 2 && ddz > -2);
-            const double dx = fma((double)ox, a.g.E, dx0);
-            const double dy = fma((double)oy, a.g.E, dy0);
-            const double dz = fma((double)oz, a.g.E, dz0);
-            double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-            r2 = ws ? r2 : 1.0;
-            const double qe = ws ? sq : 0.0;
-            const double ir = rsqrt_fp64(r2);
-            const double ir2 = ir * ir;
-            const double zr = dz * ir2;
-            const Cplx u = {dx * ir2, -dy * ir2};     // conj((x+iy)/r^2)
-            const double base = qe * ir;
-            sweep_group_cols<L, P0, NP>(zr, ir2, u, base, ar, ai);
+        if constexpr (!PER) {
+            const double dx[1] = {dx0};
+            const double qe[1] = {(near1(dcx) && near1(dcy) && near1(dcz)) ? 0.0 : sq};
+            sweep_images<L, P0, NP, 1>(dx, dy0, dz0, qe, ar, ai);
+        } else {
+            const double dxm = dx0 - E, dxp = dx0 + E;
+            const bool nxm = near1(dcx - n), nx0 = near1(dcx), nxp = near1(dcx + n);
+#pragma unroll 1
+            for (int oz = z0; oz <= z1; oz++) {
+                const double dz = fma((double)oz, E, dz0);
+                const bool nz = near1(dcz + oz * n);
+#pragma unroll 1
+                for (int oy = -1; oy <= 1; oy++) {
+                    const double dy = fma((double)oy, E, dy0);
+                    const bool nyz = nz && near1(dcy + oy * n);
+#pragma unroll 1
+                    for (int ox = -1; ox <= 1; ox++) {
+                        const double dx[1] = {ox < 0 ? dxm : (ox == 0 ? dx0 : dxp)};
+                        const bool nxs = ox < 0 ? nxm : (ox == 0 ? nx0 : nxp);
+                        const double qe[1] = {(nyz && nxs) ? 0.0 : sq};
+                        sweep_images<L, P0, NP, 1>(dx, dy, dz, qe, ar, ai);
+                    }
+                }
+            }
         }
     }
-    double2* le = a.lexp + (size_t)cell * nhalf(L);
-    sweep_group_store<L, P0, NP>(le, ar, ai);
 }
 
-template <int L, int NP, int G0>
-__device__ __forceinline__ void sweep_dispatch(const SweepArgs& a, int cell, int group) {
+template <int L, int NP, bool PER, int G0>
+__device__ __forceinline__ void sweep_dispatch(const SweepArgs& a, int cell, int group, int z0, int z1,
+                                               double (*ar)[L + 1], double (*ai)[L + 1]) {
     constexpr int NG = ((L + 1) / 2 + NP - 1) / NP;
     if constexpr (G0 < NG) {
-        if (group == G0) sweep_group<L, G0, NP>(a, cell);
-        else sweep_dispatch<L, NP, G0 + 1>(a, cell, group);
+        if (group == G0) sweep_group<L, G0, NP, PER>(a, cell, z0, z1, ar, ai);
+        else sweep_dispatch<L, NP, PER, G0 + 1>(a, cell, group, z0, z1, ar, ai);
+    }
+}
+template <int L, int NP, int G0>
+__device__ __forceinline__ void sweep_store_dispatch(double2* le, int group, double (*ar)[L + 1], double (*ai)[L + 1]) {
+    constexpr int NG = ((L + 1) / 2 + NP - 1) / NP;
+    if constexpr (G0 < NG) {
+        if (group == G0) sweep_group_store<L, G0 * NP, NP>(le, ar, ai);
+        else sweep_store_dispatch<L, NP, G0 + 1>(le, group, ar, ai);
     }
 }
 
-// grid: x = leaf blocks (lane = leaf), y = group of NP column pairs (m, L-1-m)
-template <int L, int NP>
+// Lane = leaf (threadIdx.x), column-pair group = blockIdx.y.  With SPLIT = 3 the
+// workgroup is 64 leaves x 3 waves and wave threadIdx.y takes the image layer
+// oz = threadIdx.y - 1; the two upper waves hand their sums to wave 0 through LDS
+// (fixed order), which alone updates the expansions.
+template <int L, int NP, bool PER, int SPLIT>
 __global__ void __launch_bounds__(256) k_sweep(SweepArgs a) {
-    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
-    if (cell >= a.ncells) return;
-    sweep_dispatch<L, NP, 0>(a, cell, (int)blockIdx.y);
+    constexpr int W = L + 1;                 // accumulator entries per pair
+    const int cell0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = cell0 < a.ncells;
+    const int cell = live ? cell0 : a.ncells - 1;
+    const int group = (int)blockIdx.y;
+    double ar[NP][W], ai[NP][W];
+#pragma unroll
+    for (int j = 0; j < NP; j++)
+#pragma unroll
+        for (int k = 0; k < W; k++) { ar[j][k] = 0.0; ai[j][k] = 0.0; }
+    if constexpr (SPLIT == 1) {
+        sweep_dispatch<L, NP, PER, 0>(a, cell, group, -1, 1, ar, ai);
+    } else {
+        static_assert(SPLIT == 3 && PER, "image split needs the 27-image stencil");
+        __shared__ double red[2][NP][2 * W][64];
+        const int sl = (int)threadIdx.y;
+        sweep_dispatch<L, NP, PER, 0>(a, cell, group, sl - 1, sl - 1, ar, ai);
+        if (sl > 0) {
+#pragma unroll
+            for (int j = 0; j < NP; j++)
+#pragma unroll
+                for (int k = 0; k < W; k++) {
+                    red[sl - 1][j][2 * k][threadIdx.x] = ar[j][k];
+                    red[sl - 1][j][2 * k + 1][threadIdx.x] = ai[j][k];
+                }
+        }
+        __syncthreads();
+        if (sl > 0) return;
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+#pragma unroll
+            for (int j = 0; j < NP; j++)
+#pragma unroll
+                for (int k = 0; k < W; k++) {
+                    ar[j][k] += red[u][j][2 * k][threadIdx.x];
+                    ai[j][k] += red[u][j][2 * k + 1][threadIdx.x];
+                }
+    }
+    if (!live) return;
+    sweep_store_dispatch<L, NP, 0>(a.lexp + (size_t)cell * nhalf(L), group, ar, ai);
 }
 
 // ============================================================================
@@ -623,6 +716,8 @@ struct CombineArgs {
     int P;
     const int64_t* rate_loc;     // output index, or null (identity)
     double* out;
+    double kT;
+    double* rates;               // Metropolis rates min(1, exp(-dE/kT)) per proposal, or null
 };
 
 __device__ __forceinline__ double sum_parts(const double* v, int nparts, size_t stride, size_t i) {
@@ -631,8 +726,14 @@ __device__ __forceinline__ double sum_parts(const double* v, int nparts, size_t 
     return s;
 }
 
+__device__ __forceinline__ double rate_of(double de, double kT) {
+    if (!(de == de) || isinf(de)) return 0.0;
+    const double w = exp(-de / kT);
+    return w < 1.0 ? w : 1.0;
+}
+
 template <bool MIXED>
-__global__ void k_combine(CombineArgs a) {
+__global__ void __launch_bounds__(64) k_combine(CombineArgs a) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.T) return;
     const int ow = a.owner[t];
@@ -668,6 +769,7 @@ __global__ void k_combine(CombineArgs a) {
     }
     const int64_t o = a.rate_loc ? a.rate_loc[t] : (int64_t)t;
     a.out[o] = de;
+    if (a.rates) a.rates[t] = rate_of(de, a.kT);
 }
 
 // ============================================================================
@@ -1030,6 +1132,7 @@ struct SelectArgs {
     int64_t nsteps;
     double kT;
     const double* dE;
+    const double* rates;
     const double* uniform;
     const int64_t* counter;
     int64_t* selected;
@@ -1046,40 +1149,50 @@ struct SelectArgs {
     int* flags;
 };
 
-__device__ __forceinline__ double rate_of(double de, double kT) {
-    if (!(de == de) || isinf(de)) return 0.0;
-    const double w = exp(-de / kT);
-    return w < 1.0 ? w : 1.0;
-}
 
+// Rates come from k_combine.  Each thread sums a contiguous chunk; chunk sums are
+// scanned within each wave (shuffles) and across the 16 waves (LDS); the thread
+// whose chunk holds u * total walks it to find the proposal.
 __global__ void __launch_bounds__(1024) k_select(SelectArgs a) {
-    __shared__ double sums[1024];
+    __shared__ double wsum[16];
     __shared__ int pick;
     const int tid = threadIdx.x;
+    const int lane = tid & 63, wid = tid >> 6;
     const int nt = blockDim.x;
+    const int nw = nt >> 6;
     const int chunk = (a.T + nt - 1) / nt;
-    const int b0 = tid * chunk, b1 = min(a.T, b0 + chunk);
+    const int b0 = min(a.T, tid * chunk), b1 = min(a.T, b0 + chunk);
     double s = 0.0;
-    for (int t = b0; t < b1; t++) s += rate_of(a.dE[t], a.kT);
-    sums[tid] = s;
+    for (int t = b0; t < b1; t++) s += a.rates[t];
     if (tid == 0) pick = -1;
-    __syncthreads();
-    // inclusive scan (Hillis-Steele)
-    for (int o = 1; o < nt; o <<= 1) {
-        const double v = tid >= o ? sums[tid - o] : 0.0;
-        __syncthreads();
-        sums[tid] += v;
-        __syncthreads();
+    double v = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const double y = __shfl_up(v, o, 64);
+        if (lane >= o) v += y;
     }
+    if (lane == 63) wsum[wid] = v;
+    __syncthreads();
+    if (wid == 0) {
+        double w = lane < nw ? wsum[lane] : 0.0;
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+            const double y = __shfl_up(w, o, 64);
+            if (lane >= o) w += y;
+        }
+        if (lane < nw) wsum[lane] = w;
+    }
+    __syncthreads();
+    const double incl = v + (wid > 0 ? wsum[wid - 1] : 0.0);
+    const double lo = incl - s;
     const int64_t step = *a.counter % a.nsteps;
-    const double total = sums[nt - 1];
+    const double total = wsum[nw - 1];
     const double target = a.uniform[step] * total;
-    const double lo = tid > 0 ? sums[tid - 1] : 0.0;
-    if (target >= lo && target < sums[tid] && b1 > b0) {
+    if (target >= lo && target < incl && b1 > b0) {
         double c = lo;
         int k = b1 - 1;
         for (int t = b0; t < b1; t++) {
-            c += rate_of(a.dE[t], a.kT);
+            c += a.rates[t];
             if (target < c) { k = t; break; }
         }
         pick = k;
@@ -1089,7 +1202,7 @@ __global__ void __launch_bounds__(1024) k_select(SelectArgs a) {
         int k = pick;
         if (k < 0) {   // target == total (u == 1) or no positive rate: last positive rate
             for (int t = a.T - 1; t >= 0; t--)
-                if (rate_of(a.dE[t], a.kT) > 0.0) { k = t; break; }
+                if (a.rates[t] > 0.0) { k = t; break; }
         }
         MoveRec* r = a.rec;
         a.selected[step] = k;
@@ -1232,24 +1345,27 @@ __global__ void k_finish_energy(const double* e_sr, const double* elr, double eu
 // ============================================================================
 // order dispatch tables
 // ============================================================================
-typedef void (*SweepLaunch)(SweepArgs, int, hipStream_t, int, int);
+typedef void (*SweepLaunch)(SweepArgs, int, hipStream_t, int, int, int);
 typedef void (*EvalLaunch)(EvalArgs, hipStream_t);
 typedef void (*LRLaunch)(LR2Args, hipStream_t);
 typedef void (*MpLaunch)(int, int, const double4*, double*, int, hipStream_t);
 typedef void (*MdLaunch)(TailArgs, hipStream_t);
 
 template <int L, int NP>
-static void launch_sweep_np(SweepArgs a, int ncells, hipStream_t s, int block) {
+static void launch_sweep_np(SweepArgs a, int ncells, hipStream_t s, int block, int split) {
     const int ng = ((L + 1) / 2 + NP - 1) / NP;
-    int bx = (ncells + block - 1) / block;
-    if (bx % 8) bx += 8 - bx % 8;   // all column groups of a leaf block on one XCD
-    hipLaunchKernelGGL((k_sweep<L, NP>), dim3(bx, ng), dim3(block), 0, s, a);
+    if (a.g.bc == KMC_BC_FREE_SPACE) {
+        hipLaunchKernelGGL((k_sweep<L, NP, false, 1>), dim3((ncells + block - 1) / block, ng), dim3(block), 0, s, a);
+    } else if (split == 3) {
+        hipLaunchKernelGGL((k_sweep<L, NP, true, 3>), dim3((ncells + 63) / 64, ng), dim3(64, 3), 0, s, a);
+    } else {
+        hipLaunchKernelGGL((k_sweep<L, NP, true, 1>), dim3((ncells + block - 1) / block, ng), dim3(block), 0, s, a);
+    }
 }
 template <int L>
-static void launch_sweep(SweepArgs a, int ncells, hipStream_t s, int block, int np) {
-    if (np == 1) launch_sweep_np<L, 1>(a, ncells, s, block);
-    else if (np == 3 && L <= 12) launch_sweep_np<L, 3>(a, ncells, s, block);
-    else launch_sweep_np<L, 2>(a, ncells, s, block);
+static void launch_sweep(SweepArgs a, int ncells, hipStream_t s, int block, int np, int split) {
+    if (np == 1) launch_sweep_np<L, 1>(a, ncells, s, block, split);
+    else launch_sweep_np<L, 2>(a, ncells, s, block, split);
 }
 
 template <int L>
@@ -1347,10 +1463,16 @@ struct kmc_ctx {
     double E = 0, w = 0, max_move = 0, eu = 1.0, self27 = 0.0;
     int occ_headroom = 8;
     int occmax = 0;
-    int near_lanes = 8;
+    int near_lanes = 27;
     int sweep_block = 256;
-    int sweep_pairs = 2;
+    int sweep_pairs = 1;
+    int sweep_split = 1;
     int64_t last_P = 0, last_T = 0;   // sizes of the last propose (kernel timing)
+    // side stream for work that overlaps within one call (long range beside the near and
+    // far fields; cell tables and multipole beside the accept sweep), forked from and
+    // joined back to the caller's stream with events (graph-capturable)
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork[2] = {nullptr, nullptr}, ev_join[2] = {nullptr, nullptr};
     bool have_particles = false, have_lr = false, initialised = false;
     OrderFns fns{};
 
@@ -1378,7 +1500,7 @@ struct kmc_ctx {
     // propose work buffers
     DBuf<double4> epos4;
     DBuf<int32_t> ecell, eexcl, owner;
-    DBuf<double> UD, UI, lrd, dE1;
+    DBuf<double> UD, UI, lrd, dE1, rates;
     DBuf<int64_t> one_pid, one_es;
     DBuf<double> one_pos;
     // propose_with_dats work buffers
@@ -1402,6 +1524,7 @@ static int ensure_eval(kmc_ctx* c, int64_t T, int64_t P) {
     HIPCHK(c->UD.ensure(ne));
     HIPCHK(c->UI.ensure(ne * (size_t)((c->L + 1) / 2)));
     HIPCHK(c->lrd.ensure((size_t)12 * std::max<int64_t>(T, 1)));
+    HIPCHK(c->rates.ensure((size_t)std::max<int64_t>(T, 1)));
     return KMC_OK;
 }
 
@@ -1442,22 +1565,41 @@ static LR2Args lr_args(kmc_ctx* c, int64_t P, int64_t T) {
 }
 
 // propose kernels after the evaluation arrays are filled (T new points, then P old)
+// the side stream waits for everything issued so far on s
+static int fork_side(kmc_ctx* c, hipStream_t s, int i) {
+    HIPCHK(hipEventRecord(c->ev_fork[i], s));
+    HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork[i], 0));
+    return KMC_OK;
+}
+// s waits for everything issued so far on the side stream
+static int join_side(kmc_ctx* c, hipStream_t s, int i) {
+    HIPCHK(hipEventRecord(c->ev_join[i], c->side));
+    HIPCHK(hipStreamWaitEvent(s, c->ev_join[i], 0));
+    return KMC_OK;
+}
+
 static int run_propose_core(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, double* d_out, const int64_t* rate_loc,
-                            bool mixed) {
+                            bool mixed, double kT = 0.0, double* rates = nullptr) {
     const int ne = (int)(T + P);
     c->last_P = P;
     c->last_T = T;
+    const bool lr = (c->bc == KMC_BC_PBC || c->bc == KMC_BC_FF_ONLY) && c->have_lr;
+    if (lr) {
+        int rc = ensure_lr_work(c, P);
+        if (rc) return rc;
+        rc = fork_side(c, s, 0);
+        if (rc) return rc;
+        c->fns.lr(lr_args(c, P, T), c->side);
+    }
     if (c->bc != KMC_BC_FF_ONLY) {
         EvalArgs an = eval_args(c, ne, c->epos4.p, c->ecell.p, c->eexcl.p, c->UD.p);
         launch_near(an, c->near_lanes, mixed, s);
         EvalArgs af = eval_args(c, ne, c->epos4.p, c->ecell.p, c->eexcl.p, c->UI.p);
         c->fns.far(af, s);
     }
-    const bool lr = (c->bc == KMC_BC_PBC || c->bc == KMC_BC_FF_ONLY) && c->have_lr;
     if (lr) {
-        int rc = ensure_lr_work(c, P);
+        int rc = join_side(c, s, 0);
         if (rc) return rc;
-        c->fns.lr(lr_args(c, P, T), s);
     }
     CombineArgs ca;
     ca.g = c->geom();
@@ -1474,9 +1616,11 @@ static int run_propose_core(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, dou
     ca.P = (int)P;
     ca.rate_loc = rate_loc;
     ca.out = d_out;
+    ca.kT = kT;
+    ca.rates = rates;
     if (T > 0) {
-        if (mixed) hipLaunchKernelGGL(k_combine<true>, dim3((T + 255) / 256), dim3(256), 0, s, ca);
-        else hipLaunchKernelGGL(k_combine<false>, dim3((T + 255) / 256), dim3(256), 0, s, ca);
+        if (mixed) hipLaunchKernelGGL(k_combine<true>, dim3((T + 63) / 64), dim3(64), 0, s, ca);
+        else hipLaunchKernelGGL(k_combine<false>, dim3((T + 63) / 64), dim3(64), 0, s, ca);
     }
     HIPCHK(hipGetLastError());
     return KMC_OK;
@@ -1508,18 +1652,6 @@ static int launch_setup(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, const i
 }
 
 static int run_accept_tail(kmc_ctx* c, hipStream_t s, int64_t* d_counter) {
-    if (c->bc != KMC_BC_FF_ONLY) {
-        SweepArgs sa;
-        sa.g = c->geom();
-        sa.lexp = c->lexp.p;
-        sa.ncells = (int)c->ncells;
-        sa.mode = 0;
-        sa.rec = c->rec.p;
-        sa.src_pos4 = nullptr;
-        sa.src_cell = nullptr;
-        sa.src_begin = sa.src_end = 0;
-        c->fns.sweep(sa, (int)c->ncells, s, c->sweep_block, c->sweep_pairs);
-    }
     TailArgs ta;
     ta.rec = c->rec.p;
     ta.occ = c->occ.p;
@@ -1537,7 +1669,28 @@ static int run_accept_tail(kmc_ctx* c, hipStream_t s, int64_t* d_counter) {
     ta.g = c->gvec.p;
     ta.elr = c->elr.p;
     ta.counter = d_counter;
-    c->fns.md(ta, s);
+    if (c->bc == KMC_BC_FF_ONLY) {
+        c->fns.md(ta, s);
+        HIPCHK(hipGetLastError());
+        return KMC_OK;
+    }
+    // the tail (cell tables, M, g, E_lr, counter) and the sweep (leaf expansions) only
+    // share the move record, which both read: they run side by side
+    int rc = fork_side(c, s, 1);
+    if (rc) return rc;
+    c->fns.md(ta, c->side);
+    SweepArgs sa;
+    sa.g = c->geom();
+    sa.lexp = c->lexp.p;
+    sa.ncells = (int)c->ncells;
+    sa.mode = 0;
+    sa.rec = c->rec.p;
+    sa.src_pos4 = nullptr;
+    sa.src_cell = nullptr;
+    sa.src_begin = sa.src_end = 0;
+    c->fns.sweep(sa, (int)c->ncells, s, c->sweep_block, c->sweep_pairs, c->sweep_split);
+    rc = join_side(c, s, 1);
+    if (rc) return rc;
     HIPCHK(hipGetLastError());
     return KMC_OK;
 }
@@ -1616,6 +1769,13 @@ int kmc_create(kmc_ctx** out, int64_t n, double extent, int32_t r, int32_t L, in
         rc = set_err(KMC_E_HIP, "device allocation failed");
     }
     if (rc == KMC_OK) {
+        if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) rc = set_err(KMC_E_HIP, "stream");
+        for (int i = 0; i < 2 && rc == KMC_OK; i++)
+            if (hipEventCreateWithFlags(&c->ev_fork[i], hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&c->ev_join[i], hipEventDisableTiming) != hipSuccess)
+                rc = set_err(KMC_E_HIP, "event");
+    }
+    if (rc == KMC_OK) {
         hipMemset(c->energy.p, 0, sizeof(double));
         hipMemset(c->flags.p, 0, sizeof(int));
         hipMemset(c->err.p, 0, sizeof(int));
@@ -1641,23 +1801,29 @@ int kmc_destroy(kmc_ctx* c) {
     c->UI.release(); c->lrd.release(); c->dE1.release(); c->one_pid.release(); c->one_es.release();
     c->one_pos.release(); c->d_cnt.release(); c->d_has.release(); c->d_slot.release(); c->d_off.release();
     c->d_pid.release(); c->d_es.release(); c->d_rate.release(); c->d_tot.release(); c->d_npos.release();
-    c->d_nq.release();
+    c->d_nq.release(); c->rates.release();
+    for (int i = 0; i < 2; i++) {
+        if (c->ev_fork[i]) hipEventDestroy(c->ev_fork[i]);
+        if (c->ev_join[i]) hipEventDestroy(c->ev_join[i]);
+    }
+    if (c->side) hipStreamDestroy(c->side);
     delete c;
     return KMC_OK;
 }
 
-int kmc_set_tiles(kmc_ctx* c, int32_t near_lanes, int32_t sweep_block, int32_t sweep_pairs) {
+int kmc_set_tiles(kmc_ctx* c, int32_t near_lanes, int32_t sweep_block, int32_t sweep_pairs, int32_t sweep_split) {
     if (check_ctx(c)) return KMC_E_ARG;
     if (near_lanes != 1 && near_lanes != 4 && near_lanes != 8 && near_lanes != 16 && near_lanes != 27 &&
         near_lanes != 32 && near_lanes != 64)
         return set_err(KMC_E_ARG, "near_lanes must be one of 1,4,8,16,27,32,64 (27: one lane per neighbour leaf)");
+    if (sweep_pairs < 1 || sweep_pairs > 2) return set_err(KMC_E_ARG, "sweep_pairs must be 1 or 2");
     if (sweep_block < 64 || sweep_block > 256 || sweep_block % 64)
         return set_err(KMC_E_ARG, "sweep_block must be 64, 128, 192 or 256");
-    if (sweep_pairs < 1 || sweep_pairs > 3 || (sweep_pairs == 3 && c->L > 12))
-        return set_err(KMC_E_ARG, "sweep_pairs must be 1 or 2 (3 for L <= 12)");
+    if (sweep_split != 1 && sweep_split != 3) return set_err(KMC_E_ARG, "sweep_split must be 1 or 3");
     c->near_lanes = near_lanes;
     c->sweep_block = sweep_block;
     c->sweep_pairs = sweep_pairs;
+    c->sweep_split = sweep_split;
     return KMC_OK;
 }
 
@@ -1791,7 +1957,7 @@ int kmc_initialise(kmc_ctx* c, void* stream, double* h_energy) {
             sa.src_cell = c->pcell.p;
             sa.src_begin = b;
             sa.src_end = std::min(n, b + chunk);
-            c->fns.sweep(sa, (int)c->ncells, s, c->sweep_block, c->sweep_pairs);
+            c->fns.sweep(sa, (int)c->ncells, s, c->sweep_block, c->sweep_pairs, c->sweep_split);
             HIPCHK(hipGetLastError());
         }
     }
@@ -1844,6 +2010,7 @@ int kmc_initialise(kmc_ctx* c, void* stream, double* h_energy) {
 
 int kmc_get_expansions(kmc_ctx* c, double* h_out) {
     if (check_ctx(c)) return KMC_E_ARG;
+    if (!c->have_particles) return set_err(KMC_E_STATE, "set particles first (kmc_set_particles)");
     const int L = c->L, L2 = L * L;
     std::vector<double2> h((size_t)c->ncells * c->nh);
     HIPCHK(hipMemcpy(h.data(), c->lexp.p, h.size() * sizeof(double2), hipMemcpyDeviceToHost));
@@ -1876,6 +2043,7 @@ int kmc_set_expansions(kmc_ctx* c, const double* h_in) {
 
 int kmc_get_multipole(kmc_ctx* c, double* h_out) {
     if (check_ctx(c)) return KMC_E_ARG;
+    if (!c->initialised) return set_err(KMC_E_STATE, "Run initialise before this call");
     const int L = c->L, L2 = L * L;
     std::vector<double> h(2 * c->nh);
     HIPCHK(hipMemcpy(h.data(), c->M.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -1890,12 +2058,14 @@ int kmc_get_multipole(kmc_ctx* c, double* h_out) {
 
 int kmc_store_occmax(kmc_ctx* c, int32_t* out) {
     if (check_ctx(c)) return KMC_E_ARG;
+    if (!c->have_particles) return set_err(KMC_E_STATE, "set particles first (kmc_set_particles)");
     *out = c->occmax;
     return KMC_OK;
 }
 
 int kmc_get_store(kmc_ctx* c, int32_t* h_occ, int32_t* h_ids, double* h_pos4, int32_t* h_occmax) {
     if (check_ctx(c)) return KMC_E_ARG;
+    if (!c->have_particles) return set_err(KMC_E_STATE, "set particles first (kmc_set_particles)");
     if (h_occmax) *h_occmax = c->occmax;
     if (h_occ) HIPCHK(hipMemcpy(h_occ, c->occ.p, c->ncells * sizeof(int), hipMemcpyDeviceToHost));
     if (h_ids) HIPCHK(hipMemcpy(h_ids, c->slot_id.p, (size_t)c->ncells * c->occmax * sizeof(int32_t), hipMemcpyDeviceToHost));
@@ -1905,6 +2075,7 @@ int kmc_get_store(kmc_ctx* c, int32_t* h_occ, int32_t* h_ids, double* h_pos4, in
 
 int kmc_get_particles(kmc_ctx* c, double* h_pos, int32_t* h_cells) {
     if (check_ctx(c)) return KMC_E_ARG;
+    if (!c->have_particles) return set_err(KMC_E_STATE, "set particles first (kmc_set_particles)");
     std::vector<double4> p4(c->n);
     HIPCHK(hipMemcpy(p4.data(), c->ppos4.p, c->n * sizeof(double4), hipMemcpyDeviceToHost));
     if (h_pos)
@@ -2045,12 +2216,13 @@ int kmc_step(kmc_ctx* c, void* stream, int64_t P, int64_t T, const int64_t* d_pi
     if (check_ctx(c)) return KMC_E_ARG;
     if (!c->initialised) return set_err(KMC_E_STATE, "Run initialise before this call");
     if (P < 1 || T < 1 || T >= (int64_t)1 << 30 || nsteps < 1) return set_err(KMC_E_ARG, "bad proposal counts");
-    if ((size_t)(T + P) > c->epos4.n || (size_t)T > c->owner.n)
+    const bool lr_sized = !c->have_lr || ((size_t)P * 2 * c->nh <= c->aold.n && (size_t)12 * P <= c->cold.n);
+    if ((size_t)(T + P) > c->epos4.n || (size_t)T > c->owner.n || (size_t)T > c->rates.n || !lr_sized)
         return set_err(KMC_E_STATE, "work buffers not sized: call kmc_propose once with this T before kmc_step");
     hipStream_t s = (hipStream_t)stream;
     int rc = launch_setup(c, s, P, T, d_pid, d_es, d_new_pos, nullptr);
     if (rc) return rc;
-    rc = run_propose_core(c, s, P, T, d_dE, nullptr, precision == KMC_PREC_MIXED);
+    rc = run_propose_core(c, s, P, T, d_dE, nullptr, precision == KMC_PREC_MIXED, kT, c->rates.p);
     if (rc) return rc;
     SelectArgs sa;
     sa.g = c->geom();
@@ -2059,6 +2231,7 @@ int kmc_step(kmc_ctx* c, void* stream, int64_t P, int64_t T, const int64_t* d_pi
     sa.nsteps = nsteps;
     sa.kT = kT;
     sa.dE = d_dE;
+    sa.rates = c->rates.p;
     sa.uniform = d_uniform;
     sa.counter = d_counter;
     sa.selected = d_selected;
@@ -2084,6 +2257,7 @@ int kmc_materialize_moves(kmc_ctx* c, void* stream, int64_t nsteps, int64_t P, i
                           const int64_t* d_pids_all, const double* d_disp_all, const int64_t* d_counter,
                           int64_t* d_pid, int64_t* d_es, double* d_new_pos) {
     if (check_ctx(c)) return KMC_E_ARG;
+    if (!c->have_particles) return set_err(KMC_E_STATE, "set particles first (kmc_set_particles)");
     if (P < 1 || k < 1 || nsteps < 1) return set_err(KMC_E_ARG, "bad counts");
     hipStream_t s = (hipStream_t)stream;
     const int64_t T = P * k;
@@ -2128,7 +2302,7 @@ int kmc_time_kernel(kmc_ctx* c, void* stream, int32_t which, int32_t reps, int32
             sa.src_pos4 = nullptr;
             sa.src_cell = nullptr;
             sa.src_begin = sa.src_end = 0;
-            c->fns.sweep(sa, (int)c->ncells, s, c->sweep_block, c->sweep_pairs);
+            c->fns.sweep(sa, (int)c->ncells, s, c->sweep_block, c->sweep_pairs, c->sweep_split);
         } else if (which == 1) {
             EvalArgs an = eval_args(c, (int)(T + P), c->epos4.p, c->ecell.p, c->eexcl.p, c->UD.p);
             launch_near(an, c->near_lanes, mixed, s);
@@ -2154,6 +2328,7 @@ int kmc_time_kernel(kmc_ctx* c, void* stream, int32_t which, int32_t reps, int32
 
 int kmc_sweep_hop(kmc_ctx* c, void* stream, double q, const double* h_old, const double* h_new) {
     if (check_ctx(c)) return KMC_E_ARG;
+    if (!c->have_particles) return set_err(KMC_E_STATE, "set particles first (kmc_set_particles)");
     if (c->bc == KMC_BC_FF_ONLY) return KMC_OK;
     hipStream_t s = (hipStream_t)stream;
     MoveRec r{};
@@ -2176,7 +2351,7 @@ int kmc_sweep_hop(kmc_ctx* c, void* stream, double q, const double* h_old, const
     sa.src_pos4 = nullptr;
     sa.src_cell = nullptr;
     sa.src_begin = sa.src_end = 0;
-    c->fns.sweep(sa, (int)c->ncells, s, c->sweep_block, c->sweep_pairs);
+    c->fns.sweep(sa, (int)c->ncells, s, c->sweep_block, c->sweep_pairs, c->sweep_split);
     HIPCHK(hipGetLastError());
     return KMC_OK;
 }

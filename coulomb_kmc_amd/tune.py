@@ -4,14 +4,16 @@ Tile autotuner for the propose/accept kernels.
 Tunable shapes (set on a context with KMCFMM.set_tiles / kmc_set_tiles):
   near_lanes   lanes per evaluation point in the near-field kernel (1, 4, 8, 16, 32, 64;
                27 = one lane per neighbour leaf)
-  sweep_block  threads per block of the accept sweep (64, 128, 192, 256)
-  sweep_pairs  column pairs per lane of the accept sweep (1, 2; 3 for L <= 12)
+  sweep_pairs  column pairs (m, L-1-m) per lane of the accept sweep (1, 2)
+  sweep_block  leaves (threads) per workgroup of the accept sweep (64, 128, 192, 256)
+  sweep_split  waves sharing the 27 images of one leaf (1; or 3, one image layer per
+               wave, workgroups of 64 leaves x 3; '27'/PBC only)
 
 ``autotune(kmc)`` times every candidate with HIP events (kmc_time_kernel) on the
 context's current state and returns the fastest; ``choose_tiles`` reads the table of
-tuned choices kept in tiles.json (keyed by expansion order and proposal batch
-size), falling back to defaults.  The sweep candidates are timed last because timing
-the sweep re-applies the last accepted move.
+tuned choices kept in tiles.json (keyed by expansion order, leaf count and proposal
+batch size), falling back to defaults.  The sweep candidates are timed last because
+timing the sweep re-applies the last accepted move.
 """
 
 import json
@@ -21,16 +23,23 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(_HERE, "tiles.json")
 
 NEAR_LANES = (1, 4, 8, 16, 27, 32, 64)
-SWEEP_BLOCKS = (64, 128, 256)
-DEFAULT = {"near_lanes": 27, "sweep_block": 256, "sweep_pairs": 2}
+# (sweep_pairs, sweep_block, sweep_split)
+SWEEP_SHAPES = ((1, 64, 1), (1, 128, 1), (1, 256, 1), (2, 64, 1), (2, 128, 1), (2, 256, 1), (1, 64, 3),
+                (2, 64, 3))
+DEFAULT = {"near_lanes": 27, "sweep_block": 256, "sweep_pairs": 1, "sweep_split": 1}
+KEYS = ("near_lanes", "sweep_block", "sweep_pairs", "sweep_split")
 
 
-def _key(L, T):
-    # proposal batches are bucketed by powers of two
+def _pow2(x):
     b = 1
-    while b < T:
+    while b < x:
         b *= 2
-    return "L{}_T{}".format(int(L), b)
+    return b
+
+
+def _key(L, T, ncells):
+    # proposal batches and leaf counts are bucketed by powers of two
+    return "L{}_C{}_T{}".format(int(L), _pow2(ncells), _pow2(T))
 
 
 def load_table():
@@ -40,42 +49,41 @@ def load_table():
     return {}
 
 
-def choose_tiles(L, T, near_lanes=None, sweep_block=None, sweep_pairs=None):
+def choose_tiles(L, T, ncells, **override):
     t = dict(DEFAULT)
-    t.update(load_table().get(_key(L, T), {}))
-    if near_lanes:
-        t["near_lanes"] = int(near_lanes)
-    if sweep_block:
-        t["sweep_block"] = int(sweep_block)
-    if sweep_pairs:
-        t["sweep_pairs"] = int(sweep_pairs)
-    return {"near_lanes": int(t["near_lanes"]), "sweep_block": int(t["sweep_block"]),
-            "sweep_pairs": int(t["sweep_pairs"])}
+    t.update(load_table().get(_key(L, T, ncells), {}))
+    for k in KEYS:
+        if override.get(k) is not None:
+            t[k] = int(override[k])
+    return {k: int(t[k]) for k in KEYS}
+
+
+def sweep_candidates(periodic):
+    return [s for s in SWEEP_SHAPES if periodic or s[2] == 1]
 
 
 def autotune(kmc, T, reps=10, precision="mixed", save=False, log=None):
     """Time every candidate on kmc (a propose must have run); returns (choice, timings)."""
     timings = {"near": {}, "sweep": {}}
-    cur = choose_tiles(kmc.L, T)
+    cur = choose_tiles(kmc.L, T, kmc.ncells)
     for g in NEAR_LANES:
-        kmc.set_tiles(near_lanes=g, sweep_block=cur["sweep_block"], sweep_pairs=cur["sweep_pairs"])
+        kmc.set_tiles(**dict(cur, near_lanes=g))
         kmc.time_kernel("near", reps=2, precision=precision)
         timings["near"][g] = kmc.time_kernel("near", reps=reps, precision=precision)
     best_g = min(timings["near"], key=timings["near"].get)
-    for npair in ((1, 2, 3) if kmc.L <= 12 else (1, 2)):
-        for b in SWEEP_BLOCKS:
-            kmc.set_tiles(near_lanes=best_g, sweep_block=b, sweep_pairs=npair)
-            kmc.time_kernel("sweep", reps=1)
-            timings["sweep"]["{}x{}".format(npair, b)] = kmc.time_kernel("sweep", reps=reps)
+    for npair, block, split in sweep_candidates(kmc.boundary_condition.value != "free_space"):
+        kmc.set_tiles(near_lanes=best_g, sweep_block=block, sweep_pairs=npair, sweep_split=split)
+        kmc.time_kernel("sweep", reps=1)
+        timings["sweep"]["{}x{}x{}".format(npair, block, split)] = kmc.time_kernel("sweep", reps=reps)
     best = min(timings["sweep"], key=timings["sweep"].get)
-    best_np, best_b = (int(x) for x in best.split("x"))
-    choice = {"near_lanes": int(best_g), "sweep_block": best_b, "sweep_pairs": best_np}
+    npair, block, split = (int(x) for x in best.split("x"))
+    choice = {"near_lanes": int(best_g), "sweep_block": block, "sweep_pairs": npair, "sweep_split": split}
     kmc.set_tiles(**choice)
     if log:
-        log("autotune", _key(kmc.L, T), choice, timings)
+        log("autotune", _key(kmc.L, T, kmc.ncells), choice, timings)
     if save:
         tab = load_table()
-        tab[_key(kmc.L, T)] = choice
+        tab[_key(kmc.L, T, kmc.ncells)] = choice
         with open(TABLE, "w") as f:
             json.dump(tab, f, indent=1, sort_keys=True)
     return choice, timings

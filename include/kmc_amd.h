@@ -161,11 +161,13 @@ int kmc_check_flags(kmc_ctx* ctx, void* stream, int32_t* h_flags);
 /*
  * Tile choices of the hot kernels (filled by the autotuner in coulomb_kmc_amd/tune.py):
  * near_lanes = lanes per evaluation point in the near-field kernel (1,4,8,16,32,64, or
- * 27 = one lane per neighbour leaf), sweep_block = threads per block of the accept
- * sweep (64, 128, 192, 256), sweep_pairs = column pairs (m, L-1-m) per lane of
- * the accept sweep (1, 2; 3 for L <= 12).
+ * 27 = one lane per neighbour leaf); accept sweep: sweep_pairs = column pairs
+ * (m, L-1-m) per lane (1, 2), sweep_block = leaves (threads) per workgroup (64, 128,
+ * 192, 256), sweep_split = waves sharing one leaf's 27 images (1, or 3: one image
+ * layer each, summed in LDS; workgroups of 64 leaves x 3; '27'/PBC only).
  */
-int kmc_set_tiles(kmc_ctx* ctx, int32_t near_lanes, int32_t sweep_block, int32_t sweep_pairs);
+int kmc_set_tiles(kmc_ctx* ctx, int32_t near_lanes, int32_t sweep_block, int32_t sweep_pairs,
+                  int32_t sweep_split);
 
 /* Accept sweep only, for a hop of charge q from h_old to h_new (timing / tests). */
 int kmc_sweep_hop(kmc_ctx* ctx, void* stream, double q, const double* h_old_pos, const double* h_new_pos);

@@ -263,6 +263,25 @@ def test_bad_move_raises():
     kmc.free()
 
 
+def test_device_calls_before_particles_raise():
+    """Entry points that touch the leaf expansions or the particle arrays refuse to run
+    before the particles are set (the host check returns before any launch)."""
+    import torch
+    from coulomb_kmc_amd import KMCFMM
+    rng = np.random.default_rng(0)
+    pos, q = neutral_system(rng, 50, 4.0)
+    kmc = KMCFMM(pos, q, 4.0, r=3, l=8, boundary_condition="pbc")
+    with pytest.raises(RuntimeError):
+        kmc.sweep_hop(1.0, pos[0], pos[1])
+    with pytest.raises(RuntimeError):
+        kmc.get_expansions()
+    z = torch.zeros(8, dtype=torch.int64, device="cuda")
+    d = torch.zeros(24, dtype=torch.float64, device="cuda")
+    with pytest.raises(RuntimeError):
+        kmc.materialize_moves(1, 1, z, d, z, z, z, d)
+    kmc.free()
+
+
 def test_use_before_initialise_raises():
     from coulomb_kmc_amd import KMCFMM
     rng = np.random.default_rng(0)
