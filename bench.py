@@ -14,7 +14,7 @@ Prints one JSON line (see the "value" field: mixed-precision proposals/s of the
 whole step; accepts/s, the FP64 path, the dE error of mixed vs FP64, the roofline of
 the dominant kernel and the host-CPU baseline are in the same line).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1|0|4]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload pbc-2m|pbc-100k|free-1k]
 """
 
 import argparse
@@ -30,15 +30,16 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ΔE proposals/s + accepts/s on 1 GPU, mixed vs FP64, at ΔE rel err ≤1e-8"
 
-# leaf grids as named by the workload (r = levels, leaf grid 2^(r-1) per side)
-CONFIGS = {
-    0: dict(workload="free-space N=1000 box 10 leaves 8^3 L=12, 64x4 proposals", N=1000, E=10.0, r=4, L=12,
-            bc="free_space", P=64, k=4, radius=1.0, seed=1000),
-    1: dict(workload="pbc N=100000 box 100 leaves 32^3 L=12, 1024x8 proposals", N=100000, E=100.0, r=6, L=12,
-            bc="pbc", P=1024, k=8, radius=1.5, seed=1001),
-    4: dict(workload="pbc N=2000000 box 271.4 leaves 64^3 L=16, 32768x8 proposals", N=2000000, E=271.4, r=7,
-            L=16, bc="pbc", P=32768, k=8, radius=1.5, seed=1004),
+# workloads (r = levels, leaf grid 2^(r-1) per side); steps/warmup are the defaults
+WORKLOADS = {
+    "free-1k": dict(workload="free-space N=1000 box 10 leaves 8^3 L=12, 64x4 proposals", N=1000, E=10.0, r=4,
+                    L=12, bc="free_space", P=64, k=4, radius=1.0, seed=1000, steps=100, warmup=10),
+    "pbc-100k": dict(workload="pbc N=100000 box 100 leaves 32^3 L=12, 1024x8 proposals", N=100000, E=100.0, r=6,
+                     L=12, bc="pbc", P=1024, k=8, radius=1.5, seed=1001, steps=1000, warmup=50),
+    "pbc-2m": dict(workload="pbc N=2000000 box 271.4 leaves 64^3 L=16, 32768x8 proposals", N=2000000, E=271.4,
+                   r=7, L=16, bc="pbc", P=32768, k=8, radius=1.5, seed=1004, steps=100, warmup=20),
 }
+DEFAULT_WORKLOAD = "pbc-2m"
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
@@ -150,11 +151,11 @@ def _per_dispatch(rows):
     return acc
 
 
-def traffic_from_pmc(config, fetch_dir, write_dir):
+def traffic_from_pmc(workload, fetch_dir, write_dir):
     """HBM bytes per accept-sweep launch from FETCH_SIZE / WRITE_SIZE (KiB per dispatch),
     each scaled by bytes / counter of the copy probe (which reads and writes
     COPY_PROBE_BYTES)."""
-    cfg = CONFIGS[config]
+    cfg = WORKLOADS[workload]
     res = {}
     for which, d, counter in (("fetch", fetch_dir, "FETCH_SIZE"), ("write", write_dir, "WRITE_SIZE")):
         per = _per_dispatch(_pmc_rows(d, counter))
@@ -187,9 +188,9 @@ def traffic_from_pmc(config, fetch_dir, write_dir):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--config", type=int, default=1, choices=sorted(CONFIGS))
+    ap.add_argument("--steps", type=int, default=0, help="0: the workload's default")
+    ap.add_argument("--warmup", type=int, default=-1, help="-1: the workload's default")
+    ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
     ap.add_argument("--graph-steps", type=int, default=10, help="steps captured per HIP graph")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -202,7 +203,7 @@ def main():
     args = ap.parse_args()
 
     if args.traffic_from:
-        traffic_from_pmc(args.config, *args.traffic_from)
+        traffic_from_pmc(args.workload, *args.traffic_from)
         return
 
     if args.gpus != 1:
@@ -214,14 +215,18 @@ def main():
     from coulomb_kmc_amd import KMCFMM
     from coulomb_kmc_amd import tune
 
-    cfg = CONFIGS[args.config]
+    cfg = WORKLOADS[args.workload]
+    if args.steps <= 0:
+        args.steps = cfg["steps"]
+    if args.warmup < 0:
+        args.warmup = cfg["warmup"]
     N, E, r, L, bc, P, k = cfg["N"], cfg["E"], cfg["r"], cfg["L"], cfg["bc"], cfg["P"], cfg["k"]
     T = P * k
     G = max(1, args.graph_steps)
     steps = max(G, (args.steps // G) * G)
     warm = max(G, (args.warmup // G) * G)
     nstream = warm + 2 * steps + 16
-    log("config", args.config, cfg["workload"], "steps", steps, "warmup", warm)
+    log("workload", args.workload, cfg["workload"], "steps", steps, "warmup", warm)
 
     pos0, q, rng = make_system(cfg)
     t0 = time.time()

@@ -34,11 +34,11 @@ KMC_E_STATE = 3
 # exported symbols (each declared in include/kmc_amd.h)
 EXPORTS = (
     "kmc_last_error", "kmc_min_order", "kmc_max_order", "kmc_create", "kmc_destroy",
-    "kmc_set_particles", "kmc_set_lr_operator", "kmc_initialise", "kmc_get_expansions",
+    "kmc_set_particles", "kmc_set_lr_operator", "kmc_initialise", "kmc_set_init_method", "kmc_get_expansions",
     "kmc_set_expansions", "kmc_get_multipole", "kmc_get_store", "kmc_store_occmax",
     "kmc_get_particles", "kmc_get_energy", "kmc_set_energy", "kmc_ncells", "kmc_propose",
     "kmc_propose_with_dats", "kmc_accept", "kmc_step", "kmc_materialize_moves",
-    "kmc_check_flags", "kmc_set_tiles", "kmc_sweep_hop", "kmc_time_kernel",
+    "kmc_check_flags", "kmc_set_tiles", "kmc_sweep_hop", "kmc_time_kernel", "kmc_eval_field",
 )
 
 
@@ -90,6 +90,7 @@ def lib():
         "kmc_set_particles": (ctypes.c_int, [P, P, P]),
         "kmc_set_lr_operator": (ctypes.c_int, [P, I64, P, P, P]),
         "kmc_initialise": (ctypes.c_int, [P, P, P]),
+        "kmc_set_init_method": (ctypes.c_int, [P, I32]),
         "kmc_get_expansions": (ctypes.c_int, [P, P]),
         "kmc_set_expansions": (ctypes.c_int, [P, P]),
         "kmc_get_multipole": (ctypes.c_int, [P, P]),
@@ -105,6 +106,7 @@ def lib():
         "kmc_step": (ctypes.c_int, [P, P, I64, I64, P, P, P, P, I64, P, D, P, P, I32]),
         "kmc_materialize_moves": (ctypes.c_int, [P, P, I64, I64, I64, P, P, P, P, P, P]),
         "kmc_check_flags": (ctypes.c_int, [P, P, P]),
+        "kmc_eval_field": (ctypes.c_int, [P, P, I64, P, P]),
         "kmc_set_tiles": (ctypes.c_int, [P, I32, I32, I32, I32]),
         "kmc_sweep_hop": (ctypes.c_int, [P, P, D, P, P]),
         "kmc_time_kernel": (ctypes.c_int, [P, P, I32, I32, I32, P]),

@@ -718,6 +718,7 @@ struct CombineArgs {
     double* out;
     double kT;
     double* rates;               // Metropolis rates min(1, exp(-dE/kT)) per proposal, or null
+    double* bsum;                // their sums over blocks of 64 proposals
 };
 
 __device__ __forceinline__ double sum_parts(const double* v, int nparts, size_t stride, size_t i) {
@@ -732,10 +733,9 @@ __device__ __forceinline__ double rate_of(double de, double kT) {
     return w < 1.0 ? w : 1.0;
 }
 
+// energy difference of proposal t (writes it at its output location)
 template <bool MIXED>
-__global__ void __launch_bounds__(64) k_combine(CombineArgs a) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.T) return;
+__device__ __forceinline__ double combine_one(const CombineArgs& a, int t) {
     const int ow = a.owner[t];
     const int po = a.T + ow;
     const double4 xo = a.epos4[po];
@@ -769,7 +769,26 @@ __global__ void __launch_bounds__(64) k_combine(CombineArgs a) {
     }
     const int64_t o = a.rate_loc ? a.rate_loc[t] : (int64_t)t;
     a.out[o] = de;
-    if (a.rates) a.rates[t] = rate_of(de, a.kT);
+    return de;
+}
+
+// one lane per proposal, blocks of 64; with rates: the Metropolis rate per proposal
+// and the rate sum of each block of 64 (for the selection)
+template <bool MIXED>
+__global__ void __launch_bounds__(64) k_combine(CombineArgs a) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double rate = 0.0;
+    if (t < a.T) {
+        const double de = combine_one<MIXED>(a, t);
+        if (a.rates) {
+            rate = rate_of(de, a.kT);
+            a.rates[t] = rate;
+        }
+    }
+    if (a.rates) {
+        const double bs = wave_sum(rate);
+        if (threadIdx.x == 0) a.bsum[blockIdx.x] = bs;
+    }
 }
 
 // ============================================================================
@@ -1133,6 +1152,7 @@ struct SelectArgs {
     double kT;
     const double* dE;
     const double* rates;
+    const double* bsum;
     const double* uniform;
     const int64_t* counter;
     int64_t* selected;
@@ -1150,9 +1170,10 @@ struct SelectArgs {
 };
 
 
-// Rates come from k_combine.  Each thread sums a contiguous chunk; chunk sums are
-// scanned within each wave (shuffles) and across the 16 waves (LDS); the thread
-// whose chunk holds u * total walks it to find the proposal.
+// Rates and their sums over blocks of 64 proposals come from k_combine.  Each thread
+// sums a contiguous chunk of block sums; chunk sums are scanned within each wave
+// (shuffles) and across the 16 waves (LDS); the thread whose chunk holds u * total
+// walks its block sums, then that block's rates, to find the proposal.
 __global__ void __launch_bounds__(1024) k_select(SelectArgs a) {
     __shared__ double wsum[16];
     __shared__ int pick;
@@ -1160,10 +1181,11 @@ __global__ void __launch_bounds__(1024) k_select(SelectArgs a) {
     const int lane = tid & 63, wid = tid >> 6;
     const int nt = blockDim.x;
     const int nw = nt >> 6;
-    const int chunk = (a.T + nt - 1) / nt;
-    const int b0 = min(a.T, tid * chunk), b1 = min(a.T, b0 + chunk);
+    const int nb = (a.T + 63) / 64;
+    const int chunk = (nb + nt - 1) / nt;
+    const int b0 = min(nb, tid * chunk), b1 = min(nb, b0 + chunk);
     double s = 0.0;
-    for (int t = b0; t < b1; t++) s += a.rates[t];
+    for (int b = b0; b < b1; b++) s += a.bsum[b];
     if (tid == 0) pick = -1;
     double v = s;
 #pragma unroll
@@ -1183,17 +1205,32 @@ __global__ void __launch_bounds__(1024) k_select(SelectArgs a) {
         if (lane < nw) wsum[lane] = w;
     }
     __syncthreads();
+    // chunk intervals [lo, incl): each lower bound is the very value its predecessor
+    // computed as upper bound, so the intervals tile [0, total) without gaps
     const double incl = v + (wid > 0 ? wsum[wid - 1] : 0.0);
-    const double lo = incl - s;
+    __shared__ double wlast[16];
+    if (lane == 63) wlast[wid] = incl;
+    const double prev = __shfl_up(incl, 1, 64);
+    __syncthreads();
+    const double lo = lane > 0 ? prev : (wid > 0 ? wlast[wid - 1] : 0.0);
     const int64_t step = *a.counter % a.nsteps;
-    const double total = wsum[nw - 1];
+    const double total = wlast[nw - 1];
     const double target = a.uniform[step] * total;
     if (target >= lo && target < incl && b1 > b0) {
         double c = lo;
-        int k = b1 - 1;
-        for (int t = b0; t < b1; t++) {
-            c += a.rates[t];
-            if (target < c) { k = t; break; }
+        int k = -1;
+        for (int b = b0; b < b1 && k < 0; b++) {
+            const double cb = c + a.bsum[b];
+            if (target < cb || b == b1 - 1) {
+                const int t0 = b * 64, t1 = min(a.T, t0 + 64);
+                for (int t = t0; t < t1; t++) {
+                    c += a.rates[t];
+                    if (target < c) { k = t; break; }
+                }
+                if (k < 0) k = t1 - 1;
+            } else {
+                c = cb;
+            }
         }
         pick = k;
     }
@@ -1201,8 +1238,11 @@ __global__ void __launch_bounds__(1024) k_select(SelectArgs a) {
     if (tid == 0) {
         int k = pick;
         if (k < 0) {   // target == total (u == 1) or no positive rate: last positive rate
-            for (int t = a.T - 1; t >= 0; t--)
-                if (a.rates[t] > 0.0) { k = t; break; }
+            for (int b = nb - 1; b >= 0 && k < 0; b--) {
+                if (!(a.bsum[b] > 0.0)) continue;
+                for (int t = min(a.T, b * 64 + 64) - 1; t >= b * 64; t--)
+                    if (a.rates[t] > 0.0) { k = t; break; }
+            }
         }
         MoveRec* r = a.rec;
         a.selected[step] = k;
@@ -1343,6 +1383,394 @@ __global__ void k_finish_energy(const double* e_sr, const double* elr, double eu
 }
 
 // ============================================================================
+// initial solve, hierarchical (large systems; coulomb_kmc/kmc_fmm.py:463-476,
+// 956-959 take the initial leaf expansions from ppmd's FMM)
+// ============================================================================
+// Working variables are scaled Racah harmonics (m >= 0 stored, packed index
+// l(l+1)/2 + m; the m < 0 entries are (-1)^m conj of the m > 0 ones):
+//   R~_l^m(x) = R_l^m(x) / N_lm,  I~_l^m(x) = I_l^m(x) N_lm,  N_lm = sqrt((l-m)!(l+m)!)
+// (R_l^m = r^l Y_l^m, I_l^m = r^{-l-1} Y_l^m as in the rest of this file).  At a level
+// with cell width w:
+//   multipole  Qh_lm = sum_i q_i conj(R~_l^m((x_i - c) / w))
+//   local      Lh_jk = w^{j+1} N_jk L_jk   (L_jk the leaf-expansion layout)
+// so the translations do not depend on the level:
+//   M2M  Qh_p[j,k] = 2^-j sum Qh_c[l,m] conj(R~_{j-l}^{k-m}(b)),  b = (child - parent) / w_child
+//   M2L  Lh_t[j,k] = (-1)^k sum (-1)^l I~_{j+l}^{m-k}(o) Qh_s[l,m],  o = (c_s - c_t) / w
+//   L2L  Lh_c[l,m] = sum_{j>=l} 2^-(j+1) Lh_p[j,k] R~_{j-l}^{k-m}(b)
+// The two finest levels' interaction lists (the 12^3 leaves around the leaf's
+// grandparent block, minus the 27 near leaves) are added by direct P2L of their
+// charges; M2L runs on levels 1 (2 for free space) .. R-2 only.
+
+__host__ __device__ inline double rec_alpha(int l, int m) { return (2.0 * l - 1.0) / sqrt((double)(l * l - m * m)); }
+__host__ __device__ inline double rec_beta(int l, int m) {
+    return sqrt((double)((l - 1) * (l - 1) - m * m) / (double)(l * l - m * m));
+}
+__host__ __device__ inline void half_lm(int h, int& l, int& m) {
+    l = 0;
+    while ((l + 1) * (l + 2) / 2 <= h) l++;
+    m = h - l * (l + 1) / 2;
+}
+
+// regular (IRR = false) or irregular (IRR = true) Schmidt solid harmonic of one (l, m), m >= 0
+template <bool IRR>
+__host__ __device__ inline Cplx solid_lm(int l, int m, double x, double y, double z) {
+    const double r2 = x * x + y * y + z * z;
+    double s = 1.0;
+    for (int k = 1; k <= m; k++) s *= -sqrt((2.0 * k - 1.0) / (2.0 * k));
+    Cplx w = {1.0, 0.0};
+    for (int k = 0; k < m; k++) w = Cplx{w.re * x - w.im * y, w.re * y + w.im * x};
+    double b1 = s, b2 = 0.0;
+    if (IRR) {
+        const double ir2 = 1.0 / r2;
+        double p = 1.0 / sqrt(r2);
+        for (int k = 0; k < m; k++) p *= ir2;
+        b1 = s * p;
+        for (int k = m + 1; k <= l; k++) {
+            const double b = (rec_alpha(k, m) * z * b1 - rec_beta(k, m) * b2) * ir2;
+            b2 = b1;
+            b1 = b;
+        }
+    } else {
+        for (int k = m + 1; k <= l; k++) {
+            const double b = rec_alpha(k, m) * z * b1 - rec_beta(k, m) * r2 * b2;
+            b2 = b1;
+            b1 = b;
+        }
+    }
+    return Cplx{b1 * w.re, b1 * w.im};
+}
+
+__device__ __forceinline__ double2 conj_sgn(double2 v, int m) {   // (-1)^m conj(v)
+    return (m & 1) ? make_double2(-v.x, v.y) : make_double2(v.x, -v.y);
+}
+// entry (l, m) of a packed half expansion for any -l <= m <= l
+__device__ __forceinline__ double2 half_get(const double2* a, int l, int m) {
+    const double2 v = a[hidx(l, m < 0 ? -m : m)];
+    return m < 0 ? conj_sgn(v, -m) : v;
+}
+
+struct FmmLevel {
+    int n;                 // cells per side
+    double w;              // cell width
+};
+
+// P2M straight from the particles of the leaves in each cell of a level:
+// thread per (cell, packed (l, m)).
+__global__ void __launch_bounds__(256) k_fmm_p2m(int L, FmmLevel lv, Geom g, const int* occ, const double4* slot_pos,
+                                                 int occmax, const double* nrm, double2* Q) {
+    const int nh = L * (L + 1) / 2;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long ncell = (long long)lv.n * lv.n * lv.n;
+    if (i >= ncell * nh) return;
+    const int c = (int)(i / nh), h = (int)(i % nh);
+    int l, m;
+    half_lm(h, l, m);
+    const int cx = c % lv.n, cy = (c / lv.n) % lv.n, cz = c / (lv.n * lv.n);
+    const double x0 = -0.5 * g.E + (cx + 0.5) * lv.w, y0 = -0.5 * g.E + (cy + 0.5) * lv.w,
+                 z0 = -0.5 * g.E + (cz + 0.5) * lv.w;
+    const int f = g.ncps / lv.n;
+    const double iw = 1.0 / lv.w;
+    double sr = 0.0, si = 0.0;
+    for (int lz = 0; lz < f; lz++)
+        for (int ly = 0; ly < f; ly++)
+            for (int lx = 0; lx < f; lx++) {
+                const int leaf = (cx * f + lx) + g.ncps * ((cy * f + ly) + g.ncps * (cz * f + lz));
+                const int o = occ[leaf];
+                for (int j = 0; j < o; j++) {
+                    const double4 p = slot_pos[(size_t)leaf * occmax + j];
+                    const Cplx r = solid_lm<false>(l, m, (p.x - x0) * iw, (p.y - y0) * iw, (p.z - z0) * iw);
+                    sr = fma(p.w, r.re, sr);
+                    si = fma(-p.w, r.im, si);
+                }
+            }
+    const double inv = 1.0 / nrm[hidx(l, m)];
+    Q[(size_t)c * nh + h] = make_double2(sr * inv, si * inv);
+}
+
+// M2M: thread per (parent cell, packed (j, k)); rm[oct][n][mm + L - 1] = conj(R~_n^mm(b_oct))
+__global__ void __launch_bounds__(256) k_fmm_m2m(int L, int np, const double2* Qc, const double2* rm, double2* Qp) {
+    const int nh = L * (L + 1) / 2;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)np * np * np * nh) return;
+    const int p = (int)(i / nh), h = (int)(i % nh);
+    int j, k;
+    half_lm(h, j, k);
+    const int px = p % np, py = (p / np) % np, pz = p / (np * np);
+    const int nc = 2 * np;
+    double ar = 0.0, ai = 0.0;
+    for (int oct = 0; oct < 8; oct++) {
+        const int cx = 2 * px + (oct & 1), cy = 2 * py + ((oct >> 1) & 1), cz = 2 * pz + (oct >> 2);
+        const double2* q = Qc + ((size_t)cx + (size_t)nc * (cy + (size_t)nc * cz)) * nh;
+        const double2* t = rm + (size_t)oct * L * (2 * L - 1);
+        for (int l = 0; l <= j; l++)
+            for (int m = -l; m <= l; m++) {
+                const int mm = k - m;
+                if (mm > j - l || mm < -(j - l)) continue;
+                const double2 a = half_get(q, l, m);
+                const double2 b = t[(j - l) * (2 * L - 1) + mm + L - 1];
+                ar = fma(a.x, b.x, fma(-a.y, b.y, ar));
+                ai = fma(a.x, b.y, fma(a.y, b.x, ai));
+            }
+    }
+    const double sc = ldexp(1.0, -j);
+    Qp[(size_t)p * nh + h] = make_double2(ar * sc, ai * sc);
+}
+
+// L2L: thread per (child cell, packed (l, m)), l < Lout, from a parent expansion of
+// order Lin >= Lout; rl[oct][n][mm + Lin - 1] = R~_n^mm(b_oct).  to_leaf: write the
+// leaf-expansion layout L_lm = Lh_lm / (w^{l+1} N_lm) instead.
+__global__ void __launch_bounds__(256) k_fmm_l2l(int Lin, int Lout, int nc, const double2* Lp, const double2* rl,
+                                                 double2* Lc, int to_leaf, double w, const double* nrm) {
+    const int nhi = Lin * (Lin + 1) / 2, nho = Lout * (Lout + 1) / 2;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)nc * nc * nc * nho) return;
+    const int c = (int)(i / nho), h = (int)(i % nho);
+    int l, m;
+    half_lm(h, l, m);
+    const int cx = c % nc, cy = (c / nc) % nc, cz = c / (nc * nc);
+    const int np = nc / 2;
+    const int oct = (cx & 1) | ((cy & 1) << 1) | ((cz & 1) << 2);
+    const double2* lp = Lp + ((size_t)(cx >> 1) + (size_t)np * ((cy >> 1) + (size_t)np * (cz >> 1))) * nhi;
+    const int TW = 2 * Lin - 1;
+    const double2* t = rl + (size_t)oct * Lin * TW;
+    double ar = 0.0, ai = 0.0;
+    for (int j = l; j < Lin; j++) {
+        double sr = 0.0, si = 0.0;
+        for (int k = -j; k <= j; k++) {
+            const int mm = k - m;
+            if (mm > j - l || mm < -(j - l)) continue;
+            const double2 a = half_get(lp, j, k);
+            const double2 b = t[(j - l) * TW + mm + Lin - 1];
+            sr = fma(a.x, b.x, fma(-a.y, b.y, sr));
+            si = fma(a.x, b.y, fma(a.y, b.x, si));
+        }
+        const double sc = ldexp(1.0, -(j + 1));
+        ar = fma(sc, sr, ar);
+        ai = fma(sc, si, ai);
+    }
+    if (to_leaf) {
+        const double sc = 1.0 / (pow(w, (double)(l + 1)) * nrm[hidx(l, m)]);
+        ar *= sc;
+        ai *= sc;
+    }
+    Lc[(size_t)c * nho + h] = make_double2(ar, ai);
+}
+
+// M2L at one level: block per target cell, lane per packed output (j, k).  For each
+// source of the target's interaction list (children of the parent's neighbours not
+// adjacent to the target; periodic: indices wrap, the displacement does not) the
+// block builds the I~(o) table in LDS and stages the source multipole.
+constexpr int FMM_M2L_THREADS = 256;
+__global__ void __launch_bounds__(FMM_M2L_THREADS) k_fmm_m2l(int L, int n, int periodic, const double2* Q,
+                                                             const double* nrm, double2* Lout) {
+    extern __shared__ double2 fmm_lds[];
+    const int nh = L * (L + 1) / 2;
+    const int P = 2 * L - 1;             // orders 0 .. 2L-2
+    const int QW = 4 * L - 3;            // q = -(2L-2) .. 2L-2
+    double2* itab = fmm_lds;             // [P][QW]
+    double2* qs = fmm_lds + P * QW;      // [nh]
+    const int t = blockIdx.x;
+    const int tx = t % n, ty = (t / n) % n, tz = t / (n * n);
+    const int tid = threadIdx.x;
+    int j = 0, k = 0;
+    if (tid < nh) half_lm(tid, j, k);
+    double ar = 0.0, ai = 0.0;
+    for (int oz = -2 - (tz & 1); oz <= 3 - (tz & 1); oz++)
+        for (int oy = -2 - (ty & 1); oy <= 3 - (ty & 1); oy++)
+            for (int ox = -2 - (tx & 1); ox <= 3 - (tx & 1); ox++) {
+                if (ox >= -1 && ox <= 1 && oy >= -1 && oy <= 1 && oz >= -1 && oz <= 1) continue;
+                int sx = tx + ox, sy = ty + oy, sz = tz + oz;
+                if (periodic) {
+                    sx = (sx + 2 * n) % n;
+                    sy = (sy + 2 * n) % n;
+                    sz = (sz + 2 * n) % n;
+                } else if (sx < 0 || sx >= n || sy < 0 || sy >= n || sz < 0 || sz >= n) {
+                    continue;
+                }
+                __syncthreads();   // previous source finished with the LDS tables
+                if (tid < P) {
+                    // column q = tid of I~_p^q(o), p = q .. 2L-2, by the irregular recurrence
+                    const int q = tid;
+                    const double x = ox, y = oy, z = oz;
+                    const double r2 = x * x + y * y + z * z, ir2 = 1.0 / r2;
+                    double sq = 1.0, pw = 1.0 / sqrt(r2);
+                    Cplx wq = {1.0, 0.0};
+                    for (int kk = 1; kk <= q; kk++) {
+                        sq *= -sqrt((2.0 * kk - 1.0) / (2.0 * kk));
+                        pw *= ir2;
+                        wq = Cplx{wq.re * x - wq.im * y, wq.re * y + wq.im * x};
+                    }
+                    double b2 = 0.0, b1 = sq * pw;
+                    for (int p = q; p < P; p++) {
+                        if (p > q) {
+                            const double b = (rec_alpha(p, q) * z * b1 - rec_beta(p, q) * b2) * ir2;
+                            b2 = b1;
+                            b1 = b;
+                        }
+                        const double f = b1 * nrm[hidx(p, q)];
+                        const double2 e = make_double2(f * wq.re, f * wq.im);
+                        itab[p * QW + q + P - 1] = e;
+                        if (q > 0) itab[p * QW - q + P - 1] = conj_sgn(e, q);
+                    }
+                }
+                const double2* src = Q + ((size_t)sx + (size_t)n * (sy + (size_t)n * sz)) * nh;
+                for (int h = tid; h < nh; h += blockDim.x) qs[h] = src[h];
+                __syncthreads();
+                if (tid < nh) {
+                    for (int l = 0; l < L; l++) {
+                        const double sg = (l & 1) ? -1.0 : 1.0;
+                        double sr = 0.0, si = 0.0;
+                        for (int m = -l; m <= l; m++) {
+                            const double2 a = itab[(j + l) * QW + (m - k) + P - 1];
+                            const double2 b = m < 0 ? conj_sgn(qs[hidx(l, -m)], -m) : qs[hidx(l, m)];
+                            sr = fma(a.x, b.x, fma(-a.y, b.y, sr));
+                            si = fma(a.x, b.y, fma(a.y, b.x, si));
+                        }
+                        ar = fma(sg, sr, ar);
+                        ai = fma(sg, si, ai);
+                    }
+                }
+            }
+    if (tid < nh) {
+        const double sk = (k & 1) ? -1.0 : 1.0;
+        double2 v = Lout[(size_t)t * nh + tid];
+        v.x = fma(sk, ar, v.x);
+        v.y = fma(sk, ai, v.y);
+        Lout[(size_t)t * nh + tid] = v;
+    }
+}
+
+// Leaf P2L of the charges in the 12^3 leaves around the leaf's grandparent block
+// (offsets o = -4-b .. 7-b per axis, b = leaf index mod 4) except its 27 near leaves.
+// One wave per 64 leaves sharing b; grid.y = column-pair group (as the accept sweep).
+template <int L, int G, int NP>
+__device__ __forceinline__ void p2l_near_group(const SweepArgs& a, const int* occ, const double4* slot_pos, int occmax,
+                                               int cell, int bx, int by, int bz) {
+    constexpr int P0 = G * NP;
+    constexpr int W = L + 1;
+    double ar[NP][W], ai[NP][W];
+#pragma unroll
+    for (int jj = 0; jj < NP; jj++)
+#pragma unroll
+        for (int kk = 0; kk < W; kk++) { ar[jj][kk] = 0.0; ai[jj][kk] = 0.0; }
+    const int n = a.g.ncps;
+    const double E = a.g.E;
+    const bool periodic = a.g.bc != KMC_BC_FREE_SPACE;
+    const int tx = cell % n, ty = (cell / n) % n, tz = cell / (n * n);
+    const double cx = a.g.c0 + tx * a.g.w, cy = a.g.c0 + ty * a.g.w, cz = a.g.c0 + tz * a.g.w;
+#pragma unroll 1
+    for (int oz = -4 - bz; oz <= 7 - bz; oz++) {
+        int sz = tz + oz;
+        double shz = 0.0;
+        if (sz < 0) { sz += n; shz = -E; } else if (sz >= n) { sz -= n; shz = E; }
+        if (!periodic && shz != 0.0) continue;
+#pragma unroll 1
+        for (int oy = -4 - by; oy <= 7 - by; oy++) {
+            int sy = ty + oy;
+            double shy = 0.0;
+            if (sy < 0) { sy += n; shy = -E; } else if (sy >= n) { sy -= n; shy = E; }
+            if (!periodic && shy != 0.0) continue;
+#pragma unroll 1
+            for (int ox = -4 - bx; ox <= 7 - bx; ox++) {
+                if (ox >= -1 && ox <= 1 && oy >= -1 && oy <= 1 && oz >= -1 && oz <= 1) continue;
+                int sx = tx + ox;
+                double shx = 0.0;
+                if (sx < 0) { sx += n; shx = -E; } else if (sx >= n) { sx -= n; shx = E; }
+                if (!periodic && shx != 0.0) continue;
+                const int s = sx + n * (sy + n * sz);
+                const int o = occ[s];
+#pragma unroll 1
+                for (int q = 0; q < o; q++) {
+                    const double4 p = slot_pos[(size_t)s * occmax + q];
+                    const double dx[1] = {(p.x + shx) - cx};
+                    const double qe[1] = {p.w};
+                    sweep_images<L, P0, NP, 1>(dx, (p.y + shy) - cy, (p.z + shz) - cz, qe, ar, ai);
+                }
+            }
+        }
+    }
+    sweep_group_store<L, P0, NP>(a.lexp + (size_t)cell * nhalf(L), ar, ai);
+}
+template <int L, int NP, int G0>
+__device__ __forceinline__ void p2l_near_dispatch(const SweepArgs& a, const int* occ, const double4* slot_pos, int occmax,
+                                                  int cell, int group, int bx, int by, int bz) {
+    constexpr int NG = ((L + 1) / 2 + NP - 1) / NP;
+    if constexpr (G0 < NG) {
+        if (group == G0) p2l_near_group<L, G0, NP>(a, occ, slot_pos, occmax, cell, bx, by, bz);
+        else p2l_near_dispatch<L, NP, G0 + 1>(a, occ, slot_pos, occmax, cell, group, bx, by, bz);
+    }
+}
+// grid.x = 64 (b classes) x ceil(ng^3 / 64) chunks, ng = ncps / 4; block = 64
+template <int L>
+__global__ void __launch_bounds__(64) k_fmm_p2l_near(SweepArgs a, const int* occ, const double4* slot_pos, int occmax) {
+    const int b = blockIdx.x % 64;
+    const int chunk = blockIdx.x / 64;
+    const int bx = b & 3, by = (b >> 2) & 3, bz = b >> 4;
+    const int ng = a.g.ncps / 4;
+    const int gi = chunk * 64 + threadIdx.x;
+    if (gi >= ng * ng * ng) return;
+    const int gx = gi % ng, gy = (gi / ng) % ng, gz = gi / (ng * ng);
+    const int cell = (4 * gx + bx) + a.g.ncps * ((4 * gy + by) + a.g.ncps * (4 * gz + bz));
+    p2l_near_dispatch<L, 1, 0>(a, occ, slot_pos, occmax, cell, (int)blockIdx.y, bx, by, bz);
+}
+
+// ============================================================================
+// potential at points (KMCFMM.eval_field, coulomb_kmc/kmc_fmm.py:544-567)
+// ============================================================================
+// evaluation points: unit charge, leaf of the point, no exclusion
+__global__ void k_eval_points(Geom g, int n, const double* pts, double4* pos4, int32_t* cell, int32_t* excl) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
+    pos4[i] = make_double4(x, y, z, 1.0);
+    const int nc = g.ncps;
+    cell[i] = bin_axis(x, g.E, g.w, nc) + nc * (bin_axis(y, g.E, g.w, nc) + nc * bin_axis(z, g.E, g.w, nc));
+    excl[i] = -1;
+}
+
+// long-range potential v(x) . g, v = packed multipole of a unit charge at x about
+// the origin and g = G M: the first-order change of 0.5 M.(G M)
+// (the reference evaluates its long-range local expansion at x,
+// coulomb_kmc/kmc_full_long_range.py:341-387)
+__global__ void k_eval_lr(int L, int n, const double4* pos4, const double* g, double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double4 p = pos4[i];
+    const double r2 = p.x * p.x + p.y * p.y + p.z * p.z;
+    double acc = 0.0, s = 1.0;
+    Cplx C = {1.0, 0.0};
+    for (int m = 0; m < L; m++) {
+        if (m > 0) {
+            s *= -sqrt((2.0 * m - 1.0) / (2.0 * m));
+            C = Cplx{C.re * p.x + C.im * p.y, C.im * p.x - C.re * p.y};   // (x - iy)^m
+        }
+        double b2 = 0.0, b1 = s;
+        for (int l = m; l < L; l++) {
+            if (l > m) {
+                const double b = rec_alpha(l, m) * p.z * b1 - rec_beta(l, m) * r2 * b2;
+                b2 = b1;
+                b1 = b;
+            }
+            const int h = hidx(l, m);
+            acc = fma(b1 * C.re, g[2 * h], fma(b1 * C.im, g[2 * h + 1], acc));
+        }
+    }
+    out[i] = acc;
+}
+
+__global__ void k_eval_combine(int n, const double* ud, const double* ui, int nfar, const double* lr, double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double v = 0.0;
+    if (ud) {
+        v = ud[i];
+        for (int j = 0; j < nfar; j++) v += ui[(size_t)j * n + i];
+    }
+    if (lr) v += lr[i];
+    out[i] = v;
+}
+
+// ============================================================================
 // order dispatch tables
 // ============================================================================
 typedef void (*SweepLaunch)(SweepArgs, int, hipStream_t, int, int, int);
@@ -1350,6 +1778,14 @@ typedef void (*EvalLaunch)(EvalArgs, hipStream_t);
 typedef void (*LRLaunch)(LR2Args, hipStream_t);
 typedef void (*MpLaunch)(int, int, const double4*, double*, int, hipStream_t);
 typedef void (*MdLaunch)(TailArgs, hipStream_t);
+typedef void (*P2LNearLaunch)(SweepArgs, const int*, const double4*, int, hipStream_t);
+
+template <int L>
+static void launch_p2l_near(SweepArgs a, const int* occ, const double4* slot_pos, int occmax, hipStream_t s) {
+    const int ng = a.g.ncps / 4;
+    const int chunks = (ng * ng * ng + 63) / 64;
+    hipLaunchKernelGGL((k_fmm_p2l_near<L>), dim3(64 * chunks, (L + 1) / 2), dim3(64), 0, s, a, occ, slot_pos, occmax);
+}
 
 template <int L, int NP>
 static void launch_sweep_np(SweepArgs a, int ncells, hipStream_t s, int block, int split) {
@@ -1394,10 +1830,11 @@ struct OrderFns {
     LRLaunch lr;
     MpLaunch mp;
     MdLaunch md;
+    P2LNearLaunch p2ln;
 };
 template <int L>
 static OrderFns fns_for() {
-    return {launch_sweep<L>, launch_far<L>, launch_lr<L>, launch_mp<L>, launch_md<L>};
+    return {launch_sweep<L>, launch_far<L>, launch_lr<L>, launch_mp<L>, launch_md<L>, launch_p2l_near<L>};
 }
 static bool order_fns(int L, OrderFns* f) {
     switch (L) {
@@ -1454,6 +1891,10 @@ struct DBuf {
         p = nullptr;
         n = 0;
     }
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
 };
 
 struct kmc_ctx {
@@ -1467,6 +1908,7 @@ struct kmc_ctx {
     int sweep_block = 256;
     int sweep_pairs = 1;
     int sweep_split = 1;
+    int init_method = 0;              // 0 automatic, 1 direct P2L, 2 hierarchical
     int64_t last_P = 0, last_T = 0;   // sizes of the last propose (kernel timing)
     // side stream for work that overlaps within one call (long range beside the near and
     // far fields; cell tables and multipole beside the accept sweep), forked from and
@@ -1500,7 +1942,7 @@ struct kmc_ctx {
     // propose work buffers
     DBuf<double4> epos4;
     DBuf<int32_t> ecell, eexcl, owner;
-    DBuf<double> UD, UI, lrd, dE1, rates;
+    DBuf<double> UD, UI, lrd, dE1, rates, bsum;
     DBuf<int64_t> one_pid, one_es;
     DBuf<double> one_pos;
     // propose_with_dats work buffers
@@ -1525,6 +1967,7 @@ static int ensure_eval(kmc_ctx* c, int64_t T, int64_t P) {
     HIPCHK(c->UI.ensure(ne * (size_t)((c->L + 1) / 2)));
     HIPCHK(c->lrd.ensure((size_t)12 * std::max<int64_t>(T, 1)));
     HIPCHK(c->rates.ensure((size_t)std::max<int64_t>(T, 1)));
+    HIPCHK(c->bsum.ensure((size_t)std::max<int64_t>((T + 63) / 64, 1)));
     return KMC_OK;
 }
 
@@ -1618,6 +2061,7 @@ static int run_propose_core(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, dou
     ca.out = d_out;
     ca.kT = kT;
     ca.rates = rates;
+    ca.bsum = rates ? c->bsum.p : nullptr;
     if (T > 0) {
         if (mixed) hipLaunchKernelGGL(k_combine<true>, dim3((T + 63) / 64), dim3(64), 0, s, ca);
         else hipLaunchKernelGGL(k_combine<false>, dim3((T + 63) / 64), dim3(64), 0, s, ca);
@@ -1801,7 +2245,7 @@ int kmc_destroy(kmc_ctx* c) {
     c->UI.release(); c->lrd.release(); c->dE1.release(); c->one_pid.release(); c->one_es.release();
     c->one_pos.release(); c->d_cnt.release(); c->d_has.release(); c->d_slot.release(); c->d_off.release();
     c->d_pid.release(); c->d_es.release(); c->d_rate.release(); c->d_tot.release(); c->d_npos.release();
-    c->d_nq.release(); c->rates.release();
+    c->d_nq.release(); c->rates.release(); c->bsum.release();
     for (int i = 0; i < 2; i++) {
         if (c->ev_fork[i]) hipEventDestroy(c->ev_fork[i]);
         if (c->ev_join[i]) hipEventDestroy(c->ev_join[i]);
@@ -1935,6 +2379,112 @@ int kmc_set_lr_operator(kmc_ctx* c, int64_t nnz, const double* h_data, const int
     return KMC_OK;
 }
 
+// Hierarchical initial solve of the leaf expansions (see the kernels above).
+static bool use_hierarchical(const kmc_ctx* c) {
+    if (c->bc == KMC_BC_FF_ONLY || c->ncps < 8) return false;
+    if (c->init_method == 1) return false;
+    if (c->init_method == 2) return true;
+    return (double)c->n * (double)c->ncells > 2.0e10;
+}
+
+static int run_fmm_init(kmc_ctx* c, hipStream_t s) {
+    // translations run at an internal order Li > L (M2L truncation error falls
+    // geometrically with the order); the leaves receive orders l < L
+    const int L = std::min(c->L + 4, KMAX_L), nh = nhalf(L);
+    const int Lleaf = c->L;
+    int R = 0;
+    while ((1 << R) < c->ncps) R++;
+    const bool periodic = c->bc != KMC_BC_FREE_SPACE;
+    const int lo = periodic ? 1 : 2, hi = R - 2;
+    // tables: N_pq for p <= 2L-2; conj(R~) (M2M) and R~ (L2L) at the 8 child offsets
+    const int P = 2 * L - 1;
+    std::vector<double> nrm(nhalf(P));
+    for (int p = 0; p < P; p++)
+        for (int q = 0; q <= p; q++) nrm[hidx(p, q)] = exp(0.5 * (lgamma(p - q + 1.0) + lgamma(p + q + 1.0)));
+    const int TW = 2 * L - 1;
+    std::vector<double2> rm((size_t)8 * L * TW, make_double2(0.0, 0.0)), rl(rm.size(), make_double2(0.0, 0.0));
+    for (int oct = 0; oct < 8; oct++) {
+        const double bx = (oct & 1) - 0.5, by = ((oct >> 1) & 1) - 0.5, bz = (oct >> 2) - 0.5;
+        for (int nn = 0; nn < L; nn++)
+            for (int mm = 0; mm <= nn; mm++) {
+                const Cplx v = solid_lm<false>(nn, mm, bx, by, bz);
+                const double iv = 1.0 / nrm[hidx(nn, mm)];
+                const double2 r = make_double2(v.re * iv, v.im * iv);               // R~_n^m
+                const double sg = (mm & 1) ? -1.0 : 1.0;
+                const double2 rneg = make_double2(sg * r.x, -sg * r.y);           // R~_n^-m
+                const size_t base = ((size_t)oct * L + nn) * TW + L - 1;
+                rl[base + mm] = r;
+                rl[base - mm] = rneg;
+                rm[base + mm] = make_double2(r.x, -r.y);
+                rm[base - mm] = make_double2(rneg.x, -rneg.y);
+            }
+    }
+    DBuf<double> d_nrm;
+    DBuf<double2> d_rm, d_rl;
+    HIPCHK(d_nrm.ensure(nrm.size()));
+    HIPCHK(d_rm.ensure(rm.size()));
+    HIPCHK(d_rl.ensure(rl.size()));
+    HIPCHK(hipMemcpy(d_nrm.p, nrm.data(), nrm.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_rm.p, rm.data(), rm.size() * sizeof(double2), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_rl.p, rl.data(), rl.size() * sizeof(double2), hipMemcpyHostToDevice));
+    const Geom g = c->geom();
+    auto cells = [](int lev) { return (long long)1 << (3 * lev); };
+    auto grid = [nh](long long ncell) { return dim3((unsigned)((ncell * nh + 255) / 256)); };
+    std::vector<DBuf<double2>> Q(R + 1), Lv(R + 1);
+    if (hi >= lo) {
+        HIPCHK(Q[hi].ensure((size_t)cells(hi) * nh));
+        FmmLevel lv{1 << hi, c->E / (1 << hi)};
+        hipLaunchKernelGGL(k_fmm_p2m, grid(cells(hi)), dim3(256), 0, s, L, lv, g, c->occ.p, c->slot_pos.p, c->occmax,
+                           d_nrm.p, Q[hi].p);
+        for (int lev = hi - 1; lev >= lo; lev--) {
+            HIPCHK(Q[lev].ensure((size_t)cells(lev) * nh));
+            hipLaunchKernelGGL(k_fmm_m2m, grid(cells(lev)), dim3(256), 0, s, L, 1 << lev, Q[lev + 1].p, d_rm.p,
+                               Q[lev].p);
+        }
+        const int m2l_threads = std::max(64, ((std::max(nh, P) + 63) / 64) * 64);
+        const size_t m2l_lds = ((size_t)P * (4 * L - 3) + nh) * sizeof(double2);
+        for (int lev = lo; lev <= hi; lev++) {
+            HIPCHK(Lv[lev].ensure((size_t)cells(lev) * nh));
+            if (lev == lo) {
+                HIPCHK(hipMemsetAsync(Lv[lev].p, 0, (size_t)cells(lev) * nh * sizeof(double2), s));
+            } else {
+                hipLaunchKernelGGL(k_fmm_l2l, grid(cells(lev)), dim3(256), 0, s, L, L, 1 << lev, Lv[lev - 1].p,
+                                   d_rl.p, Lv[lev].p, 0, 0.0, d_nrm.p);
+            }
+            hipLaunchKernelGGL(k_fmm_m2l, dim3((unsigned)cells(lev)), dim3(m2l_threads), m2l_lds, s, L, 1 << lev,
+                               periodic ? 1 : 0, Q[lev].p, d_nrm.p, Lv[lev].p);
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(Lv[R - 1].ensure((size_t)cells(R - 1) * nh));
+        hipLaunchKernelGGL(k_fmm_l2l, grid(cells(R - 1)), dim3(256), 0, s, L, L, 1 << (R - 1), Lv[hi].p, d_rl.p,
+                           Lv[R - 1].p, 0, 0.0, d_nrm.p);
+        const long long nleaf_out = cells(R) * c->nh;
+        hipLaunchKernelGGL(k_fmm_l2l, dim3((unsigned)((nleaf_out + 255) / 256)), dim3(256), 0, s, L, Lleaf, 1 << R,
+                           Lv[R - 1].p, d_rl.p, c->lexp.p, 1, c->w, d_nrm.p);
+        HIPCHK(hipGetLastError());
+    }
+    SweepArgs sa;
+    sa.g = g;
+    sa.lexp = c->lexp.p;
+    sa.ncells = (int)c->ncells;
+    sa.mode = 1;
+    sa.rec = nullptr;
+    sa.src_pos4 = nullptr;
+    sa.src_cell = nullptr;
+    sa.src_begin = sa.src_end = 0;
+    c->fns.p2ln(sa, c->occ.p, c->slot_pos.p, c->occmax, s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    return KMC_OK;
+}
+
+int kmc_set_init_method(kmc_ctx* c, int32_t method) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    if (method < 0 || method > 2) return set_err(KMC_E_ARG, "init method must be 0 (auto), 1 (direct) or 2 (hierarchical)");
+    c->init_method = method;
+    return KMC_OK;
+}
+
 int kmc_initialise(kmc_ctx* c, void* stream, double* h_energy) {
     if (check_ctx(c)) return KMC_E_ARG;
     if (!c->have_particles) return set_err(KMC_E_STATE, "set particles before initialise");
@@ -1943,7 +2493,10 @@ int kmc_initialise(kmc_ctx* c, void* stream, double* h_energy) {
     hipStream_t s = (hipStream_t)stream;
     const int n = (int)c->n;
     HIPCHK(hipMemsetAsync(c->lexp.p, 0, (size_t)c->ncells * c->nh * sizeof(double2), s));
-    if (c->bc != KMC_BC_FF_ONLY) {
+    if (use_hierarchical(c)) {
+        int rc = run_fmm_init(c, s);
+        if (rc) return rc;
+    } else if (c->bc != KMC_BC_FF_ONLY) {
         // P2L of every charge into every well-separated leaf, in source chunks
         const int chunk = 2048;
         for (int b = 0; b < n; b += chunk) {
@@ -2217,7 +2770,8 @@ int kmc_step(kmc_ctx* c, void* stream, int64_t P, int64_t T, const int64_t* d_pi
     if (!c->initialised) return set_err(KMC_E_STATE, "Run initialise before this call");
     if (P < 1 || T < 1 || T >= (int64_t)1 << 30 || nsteps < 1) return set_err(KMC_E_ARG, "bad proposal counts");
     const bool lr_sized = !c->have_lr || ((size_t)P * 2 * c->nh <= c->aold.n && (size_t)12 * P <= c->cold.n);
-    if ((size_t)(T + P) > c->epos4.n || (size_t)T > c->owner.n || (size_t)T > c->rates.n || !lr_sized)
+    if ((size_t)(T + P) > c->epos4.n || (size_t)T > c->owner.n || (size_t)T > c->rates.n ||
+        (size_t)((T + 63) / 64) > c->bsum.n || !lr_sized)
         return set_err(KMC_E_STATE, "work buffers not sized: call kmc_propose once with this T before kmc_step");
     hipStream_t s = (hipStream_t)stream;
     int rc = launch_setup(c, s, P, T, d_pid, d_es, d_new_pos, nullptr);
@@ -2232,6 +2786,7 @@ int kmc_step(kmc_ctx* c, void* stream, int64_t P, int64_t T, const int64_t* d_pi
     sa.kT = kT;
     sa.dE = d_dE;
     sa.rates = c->rates.p;
+    sa.bsum = c->bsum.p;
     sa.uniform = d_uniform;
     sa.counter = d_counter;
     sa.selected = d_selected;
@@ -2323,6 +2878,41 @@ int kmc_time_kernel(kmc_ctx* c, void* stream, int32_t which, int32_t reps, int32
     hipEventDestroy(e1);
     *h_ms = (double)ms / reps;
     HIPCHK(hipGetLastError());
+    return KMC_OK;
+}
+
+int kmc_eval_field(kmc_ctx* c, void* stream, int64_t n, const double* d_points, double* d_out) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    if (!c->initialised) return set_err(KMC_E_STATE, "Run initialise before this call");
+    if (n < 0 || n >= (int64_t)1 << 30) return set_err(KMC_E_ARG, "bad point count");
+    if (n == 0) return KMC_OK;
+    if (!d_points || !d_out) return set_err(KMC_E_ARG, "null pointer");
+    hipStream_t s = (hipStream_t)stream;
+    const int ni = (int)n, nfar = (c->L + 1) / 2;
+    DBuf<double4> pos4;
+    DBuf<int32_t> cell, excl;
+    DBuf<double> ud, ui, lr;
+    HIPCHK(pos4.ensure(ni));
+    HIPCHK(cell.ensure(ni));
+    HIPCHK(excl.ensure(ni));
+    hipLaunchKernelGGL(k_eval_points, dim3((ni + 255) / 256), dim3(256), 0, s, c->geom(), ni, d_points, pos4.p, cell.p,
+                       excl.p);
+    const bool near_far = c->bc != KMC_BC_FF_ONLY;
+    const bool lrt = (c->bc == KMC_BC_PBC || c->bc == KMC_BC_FF_ONLY) && c->have_lr;
+    if (near_far) {
+        HIPCHK(ud.ensure(ni));
+        HIPCHK(ui.ensure((size_t)ni * nfar));
+        launch_near(eval_args(c, ni, pos4.p, cell.p, excl.p, ud.p), c->near_lanes, false, s);
+        c->fns.far(eval_args(c, ni, pos4.p, cell.p, excl.p, ui.p), s);
+    }
+    if (lrt) {
+        HIPCHK(lr.ensure(ni));
+        hipLaunchKernelGGL(k_eval_lr, dim3((ni + 255) / 256), dim3(256), 0, s, c->L, ni, pos4.p, c->gvec.p, lr.p);
+    }
+    hipLaunchKernelGGL(k_eval_combine, dim3((ni + 255) / 256), dim3(256), 0, s, ni, near_far ? ud.p : nullptr,
+                       near_far ? ui.p : nullptr, nfar, lrt ? lr.p : nullptr, d_out);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
     return KMC_OK;
 }
 

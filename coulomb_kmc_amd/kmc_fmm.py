@@ -47,6 +47,9 @@ def _as_array(obj, ncomp, dtype):
     return a
 
 
+INIT_METHODS = {"auto": 0, "direct": 1, "fmm": 2}
+
+
 @lru_cache(maxsize=16)
 def _lr_operator_csr(L, extent):
     G = lattice.half_operator(L, extent)
@@ -84,11 +87,14 @@ class KMCFMM:
     :arg int l: number of expansion terms L (2..20).
     :arg float max_move: maximum distance of a proposed move (default: the extent).
     :arg str precision: 'fp64' or 'mixed' arithmetic for the propose kernels.
+    :arg str init_method: how ``initialise`` builds the leaf expansions: 'direct' (exact
+        P2L of every charge into every leaf), 'fmm' (hierarchical, M2L truncation at
+        order l, needs r >= 4) or 'auto' (hierarchical for large systems).
     """
 
     def __init__(self, positions, charges, domain, N=None, boundary_condition="pbc", r=None, shell_width=0.0,
                  energy_unit=1.0, _debug=False, l=None, max_move=None, cuda_direct=False, mirror_direction=None,
-                 precision="fp64", occ_headroom=8):
+                 precision="fp64", occ_headroom=8, init_method="auto"):
         if mirror_direction is not None:
             raise NotImplementedError("mirror charges (Dirichlet boundaries) are not part of this implementation")
         self._bc = BCType(boundary_condition)
@@ -122,6 +128,9 @@ class KMCFMM:
         _native.check(self._lib.kmc_create(ctypes.byref(self._ctx), self.N, self.extent, self.R, self.L,
                                            _native.BC[self._bc.value], self.max_move, self.energy_unit,
                                            int(occ_headroom)), "kmc_create")
+        if init_method not in INIT_METHODS:
+            raise ValueError("init_method must be one of {}".format(sorted(INIT_METHODS)))
+        _native.check(self._lib.kmc_set_init_method(self._ctx, INIT_METHODS[init_method]), "kmc_set_init_method")
         self.profile = {}
 
     # ------------------------------------------------------------------ helpers
@@ -290,7 +299,22 @@ class KMCFMM:
         self._profile_inc("accept", time.time() - t0)
 
     def eval_field(self, points):
-        raise NotImplementedError("eval_field is not part of the propose/accept path built here")
+        """Potential of all charges at (N, 3) points (coulomb_kmc/kmc_fmm.py:544-567):
+        near field, leaf local expansion and (pbc, ff-only) long-range term.
+        energy_unit does not affect the output.  pbc potentials are those of the cubic
+        lattice sum with conducting boundaries: the zero-mean Ewald potential minus the
+        constant (2 pi / 3V) sum_j q_j |x_j|^2."""
+        self._assert_init()
+        torch = self._torch()
+        pts = np.ascontiguousarray(np.atleast_2d(points), dtype=np.float64)
+        if pts.shape[1] != 3:
+            raise ValueError("points must have shape (N, 3)")
+        n = pts.shape[0]
+        d_pts = torch.as_tensor(pts).cuda()
+        d_out = torch.zeros(n, dtype=torch.float64, device="cuda")
+        _native.check(self._lib.kmc_eval_field(self._ctx, self._stream(), n, d_pts.data_ptr(), d_out.data_ptr()),
+                      "eval_field")
+        return d_out.cpu().numpy()
 
     def free(self):
         if getattr(self, "_ctx", None) is not None and self._ctx.value:

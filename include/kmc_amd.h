@@ -76,10 +76,21 @@ int kmc_set_lr_operator(kmc_ctx* ctx, int64_t nnz, const double* h_data, const i
 
 /*
  * KMCFMM.initialise (coulomb_kmc/kmc_fmm.py:947-993): builds every leaf local
- * expansion by direct P2L of all charges (and their 26 images for '27'/PBC) on
- * the GPU, the level-0 multipole M, and the energy.  *h_energy receives the energy.
+ * expansion from all charges (and their 26 images for '27'/PBC) on the GPU (see
+ * kmc_set_init_method), the level-0 multipole M, and the energy.  *h_energy receives
+ * the energy.
  */
 int kmc_initialise(kmc_ctx* ctx, void* stream, double* h_energy);
+
+/*
+ * How kmc_initialise builds the leaf expansions (the reference takes them from ppmd's
+ * FMM, coulomb_kmc/kmc_fmm.py:463-476, 956-959): 1 = direct P2L of every charge into
+ * every well-separated leaf (exact; O(n x leaves)); 2 = hierarchical: multipoles, M2M
+ * and M2L on the levels above the leaves' grandparents, L2L down, plus direct P2L of
+ * the charges in the 12^3 leaves around each leaf's grandparent block (M2L truncation
+ * at order L; needs >= 8^3 leaves); 0 = automatic (hierarchical for large systems).
+ */
+int kmc_set_init_method(kmc_ctx* ctx, int32_t method);
 
 /* Leaf expansions in the reference layout [ncells][2*L*L] (re block then im block,
  * index l*l+l+m; coulomb_kmc/kmc_octal.py:61-62). */
@@ -154,6 +165,16 @@ int kmc_step(kmc_ctx* ctx, void* stream, int64_t P, int64_t T, const int64_t* d_
 int kmc_materialize_moves(kmc_ctx* ctx, void* stream, int64_t nsteps, int64_t P, int64_t k,
                           const int64_t* d_pids_all, const double* d_disp_all, const int64_t* d_counter,
                           int64_t* d_pid, int64_t* d_es, double* d_new_pos);
+
+/*
+ * KMCFMM.eval_field (coulomb_kmc/kmc_fmm.py:544-567): potential of all charges at n
+ * points d_points[n][3] (device) into d_out[n]: near field over the 27 leaves around
+ * the point's leaf (coulomb_kmc/kmc_local.py:1085-1182), that leaf's local expansion
+ * (coulomb_kmc/kmc_octal.py:1417-1438) and, for pbc / ff-only, the long-range term
+ * (coulomb_kmc/kmc_full_long_range.py:341-387).  energy_unit is not applied.
+ * Synchronises the stream before returning.
+ */
+int kmc_eval_field(kmc_ctx* ctx, void* stream, int64_t n, const double* d_points, double* d_out);
 
 /* Flags raised by device-side steps (store overflow, bad moves); cleared by the call. */
 int kmc_check_flags(kmc_ctx* ctx, void* stream, int32_t* h_flags);

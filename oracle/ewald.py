@@ -84,3 +84,38 @@ def ewald_energy(pos, q, extent, tol=1e-15):
     e_rec *= 2.0 * math.pi / V
     e_self = -alpha / math.sqrt(math.pi) * float(np.sum(q * q))
     return e_real + e_rec + e_self
+
+
+def ewald_potential(points, pos, q, extent, tol=1e-15):
+    """Tinfoil Ewald potential phi(x) = sum_{j, n} q_j / |x - x_j + n E| of a neutral cubic
+    periodic system at points that are not charge positions (same split as ewald_energy;
+    the uniform k = 0 term vanishes for a neutral system)."""
+    points = np.atleast_2d(np.asarray(points, dtype=np.float64))
+    pos = np.asarray(pos, dtype=np.float64)
+    q = np.asarray(q, dtype=np.float64).ravel()
+    E = float(extent)
+    V = E ** 3
+    alpha = 5.0 / E
+    rc = 1.0
+    while erfc(alpha * rc) / rc > tol:
+        rc *= 1.1
+    nmax = int(math.ceil(rc / E)) + 1
+    phi = np.zeros(len(points))
+    diffs = points[:, None, :] - pos[None, :, :]
+    for n in itertools.product(range(-nmax, nmax + 1), repeat=3):
+        d = diffs + E * np.array(n, dtype=np.float64)
+        r = np.sqrt((d * d).sum(axis=-1))
+        mask = r < rc
+        phi += np.sum(np.where(mask, q[None, :] * erfc(alpha * r) / np.where(mask, r, 1.0), 0.0), axis=1)
+    kc = 2.0 * alpha * math.sqrt(-math.log(tol))
+    kmax = int(math.ceil(kc * E / (2.0 * math.pi)))
+    for kv in itertools.product(range(-kmax, kmax + 1), repeat=3):
+        if kv == (0, 0, 0):
+            continue
+        k = (2.0 * math.pi / E) * np.array(kv, dtype=np.float64)
+        k2 = float(k @ k)
+        if k2 > kc * kc:
+            continue
+        s = np.sum(q * np.exp(-1j * (pos @ k)))
+        phi += (4.0 * math.pi / V) * math.exp(-k2 / (4.0 * alpha * alpha)) / k2 * np.real(np.exp(1j * (points @ k)) * s)
+    return phi
