@@ -401,6 +401,42 @@ __device__ __forceinline__ void sweep_store_dispatch(double2* le, int group, dou
     }
 }
 
+// scale a group's sums in place by Gam (rescaled recurrence), then copy them to a leaf row
+template <int L, int M>
+__device__ __forceinline__ void sweep_scale_column(double* ar, double* ai) {
+#pragma unroll
+    for (int k = 0; k < L - M; k++) {
+        ar[k] *= kRec2.gam[M + k][M];
+        ai[k] *= kRec2.gam[M + k][M];
+    }
+}
+template <int L, int M>
+__device__ __forceinline__ void sweep_stage_column(double2* row, const double* ar, const double* ai) {
+#pragma unroll
+    for (int k = 0; k < L - M; k++) row[hidx(M + k, M)] = make_double2(ar[k], ai[k]);
+}
+template <int L, int P0, int NP, bool SCALE, int J = 0>
+__device__ __forceinline__ void sweep_group_stage(double2* row, double (*ar)[L + 1], double (*ai)[L + 1]) {
+    if constexpr (J < NP && P0 + J < (L + 1) / 2) {
+        constexpr int M1 = P0 + J, M2 = L - 1 - M1;
+        if constexpr (SCALE) sweep_scale_column<L, M1>(ar[J], ai[J]);
+        else sweep_stage_column<L, M1>(row, ar[J], ai[J]);
+        if constexpr (M2 != M1) {
+            if constexpr (SCALE) sweep_scale_column<L, M2>(ar[J] + (L - M1), ai[J] + (L - M1));
+            else sweep_stage_column<L, M2>(row, ar[J] + (L - M1), ai[J] + (L - M1));
+        }
+        sweep_group_stage<L, P0, NP, SCALE, J + 1>(row, ar, ai);
+    }
+}
+template <int L, int NP, bool SCALE, int G0>
+__device__ __forceinline__ void sweep_stage_dispatch(double2* row, int group, double (*ar)[L + 1], double (*ai)[L + 1]) {
+    constexpr int NG = ((L + 1) / 2 + NP - 1) / NP;
+    if constexpr (G0 < NG) {
+        if (group == G0) sweep_group_stage<L, G0 * NP, NP, SCALE>(row, ar, ai);
+        else sweep_stage_dispatch<L, NP, SCALE, G0 + 1>(row, group, ar, ai);
+    }
+}
+
 // Lane = leaf (threadIdx.x), column-pair group = blockIdx.y.  With SPLIT = 3 the
 // workgroup is 64 leaves x 3 waves and wave threadIdx.y takes the image layer
 // oz = threadIdx.y - 1; the two upper waves hand their sums to wave 0 through LDS
@@ -447,6 +483,49 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgs a) {
     }
     if (!live) return;
     sweep_store_dispatch<L, NP, 0>(a.lexp + (size_t)cell * nhalf(L), group, ar, ai);
+}
+
+// Fused layout: a workgroup owns 64 consecutive leaves and all their column groups
+// (wave threadIdx.y = group), so the update of the leaves' expansions is one
+// contiguous block of 64 * nh coefficients.  The sums are staged through LDS in two
+// halves of 32 leaves and added to the expansions with coalesced loads and stores
+// (a lane's own columns are scattered over its leaf: written directly they touch a
+// cache line per lane and the groups of a leaf would re-fetch the same lines).
+constexpr int stage_stride(int nh) { return nh + ((9 - nh % 8) % 8); }   // = 1 (mod 8): conflict-free b128 rows
+template <int L, int NP, bool PER>
+__global__ void __launch_bounds__(512, (NP == 1 ? 4 : 2)) k_sweep_fused(SweepArgs a) {
+    constexpr int W = L + 1;
+    constexpr int NH = nhalf(L);
+    constexpr int ST = stage_stride(NH);
+    __shared__ double2 stage[32 * ST];
+    const int lane = threadIdx.x;
+    const int group = __builtin_amdgcn_readfirstlane((int)threadIdx.y);   // one group per wave
+    const int cell0 = blockIdx.x * 64;
+    const int cell = cell0 + lane;
+    double ar[NP][W], ai[NP][W];
+#pragma unroll
+    for (int j = 0; j < NP; j++)
+#pragma unroll
+        for (int k = 0; k < W; k++) { ar[j][k] = 0.0; ai[j][k] = 0.0; }
+    sweep_dispatch<L, NP, PER, 0>(a, cell, group, -1, 1, ar, ai);
+    sweep_stage_dispatch<L, NP, true, 0>(nullptr, group, ar, ai);
+    const int tid = threadIdx.x + 64 * threadIdx.y;
+    const int nthr = 64 * blockDim.y;
+#pragma unroll 1
+    for (int half = 0; half < 2; half++) {
+        __syncthreads();
+        if ((lane >> 5) == half) sweep_stage_dispatch<L, NP, false, 0>(stage + (lane & 31) * ST, group, ar, ai);
+        __syncthreads();
+        double2* base = a.lexp + (size_t)(cell0 + 32 * half) * NH;
+        for (int e = tid; e < 32 * NH; e += nthr) {
+            const int j = e / NH, h = e - j * NH;
+            const double2 sv = stage[j * ST + h];
+            double2 v = base[e];
+            v.x += sv.x;
+            v.y += sv.y;
+            base[e] = v;
+        }
+    }
 }
 
 // ============================================================================
@@ -608,14 +687,26 @@ __global__ void __launch_bounds__(256) k_near_cells(EvalArgs a) {
             const int occ = min(a.occ[nc], a.occmax);
             const double px = p.x - shx, py = p.y - shy, pz = p.z - shz;
             const size_t base = (size_t)nc * a.occmax;
-#pragma unroll 2
-            for (int s = 0; s < occ; s++) {
-                const double4 q = a.slot_pos[base + s];
-                const double dx = px - q.x, dy = py - q.y, dz = pz - q.z;
-                const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-                const double contrib = q.w * rsqrt_sel<MIXED>(r2);
-                if (ex >= 0) acc += (a.slot_id[base + s] != ex) ? contrib : 0.0;
-                else acc += contrib;
+            // slots in groups of 4 with all loads issued up front (indices clamped to
+            // the cell's slot range; slots past the occupancy contribute 0)
+            for (int s0 = 0; s0 < occ; s0 += 4) {
+                double4 q[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) q[j] = a.slot_pos[base + min(s0 + j, a.occmax - 1)];
+                int id[4] = {-1, -1, -1, -1};
+                if (ex >= 0) {
+#pragma unroll
+                    for (int j = 0; j < 4; j++) id[j] = a.slot_id[base + min(s0 + j, a.occmax - 1)];
+                }
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const double dx = px - q[j].x, dy = py - q[j].y, dz = pz - q[j].z;
+                    double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                    const bool use = (s0 + j < occ) && (ex < 0 || id[j] != ex);
+                    r2 = use ? r2 : 1.0;
+                    const double contrib = q[j].w * rsqrt_sel<MIXED>(r2);
+                    acc += use ? contrib : 0.0;
+                }
             }
         }
     }
@@ -1790,6 +1881,14 @@ static void launch_p2l_near(SweepArgs a, const int* occ, const double4* slot_pos
 template <int L, int NP>
 static void launch_sweep_np(SweepArgs a, int ncells, hipStream_t s, int block, int split) {
     const int ng = ((L + 1) / 2 + NP - 1) / NP;
+    if (block == 0 && ncells % 64 == 0 && 64 * ng <= 512) {
+        if (a.g.bc == KMC_BC_FREE_SPACE)
+            hipLaunchKernelGGL((k_sweep_fused<L, NP, false>), dim3(ncells / 64), dim3(64, ng), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_sweep_fused<L, NP, true>), dim3(ncells / 64), dim3(64, ng), 0, s, a);
+        return;
+    }
+    if (block == 0) block = 64;
     if (a.g.bc == KMC_BC_FREE_SPACE) {
         hipLaunchKernelGGL((k_sweep<L, NP, false, 1>), dim3((ncells + block - 1) / block, ng), dim3(block), 0, s, a);
     } else if (split == 3) {
@@ -2261,8 +2360,8 @@ int kmc_set_tiles(kmc_ctx* c, int32_t near_lanes, int32_t sweep_block, int32_t s
         near_lanes != 32 && near_lanes != 64)
         return set_err(KMC_E_ARG, "near_lanes must be one of 1,4,8,16,27,32,64 (27: one lane per neighbour leaf)");
     if (sweep_pairs < 1 || sweep_pairs > 2) return set_err(KMC_E_ARG, "sweep_pairs must be 1 or 2");
-    if (sweep_block < 64 || sweep_block > 256 || sweep_block % 64)
-        return set_err(KMC_E_ARG, "sweep_block must be 64, 128, 192 or 256");
+    if (sweep_block != 0 && (sweep_block < 64 || sweep_block > 256 || sweep_block % 64))
+        return set_err(KMC_E_ARG, "sweep_block must be 0 (fused), 64, 128, 192 or 256");
     if (sweep_split != 1 && sweep_split != 3) return set_err(KMC_E_ARG, "sweep_split must be 1 or 3");
     c->near_lanes = near_lanes;
     c->sweep_block = sweep_block;
