@@ -586,11 +586,15 @@ __device__ __forceinline__ double far_dispatch(int pair, double dx, double dy, d
     }
 }
 
-// one lane per evaluation point, grid.y = column pair (m, L-1-m); out[pair * n + i]
+// one lane per evaluation point; the column pairs (m, L-1-m) of 64 points are the
+// waves of one workgroup (pair = blockIdx.y * blockDim.y + threadIdx.y), so the
+// waves reading one leaf expansion run together and share its cache lines;
+// out[pair * n + i]
 template <int L>
-__global__ void __launch_bounds__(256) k_far(EvalArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
+__global__ void __launch_bounds__(512) k_far(EvalArgs a) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int pair = __builtin_amdgcn_readfirstlane((int)(blockIdx.y * blockDim.y + threadIdx.y));
+    if (i >= a.n || pair >= (L + 1) / 2) return;
     const double4 p = a.pos4[i];
     const int c = a.cell[i];
     const int n = a.g.ncps;
@@ -600,7 +604,6 @@ __global__ void __launch_bounds__(256) k_far(EvalArgs a) {
     const double dz = p.z - (a.g.c0 + tz * a.g.w);
     const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
     const double2* le = a.lexp + (size_t)c * nhalf(L);
-    const int pair = blockIdx.y;
     a.out[(size_t)pair * a.n + i] = p.w * far_dispatch<L, 0>(pair, dx, dy, dz, r2, le);
 }
 
@@ -1906,7 +1909,9 @@ static void launch_sweep(SweepArgs a, int ncells, hipStream_t s, int block, int 
 template <int L>
 static void launch_far(EvalArgs a, hipStream_t s) {
     if (a.n <= 0) return;
-    hipLaunchKernelGGL(k_far<L>, dim3((a.n + 255) / 256, (L + 1) / 2), dim3(256), 0, s, a);
+    constexpr int npair = (L + 1) / 2;
+    constexpr int py = npair <= 8 ? npair : (npair + 1) / 2;   // pairs per workgroup, <= 8 waves
+    hipLaunchKernelGGL(k_far<L>, dim3((a.n + 63) / 64, (npair + py - 1) / py), dim3(64, py), 0, s, a);
 }
 template <int L>
 static void launch_lr(LR2Args a, hipStream_t s) {
