@@ -185,12 +185,118 @@ def traffic_from_pmc(workload, fetch_dir, write_dir):
     print(json.dumps({key: tab[key]}, indent=1))
 
 
+BATCH = dict(workload="batch: 256 systems, N log-uniform [2000, 200000] at density 0.1, half free-space half pbc, "
+                      "8-16 charges per leaf, L in {8, 12, 16}, 256x8 proposals + 1 accept per system per step",
+             systems=256, nmin=2000, nmax=200000, density=0.1, P=256, k=8, radius=1.0, seed=1003, steps=20, warmup=4)
+
+
+def batch_systems(spec):
+    """The systems of the batched workload (seeded)."""
+    rng = np.random.default_rng(spec["seed"])
+    out = []
+    for i in range(spec["systems"]):
+        N = int(round(np.exp(rng.uniform(np.log(spec["nmin"]), np.log(spec["nmax"])))))
+        N -= N % 2
+        E = (N / spec["density"]) ** (1.0 / 3.0)
+        ncps = 2 ** int(round(np.log2((N / 12.0) ** (1.0 / 3.0))))
+        ncps = min(32, max(8, ncps))
+        r = int(np.log2(ncps)) + 1
+        L = int(rng.choice((8, 12, 16)))
+        bc = "free_space" if i % 2 == 0 else "pbc"
+        pos = rng.uniform(-0.5 * E, 0.5 * E, (N, 3))
+        q = np.ones(N)
+        q[: N // 2] = -1.0
+        rng.shuffle(q)
+        out.append(dict(N=N, E=E, r=r, L=L, bc=bc, pos=pos, q=q, P=spec["P"], k=spec["k"], radius=spec["radius"]))
+    return out, rng
+
+
+def run_batch(args):
+    import torch
+    from coulomb_kmc_amd import KMCFMM
+    from coulomb_kmc_amd.batch import BatchSystem, KMCBatch
+    spec = BATCH
+    steps = args.steps if args.steps > 0 else spec["steps"]
+    warm = args.warmup if args.warmup >= 0 else spec["warmup"]
+    G = max(1, min(args.graph_steps, 2))
+    steps = max(G, (steps // G) * G)
+    warm = max(G, (warm // G) * G)
+    t0 = time.time()
+    specs, rng = batch_systems(spec)
+    log("batch", len(specs), "systems, charges", sum(x["N"] for x in specs), "generated", round(time.time() - t0, 2), "s")
+
+    def make(x):
+        k = KMCFMM(x["pos"].copy(), x["q"].reshape(-1, 1), x["E"], r=x["r"], l=x["L"], boundary_condition=x["bc"],
+                   max_move=x["radius"], precision="mixed", init_method="fmm")
+        k.initialise()
+        return k
+
+    nst = 16
+    systems = []
+    t0 = time.time()
+    for x in specs:
+        kmc = make(x)
+        pids, disp, us = move_stream(rng, dict(N=x["N"], P=x["P"], k=x["k"], radius=x["radius"]), nst)
+        dev = lambda a, t: torch.as_tensor(np.ascontiguousarray(a), dtype=t).cuda()
+        bs = BatchSystem(kmc, x["P"], x["k"], dev(pids, torch.int64), dev(disp.reshape(-1), torch.float64),
+                         dev(us, torch.float64))
+        bs.spec = x
+        systems.append(bs)
+    torch.cuda.synchronize()
+    t_init = time.time() - t0
+    log("initialised", len(systems), "systems in", round(t_init, 1), "s")
+    batch = KMCBatch(systems, kT=1.0, precision="mixed", streams=args.batch_streams)
+    t0 = time.time()
+    tiles = batch.autotune(lambda s: make(s.spec), reps=3, log=log)
+    log("autotune", round(time.time() - t0, 1), "s")
+    batch.capture(G)
+    torch.cuda.synchronize()
+    for _ in range(warm // G):
+        batch.replay()
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(steps // G):
+        batch.replay()
+    ev1.record()
+    ev1.synchronize()
+    ms = ev0.elapsed_time(ev1)
+    flags = [f for f in batch.check_flags() if f]
+    T = batch.proposals_per_step
+    out = {
+        "metric": METRIC,
+        "value": T * steps / (ms * 1e-3),
+        "unit": "proposals/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warm,
+        "ms_per_step": ms / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded uniform charges +-1, ball-displacement proposals, seeded Metropolis uniforms)",
+        "config": {"workload": spec["workload"], "systems": len(systems), "charges": int(sum(x["N"] for x in specs)),
+                   "proposals_per_step": T, "graph_steps": G, "streams": args.batch_streams,
+                   "tiles": {"L{}_C{}_T{}".format(*k): v for k, v in tiles.items()}},
+        "accepts_per_s": len(systems) * steps / (ms * 1e-3),
+        "initialise_s": t_init,
+        "flags": flags,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    for s in systems:
+        s.kmc.free()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0, help="0: the workload's default")
     ap.add_argument("--warmup", type=int, default=-1, help="-1: the workload's default")
-    ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS) + ["batch-256"])
+    ap.add_argument("--batch-streams", type=int, default=1, help="streams of the batched workload (1)")
     ap.add_argument("--graph-steps", type=int, default=10, help="steps captured per HIP graph")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -204,6 +310,13 @@ def main():
 
     if args.traffic_from:
         traffic_from_pmc(args.workload, *args.traffic_from)
+        return
+    if args.workload == "batch-256":
+        if args.gpus != 1:
+            print("bench.py: this build measures one GPU only; --gpus {} is not supported".format(args.gpus),
+                  file=sys.stderr)
+            sys.exit(2)
+        run_batch(args)
         return
 
     if args.gpus != 1:

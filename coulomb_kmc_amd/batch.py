@@ -1,0 +1,130 @@
+"""
+Many independent systems stepped together on one GPU.
+
+Each system is its own KMCFMM context (its own leaf expansions, cell tables and work
+buffers).  One step of the batch is one KMC step of every system: the synthetic moves
+are materialised on the device, the proposals evaluated, one proposal selected by
+Metropolis and accepted (kmc_step).  The steps of all systems are recorded into one
+HIP graph, so a whole batch step (several, with ``steps_per_graph``) is a single graph
+launch from the host.
+
+Only one stream per batch is supported: the systems' steps are recorded one after the
+other on the capturing stream (each system still overlaps its own long-range and
+accept-tail kernels with its near/far fields and sweep on its context's side
+stream).  Recording systems on extra torch side streams (forked from and joined to
+the capturing stream) ended in a host crash inside graph instantiation on this
+ROCm/PyTorch stack, so that mode raises.
+
+Tiles are chosen per (order, leaf count, proposal count) class by the autotuner
+(coulomb_kmc_amd/tune.py), timed on the first system of each class.
+"""
+
+import numpy as np
+
+from . import tune
+
+
+class BatchSystem:
+    """One system of a batch: the context and its device move stream.
+
+    pids_all [nsteps * P] int64, disp_all [nsteps * P * k * 3] float64, uniforms [nsteps]
+    float64 are device tensors; the step counter selects row (counter mod nsteps)."""
+
+    def __init__(self, kmc, P, k, pids_all, disp_all, uniforms):
+        import torch
+        self.kmc = kmc
+        self.P, self.k = int(P), int(k)
+        self.T = self.P * self.k
+        self.pids_all, self.disp_all, self.uniforms = pids_all, disp_all, uniforms
+        dev = pids_all.device
+        self.counter = torch.zeros(1, dtype=torch.int64, device=dev)
+        self.selected = torch.full((uniforms.numel(),), -1, dtype=torch.int64, device=dev)
+        self.d_pid = torch.zeros(self.P, dtype=torch.int64, device=dev)
+        self.d_es = torch.zeros(self.P + 1, dtype=torch.int64, device=dev)
+        self.d_new = torch.zeros(self.T * 3, dtype=torch.float64, device=dev)
+        self.d_dE = torch.zeros(self.T, dtype=torch.float64, device=dev)
+
+    def materialize(self, stream=None):
+        self.kmc.materialize_moves(self.P, self.k, self.pids_all, self.disp_all, self.counter, self.d_pid, self.d_es,
+                                   self.d_new, stream=stream)
+
+    def step(self, kT, precision, stream=None):
+        self.materialize(stream)
+        self.kmc.step_device(self.d_pid, self.d_es, self.d_new, self.d_dE, self.uniforms, kT, self.counter,
+                             self.selected, precision=precision, stream=stream)
+
+
+class KMCBatch:
+    def __init__(self, systems, kT=1.0, precision="mixed", streams=1):
+        if int(streams) != 1:
+            raise NotImplementedError("batches are recorded on one stream (extra side streams crashed graph "
+                                      "instantiation on this ROCm/PyTorch stack)")
+        self.systems = list(systems)
+        self.kT = float(kT)
+        self.precision = precision
+        self.graph = None
+        self.steps_per_graph = 0
+
+    @property
+    def proposals_per_step(self):
+        return sum(s.T for s in self.systems)
+
+    def autotune(self, make_copy, reps=5, log=None):
+        """Tiles per (L, leaves, proposals) class.  make_copy(system) returns a fresh,
+        initialised KMCFMM of the same charges: timing the sweep re-applies a move, so
+        the timings run on that copy and the batch's own systems keep their state."""
+        import torch
+        classes = {}
+        for s in self.systems:
+            classes.setdefault((s.kmc.L, s.kmc.ncells, s.T), []).append(s)
+        choices = {}
+        for key, members in classes.items():
+            s0 = members[0]
+            tmp = BatchSystem(make_copy(s0), s0.P, s0.k, s0.pids_all, s0.disp_all, s0.uniforms)
+            tmp.materialize()
+            tmp.kmc.propose_device(tmp.d_pid, tmp.d_es, tmp.d_new, tmp.d_dE, precision=self.precision)
+            tmp.step(self.kT, self.precision)
+            tmp.materialize()
+            tmp.kmc.propose_device(tmp.d_pid, tmp.d_es, tmp.d_new, tmp.d_dE, precision=self.precision)
+            torch.cuda.synchronize()
+            choice, _ = tune.autotune(tmp.kmc, tmp.T, reps=reps, precision=self.precision)
+            tmp.kmc.free()
+            for s in members:
+                s.kmc.set_tiles(**choice)
+            choices[key] = choice
+            if log:
+                log("batch tiles", key, choice)
+        return choices
+
+    def prepare(self):
+        """Sizes every context's work buffers (one eager propose each), outside capture."""
+        import torch
+        for s in self.systems:
+            s.materialize()
+            s.kmc.propose_device(s.d_pid, s.d_es, s.d_new, s.d_dE, precision=self.precision)
+        torch.cuda.synchronize()
+
+    def capture(self, steps_per_graph=1):
+        """Records steps_per_graph batch steps into one graph."""
+        import torch
+        self.prepare()
+        main = torch.cuda.Stream()
+        g = torch.cuda.CUDAGraph()
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g, stream=main):
+            for _ in range(steps_per_graph):
+                for s in self.systems:
+                    s.step(self.kT, self.precision, stream=main.cuda_stream)
+        self.graph = g
+        self.steps_per_graph = steps_per_graph
+        return g
+
+    def replay(self, n=1):
+        for _ in range(n):
+            self.graph.replay()
+
+    def check_flags(self):
+        return [s.kmc.check_flags() for s in self.systems]
+
+    def energies(self):
+        return np.array([s.kmc.device_energy() for s in self.systems])

@@ -261,34 +261,71 @@ __device__ __forceinline__ void sweep_store_column(double2* le, const double* ar
 }
 
 // columns (P, L-1-P) of pair P, NI images, into accumulator row ar/ai (first L-P entries: column P)
-template <int L, int P, int NI>
-__device__ __forceinline__ void sweep_pair_cols(const double* zr, const double* ir2, const Cplx* u,
-                                                const double* base, double* ar, double* ai) {
-    constexpr int M1 = P, M2 = L - 1 - P;
-    Cplx U1[NI], W[NI];
+// column M with weight base * s_M * U (U = conj((x+iy)/r^2)^M) into the accumulator row
+template <int L, int M, int NI>
+__device__ __forceinline__ void sweep_weighted_column(const double* zr, const double* ir2, const Cplx* U,
+                                                      const double* base, double* ar, double* ai) {
+    Cplx W[NI];
 #pragma unroll
     for (int j = 0; j < NI; j++) {
-        U1[j] = cpow<M1>(u[j]);
-        const double f = base[j] * kRec.s[M1];
-        W[j] = Cplx{f * U1[j].re, f * U1[j].im};
+        const double f = base[j] * kRec.s[M];
+        W[j] = Cplx{f * U[j].re, f * U[j].im};
     }
-    sweep_column<L, M1, NI>(zr, ir2, W, ar, ai);
-    if constexpr (M2 != M1) {
+    sweep_column<L, M, NI>(zr, ir2, W, ar, ai);
+}
+// low columns P0 + J, J = 0 .. NPE-1 (ascending powers: one multiply per column)
+template <int L, int P0, int NPE, int NI, int J = 0>
+__device__ __forceinline__ void sweep_low_cols(const double* zr, const double* ir2, const Cplx* u, Cplx* U,
+                                               const double* base, double (*ar)[L + 1], double (*ai)[L + 1]) {
+    if constexpr (J < NPE) {
+        if constexpr (J > 0) {
 #pragma unroll
-        for (int j = 0; j < NI; j++) {
-            const Cplx U2 = cmul(U1[j], cpow<M2 - M1>(u[j]));
-            const double f = base[j] * kRec.s[M2];
-            W[j] = Cplx{f * U2.re, f * U2.im};
+            for (int j = 0; j < NI; j++) U[j] = cmul(U[j], u[j]);
         }
-        sweep_column<L, M2, NI>(zr, ir2, W, ar + (L - M1), ai + (L - M1));
+        sweep_weighted_column<L, P0 + J, NI>(zr, ir2, U, base, ar[J], ai[J]);
+        sweep_low_cols<L, P0, NPE, NI, J + 1>(zr, ir2, u, U, base, ar, ai);
     }
 }
-template <int L, int P0, int NP, int NI, int J = 0>
+// high columns L-1-P0-J for J = NPE-1 down to 0 (ascending powers), skipping the
+// middle column that equals its low partner when L is odd
+template <int L, int P0, int NPE, int NI, int J>
+__device__ __forceinline__ void sweep_high_cols(const double* zr, const double* ir2, const Cplx* u, Cplx* U,
+                                                const double* base, double (*ar)[L + 1], double (*ai)[L + 1]) {
+    if constexpr (J >= 0) {
+        constexpr int M1 = P0 + J, M2 = L - 1 - P0 - J;
+        if constexpr (M2 != M1)
+            sweep_weighted_column<L, M2, NI>(zr, ir2, U, base, ar[J] + (L - M1), ai[J] + (L - M1));
+        if constexpr (J > 0) {
+#pragma unroll
+            for (int j = 0; j < NI; j++) U[j] = cmul(U[j], u[j]);
+        }
+        sweep_high_cols<L, P0, NPE, NI, J - 1>(zr, ir2, u, U, base, ar, ai);
+    }
+}
+// the column pairs (P, L-1-P), P = P0 .. P0+NP-1, of one lane: the powers of u run up
+// through the low columns and on (one cpow) through the high ones
+template <int L, int P0, int NP, int NI>
 __device__ __forceinline__ void sweep_group_cols(const double* zr, const double* ir2, const Cplx* u,
                                                  const double* base, double (*ar)[L + 1], double (*ai)[L + 1]) {
-    if constexpr (J < NP && P0 + J < (L + 1) / 2) {
-        sweep_pair_cols<L, P0 + J, NI>(zr, ir2, u, base, ar[J], ai[J]);
-        sweep_group_cols<L, P0, NP, NI, J + 1>(zr, ir2, u, base, ar, ai);
+    constexpr int NPAIR = (L + 1) / 2;
+    constexpr int NPE = (P0 + NP <= NPAIR) ? NP : NPAIR - P0;   // pairs in this group
+    if constexpr (NPE > 0) {
+        Cplx U[NI];
+#pragma unroll
+        for (int j = 0; j < NI; j++) U[j] = cpow<P0>(u[j]);
+        sweep_low_cols<L, P0, NPE, NI>(zr, ir2, u, U, base, ar, ai);
+        constexpr int LOW_TOP = P0 + NPE - 1;          // power held in U now
+        constexpr int HIGH_BOTTOM = L - 1 - LOW_TOP;    // lowest high column
+        if constexpr (HIGH_BOTTOM > LOW_TOP) {
+#pragma unroll
+            for (int j = 0; j < NI; j++) U[j] = cmul(U[j], cpow<HIGH_BOTTOM - LOW_TOP>(u[j]));
+            sweep_high_cols<L, P0, NPE, NI, NPE - 1>(zr, ir2, u, U, base, ar, ai);
+        } else if constexpr (HIGH_BOTTOM == LOW_TOP && NPE > 1) {
+            // L odd: the last low column is the middle one; the high columns start one above
+#pragma unroll
+            for (int j = 0; j < NI; j++) U[j] = cmul(U[j], u[j]);
+            sweep_high_cols<L, P0, NPE, NI, NPE - 2>(zr, ir2, u, U, base, ar, ai);
+        }
     }
 }
 template <int L, int P0, int NP, int J = 0>
@@ -493,7 +530,7 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgs a) {
 // cache line per lane and the groups of a leaf would re-fetch the same lines).
 constexpr int stage_stride(int nh) { return nh + ((9 - nh % 8) % 8); }   // = 1 (mod 8): conflict-free b128 rows
 template <int L, int NP, bool PER>
-__global__ void __launch_bounds__(512, (NP == 1 ? 4 : 2)) k_sweep_fused(SweepArgs a) {
+__global__ void __launch_bounds__(512, ((NP == 1 && L <= 16) ? 4 : 2)) k_sweep_fused(SweepArgs a) {
     constexpr int W = L + 1;
     constexpr int NH = nhalf(L);
     constexpr int ST = stage_stride(NH);
@@ -992,48 +1029,11 @@ __device__ __forceinline__ void lr_block_values(double x, double y, double z, do
     }
 }
 
-// per proposal: a_o . v and 0.5 v G v over one component of block B
-template <int L, int B, int COMP>
-__device__ __forceinline__ void lr_block_prop(double x, double y, double z, double q, const double* __restrict__ ao,
-                                              const double* __restrict__ gblk, double& acc) {
-    constexpr LRBlocks T = LRTab<L>::T;
-    constexpr int NB = T.n[B];
-    if constexpr (NB > 0) {
-        double v[NB];
-        lr_block_values<L, B, COMP>(x, y, z, q, v);
-        const double* G = gblk + T.off[B] + COMP * NB * NB;
-#pragma unroll
-        for (int i = 0; i < NB; i++) {
-            double s = fma(0.5 * G[i * NB + i], v[i], ao[2 * hidx(T.l[B][i], T.m[B][i]) + COMP]);
-#pragma unroll
-            for (int j = 0; j < i; j++) s = fma(G[i * NB + j], v[j], s);
-            acc = fma(v[i], s, acc);
-        }
-    }
-    __builtin_amdgcn_sched_barrier(0);
-}
-
-// per moving charge: a_o = g - G v_o over one component of block B, and its share of c_o
-template <int L, int B, int COMP>
-__device__ __forceinline__ void lr_block_part(double x, double y, double z, double q, const double* __restrict__ g,
-                                              const double* __restrict__ gblk, double* __restrict__ ao, double& c) {
-    constexpr LRBlocks T = LRTab<L>::T;
-    constexpr int NB = T.n[B];
-    if constexpr (NB > 0) {
-        double v[NB];
-        lr_block_values<L, B, COMP>(x, y, z, q, v);
-        const double* G = gblk + T.off[B] + COMP * NB * NB;
-#pragma unroll
-        for (int i = 0; i < NB; i++) {
-            const int h = 2 * hidx(T.l[B][i], T.m[B][i]) + COMP;
-            double yy = 0.0;
-#pragma unroll
-            for (int j = 0; j < NB; j++) yy = fma(G[i * NB + j], v[j], yy);
-            ao[h] = g[h] - yy;
-            c += v[i] * fma(0.5, yy, -g[h]);
-        }
-    }
-    __builtin_amdgcn_sched_barrier(0);
+__host__ __device__ constexpr int lr_nbmax(int L) {
+    const LRBlocks t = make_lr_blocks(L);
+    int m = 0;
+    for (int b = 0; b < 6; b++) m = t.n[b] > m ? t.n[b] : m;
+    return m + (m & 1);   // even: rows of a_o stay 16-byte aligned
 }
 
 struct LR2Args {
@@ -1042,50 +1042,116 @@ struct LR2Args {
     const double* gvec;
     const double4* epos4;
     const int32_t* owner;
-    double* aold;              // [P][2 nh]
+    double* aold;              // [12][P][NBMAX]  a_o per (block, component), row per moving charge
     double* cpart;             // [12][P]   c_o per (block, component)
     double* lpart;             // [12][T]   v.a_o + 0.5 v G v per (block, component)
 };
 
-template <int L, int Y>
-__device__ __forceinline__ void lr_part_dispatch(int y, double x, double yy, double z, double q, const double* g,
-                                                 const double* gblk, double* ao, double& c) {
-    if constexpr (Y < 12) {
-        if (y == Y) lr_block_part<L, Y / 2, Y % 2>(x, yy, z, q, g, gblk, ao, c);
-        else lr_part_dispatch<L, Y + 1>(y, x, yy, z, q, g, gblk, ao, c);
-    }
+// the G tile of (block B, component COMP) into LDS
+template <int L, int B, int COMP>
+__device__ __forceinline__ void lr_stage_g(const double* __restrict__ gblk, double* Gs) {
+    constexpr LRBlocks T = LRTab<L>::T;
+    constexpr int NB = T.n[B];
+    const double* G = gblk + T.off[B] + COMP * NB * NB;
+    for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) Gs[e] = G[e];
+    __syncthreads();
 }
-template <int L, int Y>
-__device__ __forceinline__ void lr_prop_dispatch(int y, double x, double yy, double z, double q, const double* ao,
-                                                 const double* gblk, double& acc) {
-    if constexpr (Y < 12) {
-        if (y == Y) lr_block_prop<L, Y / 2, Y % 2>(x, yy, z, q, ao, gblk, acc);
-        else lr_prop_dispatch<L, Y + 1>(y, x, yy, z, q, ao, gblk, acc);
+
+// per moving charge p: a_o = g - G v_o over one component of block B, its share of c_o
+template <int L, int B, int COMP>
+__device__ __forceinline__ void lr_block_part(const LR2Args& a, const double* __restrict__ gblk,
+                                              const double* __restrict__ g, double* Gs) {
+    constexpr LRBlocks T = LRTab<L>::T;
+    constexpr int NB = T.n[B];
+    constexpr int NBMAX = lr_nbmax(L);
+    constexpr int Y = 2 * B + COMP;
+    if constexpr (NB > 0) {
+        lr_stage_g<L, B, COMP>(gblk, Gs);
+        const int p = blockIdx.x * blockDim.x + threadIdx.x;
+        if (p >= a.P) return;
+        const double4 xo = a.epos4[a.T + p];
+        double v[NB];
+        lr_block_values<L, B, COMP>(xo.x, xo.y, xo.z, xo.w, v);
+        double* ao = a.aold + ((size_t)Y * a.P + p) * NBMAX;
+        double c = 0.0;
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            const int h = 2 * hidx(T.l[B][i], T.m[B][i]) + COMP;
+            double yy = 0.0;
+#pragma unroll
+            for (int j = 0; j < NB; j++) yy = fma(Gs[i * NB + j], v[j], yy);
+            ao[i] = g[h] - yy;
+            c += v[i] * fma(0.5, yy, -g[h]);
+        }
+        a.cpart[(size_t)Y * a.P + p] = c;
+    } else {
+        const int p = blockIdx.x * blockDim.x + threadIdx.x;
+        if (p < a.P) a.cpart[(size_t)Y * a.P + p] = 0.0;
     }
 }
 
+// per proposal t: v_n . a_o + 0.5 v_n G v_n over one component of block B
+template <int L, int B, int COMP>
+__device__ __forceinline__ void lr_block_prop(const LR2Args& a, const double* __restrict__ gblk, double* Gs) {
+    constexpr LRBlocks T = LRTab<L>::T;
+    constexpr int NB = T.n[B];
+    constexpr int NBMAX = lr_nbmax(L);
+    constexpr int Y = 2 * B + COMP;
+    if constexpr (NB > 0) {
+        lr_stage_g<L, B, COMP>(gblk, Gs);
+        const int t = blockIdx.x * blockDim.x + threadIdx.x;
+        if (t >= a.T) return;
+        const int p = a.owner[t];
+        const double4 xn = a.epos4[t];
+        double v[NB];
+        lr_block_values<L, B, COMP>(xn.x, xn.y, xn.z, xn.w, v);
+        const double* ao = a.aold + ((size_t)Y * a.P + p) * NBMAX;
+        double acc = 0.0;
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            double s = fma(0.5 * Gs[i * NB + i], v[i], ao[i]);
+#pragma unroll
+            for (int j = 0; j < i; j++) s = fma(Gs[i * NB + j], v[j], s);
+            acc = fma(v[i], s, acc);
+        }
+        a.lpart[(size_t)Y * a.T + t] = acc;
+    } else {
+        const int t = blockIdx.x * blockDim.x + threadIdx.x;
+        if (t < a.T) a.lpart[(size_t)Y * a.T + t] = 0.0;
+    }
+}
+
+template <int L, int Y>
+__device__ __forceinline__ void lr_part_dispatch(int y, const LR2Args& a, const double* gblk, const double* g,
+                                                 double* Gs) {
+    if constexpr (Y < 12) {
+        if (y == Y) lr_block_part<L, Y / 2, Y % 2>(a, gblk, g, Gs);
+        else lr_part_dispatch<L, Y + 1>(y, a, gblk, g, Gs);
+    }
+}
+template <int L, int Y>
+__device__ __forceinline__ void lr_prop_dispatch(int y, const LR2Args& a, const double* gblk, double* Gs) {
+    if constexpr (Y < 12) {
+        if (y == Y) lr_block_prop<L, Y / 2, Y % 2>(a, gblk, Gs);
+        else lr_prop_dispatch<L, Y + 1>(y, a, gblk, Gs);
+    }
+}
+
+// grid.y = (block, component) pair Y; the workgroup stages that G tile in LDS once and
+// its lanes (one per moving charge / proposal) read it as broadcasts
 template <int L>
-__global__ void __launch_bounds__(64) k_lr_part(LR2Args a, const double* __restrict__ gblk,
-                                                const double* __restrict__ gvec) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.P) return;
-    const double4 xo = a.epos4[a.T + p];
-    double* ao = a.aold + (size_t)p * 2 * nhalf(L);
-    double c = 0.0;
-    lr_part_dispatch<L, 0>((int)blockIdx.y, xo.x, xo.y, xo.z, xo.w, gvec, gblk, ao, c);
-    a.cpart[(size_t)blockIdx.y * a.P + p] = c;
+__global__ void __launch_bounds__(256) k_lr_part(LR2Args a, const double* __restrict__ gblk,
+                                                 const double* __restrict__ gvec) {
+    constexpr int NBMAX = lr_nbmax(L);
+    __shared__ double Gs[NBMAX * NBMAX];
+    lr_part_dispatch<L, 0>((int)blockIdx.y, a, gblk, gvec, Gs);
 }
 
 template <int L>
-__global__ void __launch_bounds__(64) k_lr_prop2(LR2Args a, const double* __restrict__ gblk) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.T) return;
-    const int p = a.owner[t];
-    const double4 xn = a.epos4[t];
-    const double* ao = a.aold + (size_t)p * 2 * nhalf(L);
-    double acc = 0.0;
-    lr_prop_dispatch<L, 0>((int)blockIdx.y, xn.x, xn.y, xn.z, xn.w, ao, gblk, acc);
-    a.lpart[(size_t)blockIdx.y * a.T + t] = acc;
+__global__ void __launch_bounds__(256) k_lr_prop2(LR2Args a, const double* __restrict__ gblk) {
+    constexpr int NBMAX = lr_nbmax(L);
+    __shared__ double Gs[NBMAX * NBMAX];
+    lr_prop_dispatch<L, 0>((int)blockIdx.y, a, gblk, Gs);
 }
 
 // g = G M and E_lr = 0.5 M.g  (one block)
@@ -1916,8 +1982,8 @@ static void launch_far(EvalArgs a, hipStream_t s) {
 template <int L>
 static void launch_lr(LR2Args a, hipStream_t s) {
     if (a.T <= 0 || a.P <= 0) return;
-    hipLaunchKernelGGL(k_lr_part<L>, dim3((a.P + 63) / 64, 12), dim3(64), 0, s, a, a.gblk, a.gvec);
-    hipLaunchKernelGGL(k_lr_prop2<L>, dim3((a.T + 63) / 64, 12), dim3(64), 0, s, a, a.gblk);
+    hipLaunchKernelGGL(k_lr_part<L>, dim3((a.P + 255) / 256, 12), dim3(256), 0, s, a, a.gblk, a.gvec);
+    hipLaunchKernelGGL(k_lr_prop2<L>, dim3((a.T + 255) / 256, 12), dim3(256), 0, s, a, a.gblk);
 }
 template <int L>
 static void launch_mp(int n, int chunk, const double4* p, double* partial, int nblk, hipStream_t s) {
@@ -2009,8 +2075,8 @@ struct kmc_ctx {
     int occ_headroom = 8;
     int occmax = 0;
     int near_lanes = 27;
-    int sweep_block = 256;
-    int sweep_pairs = 1;
+    int sweep_block = 0;
+    int sweep_pairs = 2;
     int sweep_split = 1;
     int init_method = 0;              // 0 automatic, 1 direct P2L, 2 hierarchical
     int64_t last_P = 0, last_T = 0;   // sizes of the last propose (kernel timing)
@@ -2092,7 +2158,7 @@ static EvalArgs eval_args(kmc_ctx* c, int n, const double4* pos4, const int32_t*
 }
 
 static int ensure_lr_work(kmc_ctx* c, int64_t P) {
-    HIPCHK(c->aold.ensure((size_t)P * 2 * c->nh));
+    HIPCHK(c->aold.ensure((size_t)12 * P * lr_nbmax(c->L)));
     HIPCHK(c->cold.ensure((size_t)12 * P));
     return KMC_OK;
 }
@@ -2873,7 +2939,8 @@ int kmc_step(kmc_ctx* c, void* stream, int64_t P, int64_t T, const int64_t* d_pi
     if (check_ctx(c)) return KMC_E_ARG;
     if (!c->initialised) return set_err(KMC_E_STATE, "Run initialise before this call");
     if (P < 1 || T < 1 || T >= (int64_t)1 << 30 || nsteps < 1) return set_err(KMC_E_ARG, "bad proposal counts");
-    const bool lr_sized = !c->have_lr || ((size_t)P * 2 * c->nh <= c->aold.n && (size_t)12 * P <= c->cold.n);
+    const bool lr_sized =
+        !c->have_lr || ((size_t)12 * P * lr_nbmax(c->L) <= c->aold.n && (size_t)12 * P <= c->cold.n);
     if ((size_t)(T + P) > c->epos4.n || (size_t)T > c->owner.n || (size_t)T > c->rates.n ||
         (size_t)((T + 63) / 64) > c->bsum.n || !lr_sized)
         return set_err(KMC_E_STATE, "work buffers not sized: call kmc_propose once with this T before kmc_step");

@@ -166,7 +166,7 @@ class KMCFMM:
         """The C-ABI context (kmc_ctx*)."""
         return self._ctx
 
-    def set_tiles(self, near_lanes=27, sweep_block=256, sweep_pairs=1, sweep_split=1):
+    def set_tiles(self, near_lanes=27, sweep_block=0, sweep_pairs=2, sweep_split=1):
         _native.check(self._lib.kmc_set_tiles(self._ctx, int(near_lanes), int(sweep_block), int(sweep_pairs),
                                               int(sweep_split)),
                       "kmc_set_tiles")

@@ -33,6 +33,7 @@ Method (independent of ppmd's):
   reference's (SX, SY, SZ) form.
 """
 
+import functools
 import math
 import numpy as np
 
@@ -96,6 +97,7 @@ def _ring_vectors(lo, hi):
     return g[(ninf >= lo) & (ninf <= hi)].astype(np.float64)
 
 
+@functools.lru_cache(maxsize=8)
 def far_lattice_sums(pmax, extra=48):
     """S_p^q = sum over n in Z^3 with |n|_inf >= 2 of I~_p^q(n) (unit box), cube order.
 
@@ -129,6 +131,7 @@ def far_lattice_sums(pmax, extra=48):
     for p in range(pmax + 1):
         for q in range(-p, p + 1):
             out[p, q + pmax] = S[p, q + ptop].real
+    out.setflags(write=False)   # cached: shared by every caller
     return out
 
 

@@ -27,7 +27,7 @@ TABLE = os.path.join(_HERE, "tiles.json")
 NEAR_LANES = (1, 4, 8, 16, 27, 32, 64)
 # (sweep_pairs, sweep_block, sweep_split)
 SWEEP_SHAPES = ((1, 0, 1), (2, 0, 1), (1, 64, 1), (1, 256, 1), (2, 64, 1), (2, 256, 1), (1, 64, 3), (2, 64, 3))
-DEFAULT = {"near_lanes": 27, "sweep_block": 256, "sweep_pairs": 1, "sweep_split": 1}
+DEFAULT = {"near_lanes": 27, "sweep_block": 0, "sweep_pairs": 2, "sweep_split": 1}
 KEYS = ("near_lanes", "sweep_block", "sweep_pairs", "sweep_split")
 
 

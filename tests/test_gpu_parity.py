@@ -160,6 +160,34 @@ def test_accept_sequence_matches_oracle(bc):
     kmc.free()
 
 
+@pytest.mark.parametrize("bc", ("pbc", "free_space"))
+@pytest.mark.parametrize("L", (8, 11, 16))
+def test_every_tile_shape_matches_oracle(bc, L):
+    """Every accept-sweep tile shape (column pairs per lane, split or fused workgroup
+    layout with the LDS-staged update, image split over waves) and every near-field
+    lane count gives the oracle's dE and, after an accept, its expansions (one system,
+    one propose + accept per shape, mirrored on the oracle)."""
+    from coulomb_kmc_amd import tune
+    shapes = tune.sweep_candidates(bc != "free_space")
+    near = tune.NEAR_LANES
+    kmc, orc, pos, q, rng = make_pair(bc, N=300, E=6.0, r=4, L=L, seed=100 + L)
+    for i, (npair, block, split) in enumerate(shapes):
+        kmc.set_tiles(near_lanes=near[i % len(near)], sweep_block=block, sweep_pairs=npair, sweep_split=split)
+        pids = rng.choice(len(q), 4, replace=False)
+        moves = ball_moves(rng, kmc._pos(), pids, 3, 1.0, 6.0, bc)
+        dg = np.concatenate(kmc.propose(moves)) - kmc.energy
+        dc = np.concatenate(orc.propose(moves)) - orc.energy
+        assert np.max(np.abs(dg - dc) / np.maximum(np.abs(dc), 1e-3)) <= 1e-10, (npair, block, split)
+        pid, npos = moves[0][0], moves[0][1][1]
+        kmc.accept((pid, npos))
+        orc.accept(pid, npos)
+        eg, ec = kmc.get_expansions(), orc.get_expansions()
+        err = np.max(np.abs(eg - ec)) / np.max(np.abs(ec))
+        _print("tiles", bc, L, (npair, block, split), near[i % len(near)], "expansion err", err)
+        assert err <= 1e-11, (npair, block, split)
+    kmc.free()
+
+
 def test_accept_equals_reinitialise():
     """Sweep after a move == fresh initialise of the moved system (direct P2L, exact)."""
     from coulomb_kmc_amd import KMCFMM
