@@ -245,10 +245,12 @@ def run_batch(args):
     torch.cuda.synchronize()
     t_init = time.time() - t0
     log("initialised", len(systems), "systems in", round(t_init, 1), "s")
-    batch = KMCBatch(systems, kT=1.0, precision="mixed", streams=args.batch_streams)
-    t0 = time.time()
-    tiles = batch.autotune(lambda s: make(s.spec), reps=3, log=log)
-    log("autotune", round(time.time() - t0, 1), "s")
+    batch = KMCBatch(systems, kT=1.0, precision="mixed", streams=args.batch_streams, mode=args.batch_mode)
+    tiles = {}
+    if args.batch_mode == "graph":
+        t0 = time.time()
+        tiles = batch.autotune(lambda s: make(s.spec), reps=3, log=log)
+        log("autotune", round(time.time() - t0, 1), "s")
     batch.capture(G)
     torch.cuda.synchronize()
     for _ in range(warm // G):
@@ -278,7 +280,9 @@ def run_batch(args):
         "data": "synthetic (seeded uniform charges +-1, ball-displacement proposals, seeded Metropolis uniforms)",
         "config": {"workload": spec["workload"], "systems": len(systems), "charges": int(sum(x["N"] for x in specs)),
                    "proposals_per_step": T, "graph_steps": G, "streams": args.batch_streams,
-                   "tiles": {"L{}_C{}_T{}".format(*k): v for k, v in tiles.items()}},
+                   "mode": args.batch_mode,
+                   "tiles": {"L{}_C{}_T{}".format(*k): v for k, v in tiles.items()} or
+                            "batched kernels: fused sweep, 2 column pairs per lane, 32 near lanes per point"},
         "accepts_per_s": len(systems) * steps / (ms * 1e-3),
         "initialise_s": t_init,
         "flags": flags,
@@ -286,6 +290,7 @@ def run_batch(args):
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
+    batch.free()
     for s in systems:
         s.kmc.free()
 
@@ -297,6 +302,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=-1, help="-1: the workload's default")
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS) + ["batch-256"])
     ap.add_argument("--batch-streams", type=int, default=1, help="streams of the batched workload (1)")
+    ap.add_argument("--batch-mode", default="kernels", choices=("kernels", "graph"),
+                    help="batched kernels (one launch per kernel kind) or per-system steps in one graph")
     ap.add_argument("--graph-steps", type=int, default=10, help="steps captured per HIP graph")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")

@@ -4,24 +4,28 @@ Many independent systems stepped together on one GPU.
 Each system is its own KMCFMM context (its own leaf expansions, cell tables and work
 buffers).  One step of the batch is one KMC step of every system: the synthetic moves
 are materialised on the device, the proposals evaluated, one proposal selected by
-Metropolis and accepted (kmc_step).  The steps of all systems are recorded into one
-HIP graph, so a whole batch step (several, with ``steps_per_graph``) is a single graph
-launch from the host.
+Metropolis and accepted.  Two ways to run it:
 
-Only one stream per batch is supported: the systems' steps are recorded one after the
-other on the capturing stream (each system still overlaps its own long-range and
-accept-tail kernels with its near/far fields and sweep on its context's side
-stream).  Recording systems on extra torch side streams (forked from and joined to
-the capturing stream) ended in a host crash inside graph instantiation on this
-ROCm/PyTorch stack, so that mode raises.
+* ``mode="kernels"`` (default): kmc_batch_step launches each kernel kind once for the
+  whole batch (once per expansion order for the order-specialised kernels), the
+  system being blockIdx.z of a table of per-system kernel arguments, so a batch step
+  is ~20 launches whatever the number of systems.  The accept sweep uses the fused
+  layout with two column pairs per lane, the near field 32 lanes per point.
+* ``mode="graph"``: the systems' own kmc_step calls one after the other, with each
+  system's tiles from the autotuner (per order / leaf count / proposal count class).
 
-Tiles are chosen per (order, leaf count, proposal count) class by the autotuner
-(coulomb_kmc_amd/tune.py), timed on the first system of each class.
+Either way ``capture`` records ``steps_per_graph`` batch steps into one HIP graph, a
+batch step is one graph launch from the host.  The systems are recorded on one
+stream: recording them on extra torch side streams (forked from and joined to the
+capturing stream) ended in a host crash inside graph instantiation on this
+ROCm/PyTorch stack, so that option raises.
 """
+
+import ctypes
 
 import numpy as np
 
-from . import tune
+from . import _native, tune
 
 
 class BatchSystem:
@@ -55,15 +59,42 @@ class BatchSystem:
 
 
 class KMCBatch:
-    def __init__(self, systems, kT=1.0, precision="mixed", streams=1):
+    def __init__(self, systems, kT=1.0, precision="mixed", streams=1, mode="kernels"):
         if int(streams) != 1:
             raise NotImplementedError("batches are recorded on one stream (extra side streams crashed graph "
                                       "instantiation on this ROCm/PyTorch stack)")
+        if mode not in ("kernels", "graph"):
+            raise ValueError("mode must be 'kernels' or 'graph'")
         self.systems = list(systems)
         self.kT = float(kT)
         self.precision = precision
+        self.mode = mode
         self.graph = None
         self.steps_per_graph = 0
+        self._batch = None
+        if mode == "kernels" and self.systems:
+            lib = _native.lib()
+            n = len(self.systems)
+            arr = (ctypes.c_void_p * n)(*[s.kmc.handle.value for s in self.systems])
+            h = ctypes.c_void_p()
+            _native.check(lib.kmc_batch_create(ctypes.byref(h), arr, n), "kmc_batch_create")
+            self._batch = h
+            for i, s in enumerate(self.systems):
+                _native.check(lib.kmc_batch_set_system(
+                    h, i, s.uniforms.numel(), s.P, s.k, s.pids_all.data_ptr(), s.disp_all.data_ptr(),
+                    s.uniforms.data_ptr(), s.counter.data_ptr(), s.selected.data_ptr(), s.d_pid.data_ptr(),
+                    s.d_es.data_ptr(), s.d_new.data_ptr(), s.d_dE.data_ptr()), "kmc_batch_set_system")
+
+    def free(self):
+        if self._batch is not None and self._batch.value:
+            _native.lib().kmc_batch_destroy(self._batch)
+        self._batch = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
     @property
     def proposals_per_step(self):
@@ -97,12 +128,27 @@ class KMCBatch:
         return choices
 
     def prepare(self):
-        """Sizes every context's work buffers (one eager propose each), outside capture."""
+        """Sizes every context's work buffers (and, in kernels mode, uploads the argument
+        tables); allocates, so it runs outside capture."""
         import torch
-        for s in self.systems:
-            s.materialize()
-            s.kmc.propose_device(s.d_pid, s.d_es, s.d_new, s.d_dE, precision=self.precision)
+        if self.mode == "kernels":
+            _native.check(_native.lib().kmc_batch_prepare(self._batch, self.kT, _native.PREC[self.precision]),
+                          "kmc_batch_prepare")
+        else:
+            for s in self.systems:
+                s.materialize()
+                s.kmc.propose_device(s.d_pid, s.d_es, s.d_new, s.d_dE, precision=self.precision)
         torch.cuda.synchronize()
+
+    def step(self, stream=None):
+        """One KMC step of every system on the stream (torch's current stream by default)."""
+        import torch
+        st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        if self.mode == "kernels":
+            _native.check(_native.lib().kmc_batch_step(self._batch, ctypes.c_void_p(st)), "kmc_batch_step")
+        else:
+            for s in self.systems:
+                s.step(self.kT, self.precision, stream=st)
 
     def capture(self, steps_per_graph=1):
         """Records steps_per_graph batch steps into one graph."""
@@ -113,8 +159,7 @@ class KMCBatch:
         torch.cuda.synchronize()
         with torch.cuda.graph(g, stream=main):
             for _ in range(steps_per_graph):
-                for s in self.systems:
-                    s.step(self.kT, self.precision, stream=main.cuda_stream)
+                self.step(main.cuda_stream)
         self.graph = g
         self.steps_per_graph = steps_per_graph
         return g

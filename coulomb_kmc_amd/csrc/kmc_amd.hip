@@ -530,7 +530,7 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgs a) {
 // cache line per lane and the groups of a leaf would re-fetch the same lines).
 constexpr int stage_stride(int nh) { return nh + ((9 - nh % 8) % 8); }   // = 1 (mod 8): conflict-free b128 rows
 template <int L, int NP, bool PER>
-__global__ void __launch_bounds__(512, ((NP == 1 && L <= 16) ? 4 : 2)) k_sweep_fused(SweepArgs a) {
+__device__ __forceinline__ void sweep_fused_body(const SweepArgs& a) {
     constexpr int W = L + 1;
     constexpr int NH = nhalf(L);
     constexpr int ST = stage_stride(NH);
@@ -564,6 +564,17 @@ __global__ void __launch_bounds__(512, ((NP == 1 && L <= 16) ? 4 : 2)) k_sweep_f
         }
     }
 }
+template <int L, int NP, bool PER>
+__global__ void __launch_bounds__(512, ((NP == 1 && L <= 16) ? 4 : 2)) k_sweep_fused(SweepArgs a) {
+    sweep_fused_body<L, NP, PER>(a);
+}
+template <int L, int NP, bool PER>
+__global__ void __launch_bounds__(512, ((NP == 1 && L <= 16) ? 4 : 2)) kb_sweep_fused(const SweepArgs* __restrict__ args) {
+    const SweepArgs& a = args[blockIdx.z];
+    if ((int)blockIdx.x * 64 >= a.ncells) return;   // grid sized for the largest system of the batch
+    sweep_fused_body<L, NP, PER>(a);
+}
+
 
 // ============================================================================
 // far field: phi = Re sum L_lm R_l^m(x - centre), one lane per evaluation point
@@ -628,7 +639,7 @@ __device__ __forceinline__ double far_dispatch(int pair, double dx, double dy, d
 // waves reading one leaf expansion run together and share its cache lines;
 // out[pair * n + i]
 template <int L>
-__global__ void __launch_bounds__(512) k_far(EvalArgs a) {
+__device__ __forceinline__ void far_body(const EvalArgs& a) {
     const int i = blockIdx.x * 64 + threadIdx.x;
     const int pair = __builtin_amdgcn_readfirstlane((int)(blockIdx.y * blockDim.y + threadIdx.y));
     if (i >= a.n || pair >= (L + 1) / 2) return;
@@ -643,6 +654,11 @@ __global__ void __launch_bounds__(512) k_far(EvalArgs a) {
     const double2* le = a.lexp + (size_t)c * nhalf(L);
     a.out[(size_t)pair * a.n + i] = p.w * far_dispatch<L, 0>(pair, dx, dy, dz, r2, le);
 }
+template <int L>
+__global__ void __launch_bounds__(512) k_far(EvalArgs a) { far_body<L>(a); }
+template <int L>
+__global__ void __launch_bounds__(512) kb_far(const EvalArgs* __restrict__ args) { far_body<L>(args[blockIdx.z]); }
+
 
 // ============================================================================
 // near field: q_t * sum over the 27 neighbour leaves of q_j / |x_t - x_j|
@@ -699,7 +715,7 @@ __global__ void __launch_bounds__(256) k_near(EvalArgs a) {
 // near field, one lane per neighbour leaf: 32 lanes per evaluation point, lanes
 // 0..26 each walk one leaf of the stencil, shuffle reduction over the 32 lanes.
 template <bool MIXED>
-__global__ void __launch_bounds__(256) k_near_cells(EvalArgs a) {
+__device__ __forceinline__ void near_cells_body(const EvalArgs& a) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = gid >> 5;
     const int k = gid & 31;
@@ -754,6 +770,13 @@ __global__ void __launch_bounds__(256) k_near_cells(EvalArgs a) {
     for (int o = 16; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 32);
     if (i < a.n && k == 0) a.out[i] = acc * a.pos4[i].w;
 }
+template <bool MIXED>
+__global__ void __launch_bounds__(256) k_near_cells(EvalArgs a) { near_cells_body<MIXED>(a); }
+template <bool MIXED>
+__global__ void __launch_bounds__(256) kb_near_cells(const EvalArgs* __restrict__ args) {
+    near_cells_body<MIXED>(args[blockIdx.z]);
+}
+
 
 // ============================================================================
 // proposal setup (tuple form) and validation
@@ -788,7 +811,7 @@ __device__ __forceinline__ int owner_of(int t, const int64_t* es, int P) {
 
 // One thread per proposal t (< T) and per moving charge p (< P).
 // kmc_mpi_decomp.py:312-345 (domain and max_move checks, minimum image for '27'/PBC)
-__global__ void k_setup(SetupArgs a) {
+__device__ __forceinline__ void setup_body(const SetupArgs& a) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     int bad = 0;
     if (x < a.P) {
@@ -828,6 +851,10 @@ __global__ void k_setup(SetupArgs a) {
     }
     if (bad) atomicAdd(a.err, bad);
 }
+__global__ void k_setup(SetupArgs a) { setup_body(a); }
+// batched form: system blockIdx.z of an argument table
+__global__ void kb_setup(const SetupArgs* __restrict__ args) { setup_body(args[blockIdx.z]); }
+
 
 // ============================================================================
 // self term + long-range difference + combine (diff_lib)
@@ -906,7 +933,7 @@ __device__ __forceinline__ double combine_one(const CombineArgs& a, int t) {
 // one lane per proposal, blocks of 64; with rates: the Metropolis rate per proposal
 // and the rate sum of each block of 64 (for the selection)
 template <bool MIXED>
-__global__ void __launch_bounds__(64) k_combine(CombineArgs a) {
+__device__ __forceinline__ void combine_body(const CombineArgs& a) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     double rate = 0.0;
     if (t < a.T) {
@@ -916,11 +943,18 @@ __global__ void __launch_bounds__(64) k_combine(CombineArgs a) {
             a.rates[t] = rate;
         }
     }
-    if (a.rates) {
+    if (a.rates && (int)blockIdx.x * 64 < a.T) {   // (a batched grid can run past this system's T)
         const double bs = wave_sum(rate);
         if (threadIdx.x == 0) a.bsum[blockIdx.x] = bs;
     }
 }
+template <bool MIXED>
+__global__ void __launch_bounds__(64) k_combine(CombineArgs a) { combine_body<MIXED>(a); }
+template <bool MIXED>
+__global__ void __launch_bounds__(64) kb_combine(const CombineArgs* __restrict__ args) {
+    combine_body<MIXED>(args[blockIdx.z]);
+}
+
 
 // ============================================================================
 // long range
@@ -1153,6 +1187,20 @@ __global__ void __launch_bounds__(256) k_lr_prop2(LR2Args a, const double* __res
     __shared__ double Gs[NBMAX * NBMAX];
     lr_prop_dispatch<L, 0>((int)blockIdx.y, a, gblk, Gs);
 }
+template <int L>
+__global__ void __launch_bounds__(256) kb_lr_part(const LR2Args* __restrict__ args) {
+    constexpr int NBMAX = lr_nbmax(L);
+    __shared__ double Gs[NBMAX * NBMAX];
+    const LR2Args& a = args[blockIdx.z];
+    lr_part_dispatch<L, 0>((int)blockIdx.y, a, a.gblk, a.gvec, Gs);
+}
+template <int L>
+__global__ void __launch_bounds__(256) kb_lr_prop2(const LR2Args* __restrict__ args) {
+    constexpr int NBMAX = lr_nbmax(L);
+    __shared__ double Gs[NBMAX * NBMAX];
+    const LR2Args& a = args[blockIdx.z];
+    lr_prop_dispatch<L, 0>((int)blockIdx.y, a, a.gblk, Gs);
+}
 
 // g = G M and E_lr = 0.5 M.g  (one block)
 __global__ void __launch_bounds__(256) k_lr_refresh(int nrow, const double* gdat, const int64_t* gptr,
@@ -1267,7 +1315,7 @@ struct TailArgs {
 // particle arrays (thread 0), M += m(old, -q) then += m(new, +q)
 // (kmc_full_long_range.py:270-316), g = G M, E_lr = 0.5 M.g, step counter.
 template <int L>
-__global__ void __launch_bounds__(256) k_accept_tail(TailArgs a) {
+__device__ __forceinline__ void accept_tail_body(const TailArgs& a) {
     constexpr int NR = 2 * nhalf(L);
     __shared__ double dold[NR], dnew[NR];
     __shared__ double part[256];
@@ -1300,6 +1348,13 @@ __global__ void __launch_bounds__(256) k_accept_tail(TailArgs a) {
     }
     if (tid == 0 && a.counter) *a.counter += 1;
 }
+template <int L>
+__global__ void __launch_bounds__(256) k_accept_tail(TailArgs a) { accept_tail_body<L>(a); }
+template <int L>
+__global__ void __launch_bounds__(256) kb_accept_tail(const TailArgs* __restrict__ args) {
+    accept_tail_body<L>(args[blockIdx.z]);
+}
+
 
 // ============================================================================
 // Metropolis selection on the device (one block)
@@ -1334,7 +1389,7 @@ struct SelectArgs {
 // sums a contiguous chunk of block sums; chunk sums are scanned within each wave
 // (shuffles) and across the 16 waves (LDS); the thread whose chunk holds u * total
 // walks its block sums, then that block's rates, to find the proposal.
-__global__ void __launch_bounds__(1024) k_select(SelectArgs a) {
+__device__ __forceinline__ void select_body(const SelectArgs& a) {
     __shared__ double wsum[16];
     __shared__ int pick;
     const int tid = threadIdx.x;
@@ -1425,6 +1480,11 @@ __global__ void __launch_bounds__(1024) k_select(SelectArgs a) {
         *a.energy += a.dE[k];
     }
 }
+__global__ void __launch_bounds__(1024) k_select(SelectArgs a) { select_body(a); }
+__global__ void __launch_bounds__(1024) kb_select(const SelectArgs* __restrict__ args) {
+    select_body(args[blockIdx.z]);
+}
+
 
 
 // ============================================================================
@@ -1490,32 +1550,47 @@ __global__ void k_dats_scatter(int n, int64_t max_prop, const int64_t* smc, cons
 }
 
 // synthetic move stream (benchmarks/tests)
-__global__ void k_materialize(Geom g, int n, int64_t nsteps, int P, int k, const int64_t* pids_all,
-                              const double* disp_all, const int64_t* counter, const double4* ppos4, int64_t* pid,
-                              int64_t* es, double* new_pos, int* flags) {
+struct MatArgs {
+    Geom g;
+    int n;
+    int64_t nsteps;
+    int P, k;
+    const int64_t* pids_all;
+    const double* disp_all;
+    const int64_t* counter;
+    const double4* ppos4;
+    int64_t* pid;
+    int64_t* es;
+    double* new_pos;
+    int* flags;
+};
+__device__ __forceinline__ void materialize_body(const MatArgs& a) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int P = a.P, k = a.k;
     const int T = P * k;
-    const int64_t step = *counter % nsteps;
-    if (t <= P && t >= 0 && t < P + 1) es[t] = (int64_t)t * k;
+    const int64_t step = *a.counter % a.nsteps;
+    if (t <= P && t >= 0 && t < P + 1) a.es[t] = (int64_t)t * k;
     if (t >= T) return;
     const int p = t / k;
-    int64_t i = pids_all[step * P + p];
-    if (i < 0 || i >= n) { atomicOr(flags, 16); i = 0; }
-    if (t % k == 0) pid[p] = i;
-    const double4 o = ppos4[i];
-    const double* d = disp_all + (step * T + t) * 3;
+    int64_t i = a.pids_all[step * P + p];
+    if (i < 0 || i >= a.n) { atomicOr(a.flags, 16); i = 0; }
+    if (t % k == 0) a.pid[p] = i;
+    const double4 o = a.ppos4[i];
+    const double* d = a.disp_all + (step * T + t) * 3;
     double v[3] = {o.x + d[0], o.y + d[1], o.z + d[2]};
-    const double h = 0.5 * g.E;
+    const double h = 0.5 * a.g.E;
     for (int c = 0; c < 3; c++) {
-        if (g.bc == KMC_BC_FREE_SPACE) {
+        if (a.g.bc == KMC_BC_FREE_SPACE) {
             v[c] = v[c] < -h ? -h : (v[c] > h ? h : v[c]);
         } else {
-            if (v[c] < -h) v[c] += g.E;
-            if (v[c] >= h) v[c] -= g.E;
+            if (v[c] < -h) v[c] += a.g.E;
+            if (v[c] >= h) v[c] -= a.g.E;
         }
-        new_pos[3 * t + c] = v[c];
+        a.new_pos[3 * t + c] = v[c];
     }
 }
+__global__ void k_materialize(MatArgs a) { materialize_body(a); }
+__global__ void kb_materialize(const MatArgs* __restrict__ args) { materialize_body(args[blockIdx.z]); }
 
 __global__ void k_half_sum(int n, const double* a, const double* b, int nb, double* out) {
     // out = 0.5 * sum_i (a_i + sum_j b[j*n + i])  (one block, deterministic)
@@ -1994,6 +2069,41 @@ static void launch_md(TailArgs a, hipStream_t s) {
     hipLaunchKernelGGL(k_accept_tail<L>, dim3(1), dim3(256), 0, s, a);
 }
 
+// batched launches: argument tables in device memory, system = blockIdx.z
+struct BatchShape {
+    int n;           // systems in the table
+    int max_ne;      // largest T + P
+    int max_P, max_T;
+    int max_cells;   // largest leaf count (a multiple of 64)
+};
+typedef void (*BFarLaunch)(const EvalArgs*, BatchShape, hipStream_t);
+typedef void (*BLRLaunch)(const LR2Args*, BatchShape, hipStream_t);
+typedef void (*BSweepLaunch)(const SweepArgs*, BatchShape, bool, hipStream_t);
+typedef void (*BTailLaunch)(const TailArgs*, BatchShape, hipStream_t);
+
+template <int L>
+static void blaunch_far(const EvalArgs* t, BatchShape b, hipStream_t s) {
+    constexpr int npair = (L + 1) / 2;
+    constexpr int py = npair <= 8 ? npair : (npair + 1) / 2;
+    hipLaunchKernelGGL(kb_far<L>, dim3((b.max_ne + 63) / 64, (npair + py - 1) / py, b.n), dim3(64, py), 0, s, t);
+}
+template <int L>
+static void blaunch_lr(const LR2Args* t, BatchShape b, hipStream_t s) {
+    hipLaunchKernelGGL(kb_lr_part<L>, dim3((b.max_P + 255) / 256, 12, b.n), dim3(256), 0, s, t);
+    hipLaunchKernelGGL(kb_lr_prop2<L>, dim3((b.max_T + 255) / 256, 12, b.n), dim3(256), 0, s, t);
+}
+// fused accept sweep, two column pairs per lane
+template <int L>
+static void blaunch_sweep(const SweepArgs* t, BatchShape b, bool per, hipStream_t s) {
+    constexpr int ng = ((L + 1) / 2 + 1) / 2;
+    if (per) hipLaunchKernelGGL((kb_sweep_fused<L, 2, true>), dim3(b.max_cells / 64, 1, b.n), dim3(64, ng), 0, s, t);
+    else hipLaunchKernelGGL((kb_sweep_fused<L, 2, false>), dim3(b.max_cells / 64, 1, b.n), dim3(64, ng), 0, s, t);
+}
+template <int L>
+static void blaunch_tail(const TailArgs* t, BatchShape b, hipStream_t s) {
+    hipLaunchKernelGGL(kb_accept_tail<L>, dim3(1, 1, b.n), dim3(256), 0, s, t);
+}
+
 struct OrderFns {
     SweepLaunch sweep;
     EvalLaunch far;
@@ -2001,10 +2111,15 @@ struct OrderFns {
     MpLaunch mp;
     MdLaunch md;
     P2LNearLaunch p2ln;
+    BFarLaunch bfar;
+    BLRLaunch blr;
+    BSweepLaunch bsweep;
+    BTailLaunch btail;
 };
 template <int L>
 static OrderFns fns_for() {
-    return {launch_sweep<L>, launch_far<L>, launch_lr<L>, launch_mp<L>, launch_md<L>, launch_p2l_near<L>};
+    return {launch_sweep<L>, launch_far<L>,  launch_lr<L>,    launch_mp<L>,     launch_md<L>,
+            launch_p2l_near<L>, blaunch_far<L>, blaunch_lr<L>, blaunch_sweep<L>, blaunch_tail<L>};
 }
 static bool order_fns(int L, OrderFns* f) {
     switch (L) {
@@ -2178,6 +2293,128 @@ static LR2Args lr_args(kmc_ctx* c, int64_t P, int64_t T) {
 }
 
 // propose kernels after the evaluation arrays are filled (T new points, then P old)
+// ---------------------------------------------------------------------------
+// kernel arguments of one context (shared by the single-system and batched paths)
+// ---------------------------------------------------------------------------
+static bool uses_lr(const kmc_ctx* c) { return (c->bc == KMC_BC_PBC || c->bc == KMC_BC_FF_ONLY) && c->have_lr; }
+
+static SetupArgs setup_args(kmc_ctx* c, int64_t P, int64_t T, const int64_t* d_pid, const int64_t* d_es,
+                            const double* d_new_pos, const double* d_new_q) {
+    SetupArgs a;
+    a.g = c->geom();
+    a.max_move = c->max_move;
+    a.P = (int)P;
+    a.T = (int)T;
+    a.n = (int)c->n;
+    a.pid = d_pid;
+    a.es = d_es;
+    a.new_pos = d_new_pos;
+    a.new_q = d_new_q;
+    a.ppos4 = c->ppos4.p;
+    a.pcell = c->pcell.p;
+    a.epos4 = c->epos4.p;
+    a.ecell = c->ecell.p;
+    a.eexcl = c->eexcl.p;
+    a.owner = c->owner.p;
+    a.err = c->err.p;
+    return a;
+}
+
+static CombineArgs combine_args(kmc_ctx* c, int64_t P, int64_t T, double* d_out, const int64_t* rate_loc, double kT,
+                                double* rates) {
+    const bool lr = uses_lr(c);
+    CombineArgs ca;
+    ca.g = c->geom();
+    ca.T = (int)T;
+    ca.eu = c->eu;
+    ca.self27 = c->self27;
+    ca.epos4 = c->epos4.p;
+    ca.owner = c->owner.p;
+    ca.UD = c->UD.p;
+    ca.UI = c->UI.p;
+    ca.nfar = (c->L + 1) / 2;
+    ca.lpart = lr ? c->lrd.p : nullptr;
+    ca.cpart = lr ? c->cold.p : nullptr;
+    ca.P = (int)P;
+    ca.rate_loc = rate_loc;
+    ca.out = d_out;
+    ca.kT = kT;
+    ca.rates = rates;
+    ca.bsum = rates ? c->bsum.p : nullptr;
+    return ca;
+}
+
+static TailArgs tail_args(kmc_ctx* c, int64_t* d_counter) {
+    TailArgs ta;
+    ta.rec = c->rec.p;
+    ta.occ = c->occ.p;
+    ta.slot_pos = c->slot_pos.p;
+    ta.slot_id = c->slot_id.p;
+    ta.occmax = c->occmax;
+    ta.ppos4 = c->ppos4.p;
+    ta.pcell = c->pcell.p;
+    ta.flags = c->flags.p;
+    ta.lr = uses_lr(c) ? 1 : 0;
+    ta.M = c->M.p;
+    ta.gdat = c->gdat.p;
+    ta.gptr = c->gptr.p;
+    ta.gidx = c->gidx.p;
+    ta.g = c->gvec.p;
+    ta.elr = c->elr.p;
+    ta.counter = d_counter;
+    return ta;
+}
+
+static SweepArgs sweep_rec_args(kmc_ctx* c) {
+    SweepArgs sa;
+    sa.g = c->geom();
+    sa.lexp = c->lexp.p;
+    sa.ncells = (int)c->ncells;
+    sa.mode = 0;
+    sa.rec = c->rec.p;
+    sa.src_pos4 = nullptr;
+    sa.src_cell = nullptr;
+    sa.src_begin = sa.src_end = 0;
+    return sa;
+}
+
+// work buffers sized for P moving charges and T proposals (a step allocates nothing)
+static bool step_sized(const kmc_ctx* c, int64_t P, int64_t T) {
+    const bool lr_sized =
+        !c->have_lr || ((size_t)12 * P * lr_nbmax(c->L) <= c->aold.n && (size_t)12 * P <= c->cold.n);
+    return (size_t)(T + P) <= c->epos4.n && (size_t)T <= c->owner.n && (size_t)T <= c->rates.n &&
+           (size_t)((T + 63) / 64) <= c->bsum.n && lr_sized;
+}
+
+static SelectArgs select_args(kmc_ctx* c, int64_t P, int64_t T, const int64_t* d_pid, double* d_dE, int64_t nsteps,
+                              const double* d_uniform, double kT, int64_t* d_counter, int64_t* d_selected) {
+    SelectArgs sa;
+    sa.g = c->geom();
+    sa.T = (int)T;
+    sa.n = (int)c->n;
+    sa.nsteps = nsteps;
+    sa.kT = kT;
+    sa.dE = d_dE;
+    sa.rates = c->rates.p;
+    sa.bsum = c->bsum.p;
+    sa.uniform = d_uniform;
+    sa.counter = d_counter;
+    sa.selected = d_selected;
+    sa.epos4 = c->epos4.p;
+    sa.ecell = c->ecell.p;
+    sa.owner = c->owner.p;
+    sa.pid = d_pid;
+    sa.ppos4 = c->ppos4.p;
+    sa.pcell = c->pcell.p;
+    sa.occ = c->occ.p;
+    sa.occmax = c->occmax;
+    sa.energy = c->energy.p;
+    sa.rec = c->rec.p;
+    sa.flags = c->flags.p;
+    (void)P;
+    return sa;
+}
+
 // the side stream waits for everything issued so far on s
 static int fork_side(kmc_ctx* c, hipStream_t s, int i) {
     HIPCHK(hipEventRecord(c->ev_fork[i], s));
@@ -2214,24 +2451,7 @@ static int run_propose_core(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, dou
         int rc = join_side(c, s, 0);
         if (rc) return rc;
     }
-    CombineArgs ca;
-    ca.g = c->geom();
-    ca.T = (int)T;
-    ca.eu = c->eu;
-    ca.self27 = c->self27;
-    ca.epos4 = c->epos4.p;
-    ca.owner = c->owner.p;
-    ca.UD = c->UD.p;
-    ca.UI = c->UI.p;
-    ca.nfar = (c->L + 1) / 2;
-    ca.lpart = lr ? c->lrd.p : nullptr;
-    ca.cpart = lr ? c->cold.p : nullptr;
-    ca.P = (int)P;
-    ca.rate_loc = rate_loc;
-    ca.out = d_out;
-    ca.kT = kT;
-    ca.rates = rates;
-    ca.bsum = rates ? c->bsum.p : nullptr;
+    const CombineArgs ca = combine_args(c, P, T, d_out, rate_loc, kT, rates);
     if (T > 0) {
         if (mixed) hipLaunchKernelGGL(k_combine<true>, dim3((T + 63) / 64), dim3(64), 0, s, ca);
         else hipLaunchKernelGGL(k_combine<false>, dim3((T + 63) / 64), dim3(64), 0, s, ca);
@@ -2242,23 +2462,7 @@ static int run_propose_core(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, dou
 
 static int launch_setup(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, const int64_t* d_pid, const int64_t* d_es,
                         const double* d_new_pos, const double* d_new_q) {
-    SetupArgs a;
-    a.g = c->geom();
-    a.max_move = c->max_move;
-    a.P = (int)P;
-    a.T = (int)T;
-    a.n = (int)c->n;
-    a.pid = d_pid;
-    a.es = d_es;
-    a.new_pos = d_new_pos;
-    a.new_q = d_new_q;
-    a.ppos4 = c->ppos4.p;
-    a.pcell = c->pcell.p;
-    a.epos4 = c->epos4.p;
-    a.ecell = c->ecell.p;
-    a.eexcl = c->eexcl.p;
-    a.owner = c->owner.p;
-    a.err = c->err.p;
+    const SetupArgs a = setup_args(c, P, T, d_pid, d_es, d_new_pos, d_new_q);
     const int64_t nthr = std::max(T, P);
     if (P > 0) hipLaunchKernelGGL(k_setup, dim3((nthr + 255) / 256), dim3(256), 0, s, a);
     HIPCHK(hipGetLastError());
@@ -2266,23 +2470,7 @@ static int launch_setup(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, const i
 }
 
 static int run_accept_tail(kmc_ctx* c, hipStream_t s, int64_t* d_counter) {
-    TailArgs ta;
-    ta.rec = c->rec.p;
-    ta.occ = c->occ.p;
-    ta.slot_pos = c->slot_pos.p;
-    ta.slot_id = c->slot_id.p;
-    ta.occmax = c->occmax;
-    ta.ppos4 = c->ppos4.p;
-    ta.pcell = c->pcell.p;
-    ta.flags = c->flags.p;
-    ta.lr = ((c->bc == KMC_BC_PBC || c->bc == KMC_BC_FF_ONLY) && c->have_lr) ? 1 : 0;
-    ta.M = c->M.p;
-    ta.gdat = c->gdat.p;
-    ta.gptr = c->gptr.p;
-    ta.gidx = c->gidx.p;
-    ta.g = c->gvec.p;
-    ta.elr = c->elr.p;
-    ta.counter = d_counter;
+    const TailArgs ta = tail_args(c, d_counter);
     if (c->bc == KMC_BC_FF_ONLY) {
         c->fns.md(ta, s);
         HIPCHK(hipGetLastError());
@@ -2293,15 +2481,7 @@ static int run_accept_tail(kmc_ctx* c, hipStream_t s, int64_t* d_counter) {
     int rc = fork_side(c, s, 1);
     if (rc) return rc;
     c->fns.md(ta, c->side);
-    SweepArgs sa;
-    sa.g = c->geom();
-    sa.lexp = c->lexp.p;
-    sa.ncells = (int)c->ncells;
-    sa.mode = 0;
-    sa.rec = c->rec.p;
-    sa.src_pos4 = nullptr;
-    sa.src_cell = nullptr;
-    sa.src_begin = sa.src_end = 0;
+    const SweepArgs sa = sweep_rec_args(c);
     c->fns.sweep(sa, (int)c->ncells, s, c->sweep_block, c->sweep_pairs, c->sweep_split);
     rc = join_side(c, s, 1);
     if (rc) return rc;
@@ -2939,44 +3119,39 @@ int kmc_step(kmc_ctx* c, void* stream, int64_t P, int64_t T, const int64_t* d_pi
     if (check_ctx(c)) return KMC_E_ARG;
     if (!c->initialised) return set_err(KMC_E_STATE, "Run initialise before this call");
     if (P < 1 || T < 1 || T >= (int64_t)1 << 30 || nsteps < 1) return set_err(KMC_E_ARG, "bad proposal counts");
-    const bool lr_sized =
-        !c->have_lr || ((size_t)12 * P * lr_nbmax(c->L) <= c->aold.n && (size_t)12 * P <= c->cold.n);
-    if ((size_t)(T + P) > c->epos4.n || (size_t)T > c->owner.n || (size_t)T > c->rates.n ||
-        (size_t)((T + 63) / 64) > c->bsum.n || !lr_sized)
+    if (!step_sized(c, P, T))
         return set_err(KMC_E_STATE, "work buffers not sized: call kmc_propose once with this T before kmc_step");
     hipStream_t s = (hipStream_t)stream;
     int rc = launch_setup(c, s, P, T, d_pid, d_es, d_new_pos, nullptr);
     if (rc) return rc;
     rc = run_propose_core(c, s, P, T, d_dE, nullptr, precision == KMC_PREC_MIXED, kT, c->rates.p);
     if (rc) return rc;
-    SelectArgs sa;
-    sa.g = c->geom();
-    sa.T = (int)T;
-    sa.n = (int)c->n;
-    sa.nsteps = nsteps;
-    sa.kT = kT;
-    sa.dE = d_dE;
-    sa.rates = c->rates.p;
-    sa.bsum = c->bsum.p;
-    sa.uniform = d_uniform;
-    sa.counter = d_counter;
-    sa.selected = d_selected;
-    sa.epos4 = c->epos4.p;
-    sa.ecell = c->ecell.p;
-    sa.owner = c->owner.p;
-    sa.pid = d_pid;
-    sa.ppos4 = c->ppos4.p;
-    sa.pcell = c->pcell.p;
-    sa.occ = c->occ.p;
-    sa.occmax = c->occmax;
-    sa.energy = c->energy.p;
-    sa.rec = c->rec.p;
-    sa.flags = c->flags.p;
+    const SelectArgs sa = select_args(c, P, T, d_pid, d_dE, nsteps, d_uniform, kT, d_counter, d_selected);
     hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, s, sa);
     rc = run_accept_tail(c, s, d_counter);
     if (rc) return rc;
     HIPCHK(hipGetLastError());
     return KMC_OK;
+}
+
+static MatArgs mat_args(kmc_ctx* c, int64_t nsteps, int64_t P, int64_t k, const int64_t* d_pids_all,
+                        const double* d_disp_all, const int64_t* d_counter, int64_t* d_pid, int64_t* d_es,
+                        double* d_new_pos) {
+    MatArgs ma;
+    ma.g = c->geom();
+    ma.n = (int)c->n;
+    ma.nsteps = nsteps;
+    ma.P = (int)P;
+    ma.k = (int)k;
+    ma.pids_all = d_pids_all;
+    ma.disp_all = d_disp_all;
+    ma.counter = d_counter;
+    ma.ppos4 = c->ppos4.p;
+    ma.pid = d_pid;
+    ma.es = d_es;
+    ma.new_pos = d_new_pos;
+    ma.flags = c->flags.p;
+    return ma;
 }
 
 int kmc_materialize_moves(kmc_ctx* c, void* stream, int64_t nsteps, int64_t P, int64_t k,
@@ -2988,9 +3163,8 @@ int kmc_materialize_moves(kmc_ctx* c, void* stream, int64_t nsteps, int64_t P, i
     hipStream_t s = (hipStream_t)stream;
     const int64_t T = P * k;
     const int64_t threads = std::max(T, P + 1);
-    hipLaunchKernelGGL(k_materialize, dim3((threads + 255) / 256), dim3(256), 0, s, c->geom(), (int)c->n, nsteps,
-                       (int)P, (int)k, d_pids_all, d_disp_all, d_counter, c->ppos4.p, d_pid, d_es, d_new_pos,
-                       c->flags.p);
+    MatArgs ma = mat_args(c, nsteps, P, k, d_pids_all, d_disp_all, d_counter, d_pid, d_es, d_new_pos);
+    hipLaunchKernelGGL(k_materialize, dim3((threads + 255) / 256), dim3(256), 0, s, ma);
     HIPCHK(hipGetLastError());
     return KMC_OK;
 }
@@ -3118,3 +3292,225 @@ int kmc_sweep_hop(kmc_ctx* c, void* stream, double q, const double* h_old, const
 }
 
 }  // extern "C"
+
+// ============================================================================
+// batches of independent systems: one launch per kernel kind (and order) for all
+// ============================================================================
+struct kmc_batch {
+    struct Sys {
+        kmc_ctx* c = nullptr;
+        int64_t nsteps = 0, P = 0, k = 0;
+        const int64_t* pids_all = nullptr;
+        const double* disp_all = nullptr;
+        const double* uniform = nullptr;
+        int64_t* counter = nullptr;
+        int64_t* selected = nullptr;
+        int64_t* pid = nullptr;
+        int64_t* es = nullptr;
+        double* new_pos = nullptr;
+        double* dE = nullptr;
+        bool set = false;
+    };
+    struct Group {                    // systems of one order L and one periodicity
+        int L = 0;
+        bool per = false;
+        OrderFns fns{};
+        BatchShape shape{}, lr_shape{};
+        DBuf<EvalArgs> far;
+        DBuf<LR2Args> lr;
+        DBuf<SweepArgs> sweep;
+        DBuf<TailArgs> tail;
+    };
+    std::vector<Sys> sys;
+    std::vector<Group*> groups;
+    double kT = 1.0;
+    bool mixed = true;
+    bool ready = false;
+    BatchShape all{};
+    DBuf<MatArgs> mat;
+    DBuf<SetupArgs> setup;
+    DBuf<EvalArgs> nearA;
+    DBuf<CombineArgs> comb;
+    DBuf<SelectArgs> sel;
+    ~kmc_batch() {
+        for (Group* g : groups) delete g;
+    }
+};
+
+int kmc_batch_create(kmc_batch** out, kmc_ctx* const* ctxs, int32_t n) {
+    if (!out) return set_err(KMC_E_ARG, "null output pointer");
+    *out = nullptr;
+    if (n < 1 || n > 65535 || !ctxs) return set_err(KMC_E_ARG, "batch size must be 1 .. 65535");
+    for (int i = 0; i < n; i++) {
+        kmc_ctx* c = ctxs[i];
+        if (!c) return set_err(KMC_E_ARG, "null context in batch");
+        if (!c->initialised) return set_err(KMC_E_STATE, "Run initialise before batching a context");
+        if (c->bc == KMC_BC_FF_ONLY) return set_err(KMC_E_UNSUPPORTED, "ff-only systems are not batched");
+        if (c->ncells % 64) return set_err(KMC_E_UNSUPPORTED, "batched systems need at least 4^3 leaves (r >= 3)");
+    }
+    kmc_batch* b = new kmc_batch();
+    b->sys.resize(n);
+    for (int i = 0; i < n; i++) b->sys[i].c = ctxs[i];
+    *out = b;
+    return KMC_OK;
+}
+
+int kmc_batch_destroy(kmc_batch* b) {
+    delete b;
+    return KMC_OK;
+}
+
+int kmc_batch_set_system(kmc_batch* b, int32_t i, int64_t nsteps, int64_t P, int64_t k, const int64_t* d_pids_all,
+                         const double* d_disp_all, const double* d_uniform, int64_t* d_counter, int64_t* d_selected,
+                         int64_t* d_pid, int64_t* d_es, double* d_new_pos, double* d_dE) {
+    if (!b) return set_err(KMC_E_ARG, "null batch");
+    if (i < 0 || i >= (int)b->sys.size()) return set_err(KMC_E_ARG, "system index out of range");
+    if (nsteps < 1 || P < 1 || k < 1 || P * k >= (int64_t)1 << 30) return set_err(KMC_E_ARG, "bad counts");
+    if (!d_pids_all || !d_disp_all || !d_uniform || !d_counter || !d_selected || !d_pid || !d_es || !d_new_pos || !d_dE)
+        return set_err(KMC_E_ARG, "null device pointer");
+    kmc_batch::Sys& y = b->sys[i];
+    y.nsteps = nsteps;
+    y.P = P;
+    y.k = k;
+    y.pids_all = d_pids_all;
+    y.disp_all = d_disp_all;
+    y.uniform = d_uniform;
+    y.counter = d_counter;
+    y.selected = d_selected;
+    y.pid = d_pid;
+    y.es = d_es;
+    y.new_pos = d_new_pos;
+    y.dE = d_dE;
+    y.set = true;
+    b->ready = false;
+    return KMC_OK;
+}
+
+template <typename T>
+static hipError_t upload(DBuf<T>& d, const std::vector<T>& h) {
+    hipError_t e = d.ensure(std::max<size_t>(h.size(), 1));
+    if (e != hipSuccess) return e;
+    return hipMemcpy(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+int kmc_batch_prepare(kmc_batch* b, double kT, int32_t precision) {
+    typedef kmc_batch::Group Group;
+    if (!b) return set_err(KMC_E_ARG, "null batch");
+    if (precision != KMC_PREC_FP64 && precision != KMC_PREC_MIXED) return set_err(KMC_E_ARG, "bad precision");
+    b->kT = kT;
+    b->mixed = precision == KMC_PREC_MIXED;
+    std::vector<MatArgs> mat;
+    std::vector<SetupArgs> setup;
+    std::vector<EvalArgs> nearA;
+    std::vector<CombineArgs> comb;
+    std::vector<SelectArgs> sel;
+    for (Group* g : b->groups) delete g;
+    b->groups.clear();
+    std::vector<std::vector<EvalArgs>> far;
+    std::vector<std::vector<LR2Args>> lr;
+    std::vector<std::vector<SweepArgs>> sweep;
+    std::vector<std::vector<TailArgs>> tail;
+    BatchShape all{0, 0, 0, 0, 0};
+    for (size_t i = 0; i < b->sys.size(); i++) {
+        kmc_batch::Sys& y = b->sys[i];
+        if (!y.set) return set_err(KMC_E_STATE, "batch system " + std::to_string(i) + " has no move stream");
+        kmc_ctx* c = y.c;
+        const int64_t P = y.P, T = y.P * y.k;
+        int rc = ensure_eval(c, T, P);
+        if (rc) return rc;
+        if (uses_lr(c)) {
+            rc = ensure_lr_work(c, P);
+            if (rc) return rc;
+        }
+        if (!step_sized(c, P, T)) return set_err(KMC_E_STATE, "batch: work buffers not sized");
+        const int ne = (int)(T + P);
+        mat.push_back(mat_args(c, y.nsteps, P, y.k, y.pids_all, y.disp_all, y.counter, y.pid, y.es, y.new_pos));
+        setup.push_back(setup_args(c, P, T, y.pid, y.es, y.new_pos, nullptr));
+        nearA.push_back(eval_args(c, ne, c->epos4.p, c->ecell.p, c->eexcl.p, c->UD.p));
+        comb.push_back(combine_args(c, P, T, y.dE, nullptr, kT, c->rates.p));
+        sel.push_back(select_args(c, P, T, y.pid, y.dE, y.nsteps, y.uniform, kT, y.counter, y.selected));
+        all.n++;
+        all.max_ne = std::max(all.max_ne, ne);
+        all.max_P = std::max(all.max_P, (int)P);
+        all.max_T = std::max(all.max_T, (int)T);
+        all.max_cells = std::max(all.max_cells, (int)c->ncells);
+        // group by (order, periodicity)
+        const bool per = c->bc != KMC_BC_FREE_SPACE;
+        size_t gi = 0;
+        while (gi < b->groups.size() && !(b->groups[gi]->L == c->L && b->groups[gi]->per == per)) gi++;
+        if (gi == b->groups.size()) {
+            Group* g = new Group();
+            g->L = c->L;
+            g->per = per;
+            g->fns = c->fns;
+            b->groups.push_back(g);
+            far.emplace_back();
+            lr.emplace_back();
+            sweep.emplace_back();
+            tail.emplace_back();
+        }
+        Group* g = b->groups[gi];
+        far[gi].push_back(eval_args(c, ne, c->epos4.p, c->ecell.p, c->eexcl.p, c->UI.p));
+        sweep[gi].push_back(sweep_rec_args(c));
+        tail[gi].push_back(tail_args(c, y.counter));
+        g->shape.n++;
+        g->shape.max_ne = std::max(g->shape.max_ne, ne);
+        g->shape.max_P = std::max(g->shape.max_P, (int)P);
+        g->shape.max_T = std::max(g->shape.max_T, (int)T);
+        g->shape.max_cells = std::max(g->shape.max_cells, (int)c->ncells);
+        if (uses_lr(c)) {
+            lr[gi].push_back(lr_args(c, P, T));
+            g->lr_shape.n++;
+            g->lr_shape.max_P = std::max(g->lr_shape.max_P, (int)P);
+            g->lr_shape.max_T = std::max(g->lr_shape.max_T, (int)T);
+        }
+    }
+    HIPCHK(upload(b->mat, mat));
+    HIPCHK(upload(b->setup, setup));
+    HIPCHK(upload(b->nearA, nearA));
+    HIPCHK(upload(b->comb, comb));
+    HIPCHK(upload(b->sel, sel));
+    for (size_t gi = 0; gi < b->groups.size(); gi++) {
+        Group* g = b->groups[gi];
+        HIPCHK(upload(g->far, far[gi]));
+        HIPCHK(upload(g->lr, lr[gi]));
+        HIPCHK(upload(g->sweep, sweep[gi]));
+        HIPCHK(upload(g->tail, tail[gi]));
+    }
+    b->all = all;
+    b->ready = true;
+    return KMC_OK;
+}
+
+int kmc_batch_step(kmc_batch* b, void* stream) {
+    if (!b) return set_err(KMC_E_ARG, "null batch");
+    if (!b->ready) return set_err(KMC_E_STATE, "call kmc_batch_prepare before kmc_batch_step");
+    hipStream_t s = (hipStream_t)stream;
+    const BatchShape& a = b->all;
+    const int n = a.n;
+    const int mat_threads = std::max(a.max_T, a.max_P + 1);
+    hipLaunchKernelGGL(kb_materialize, dim3((mat_threads + 255) / 256, 1, n), dim3(256), 0, s, b->mat.p);
+    hipLaunchKernelGGL(kb_setup, dim3((std::max(a.max_T, a.max_P) + 255) / 256, 1, n), dim3(256), 0, s, b->setup.p);
+    const long long near_threads = (long long)a.max_ne * 32;
+    if (b->mixed)
+        hipLaunchKernelGGL(kb_near_cells<true>, dim3((unsigned)((near_threads + 255) / 256), 1, n), dim3(256), 0, s,
+                           b->nearA.p);
+    else
+        hipLaunchKernelGGL(kb_near_cells<false>, dim3((unsigned)((near_threads + 255) / 256), 1, n), dim3(256), 0, s,
+                           b->nearA.p);
+    for (kmc_batch::Group* g : b->groups) {
+        g->fns.bfar(g->far.p, g->shape, s);
+        if (g->lr_shape.n > 0) g->fns.blr(g->lr.p, g->lr_shape, s);
+    }
+    if (b->mixed)
+        hipLaunchKernelGGL(kb_combine<true>, dim3((a.max_T + 63) / 64, 1, n), dim3(64), 0, s, b->comb.p);
+    else
+        hipLaunchKernelGGL(kb_combine<false>, dim3((a.max_T + 63) / 64, 1, n), dim3(64), 0, s, b->comb.p);
+    hipLaunchKernelGGL(kb_select, dim3(1, 1, n), dim3(1024), 0, s, b->sel.p);
+    for (kmc_batch::Group* g : b->groups) {
+        g->fns.bsweep(g->sweep.p, g->shape, g->per, s);
+        g->fns.btail(g->tail.p, g->shape, s);
+    }
+    HIPCHK(hipGetLastError());
+    return KMC_OK;
+}

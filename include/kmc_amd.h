@@ -202,6 +202,29 @@ int kmc_sweep_hop(kmc_ctx* ctx, void* stream, double q, const double* h_old_pos,
  */
 int kmc_time_kernel(kmc_ctx* ctx, void* stream, int32_t which, int32_t reps, int32_t precision, double* h_ms);
 
+/*
+ * Batches of independent systems (no reference counterpart: the reference steps one
+ * system per process).  The step of every system in the batch is one launch per kernel
+ * kind (per expansion order for the order-specialised kernels), the system being
+ * blockIdx.z of a table of per-system kernel arguments.  Systems: initialised contexts,
+ * not ff-only, >= 4^3 leaves.  Per system, the same device move stream as
+ * kmc_materialize_moves + kmc_step (pids_all[nsteps*P], disp_all[nsteps*P*k*3],
+ * uniform[nsteps], counter, selected[nsteps], and pid[P], es[P+1], new_pos[P*k*3],
+ * dE[P*k] work arrays).  kmc_batch_prepare sizes every context's work buffers and
+ * uploads the argument tables (it allocates; call it again after host-side accepts
+ * or anything else that may reallocate a context's tables); kmc_batch_step allocates
+ * nothing and can be captured in a graph.  The accept sweep of a batch always uses
+ * the fused layout with two column pairs per lane, the near field 32 lanes per point.
+ */
+typedef struct kmc_batch kmc_batch;
+int kmc_batch_create(kmc_batch** out, kmc_ctx* const* ctxs, int32_t n);
+int kmc_batch_destroy(kmc_batch* batch);
+int kmc_batch_set_system(kmc_batch* batch, int32_t i, int64_t nsteps, int64_t P, int64_t k, const int64_t* d_pids_all,
+                         const double* d_disp_all, const double* d_uniform, int64_t* d_counter, int64_t* d_selected,
+                         int64_t* d_pid, int64_t* d_es, double* d_new_pos, double* d_dE);
+int kmc_batch_prepare(kmc_batch* batch, double kT, int32_t precision);
+int kmc_batch_step(kmc_batch* batch, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

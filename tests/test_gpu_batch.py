@@ -1,6 +1,7 @@
-"""Batched stepping (coulomb_kmc_amd/batch.py): several independent systems recorded
-into one graph give exactly the results of stepping each system alone (same move
-streams, same tiles)."""
+"""Batched stepping (coulomb_kmc_amd/batch.py): several independent systems of different
+orders and boundary conditions, stepped by the batched kernels (one launch per kernel
+kind, system = blockIdx.z) or by their own steps in one graph, give exactly the results
+of stepping each system alone (same move streams, same tiles)."""
 
 import numpy as np
 import pytest
@@ -33,13 +34,15 @@ def _systems(seed):
     return out
 
 
-def _check_batch(streams):
+def _check_batch(streams, mode="graph"):
     import torch
     from coulomb_kmc_amd.batch import KMCBatch
     steps = 4
     batched = _systems(5)
     single = _systems(5)
-    b = KMCBatch(batched, kT=1.0, precision="mixed", streams=streams)
+    for s in batched + single:   # the batched kernels' shapes
+        s.kmc.set_tiles(near_lanes=27, sweep_block=0, sweep_pairs=2, sweep_split=1)
+    b = KMCBatch(batched, kT=1.0, precision="mixed", streams=streams, mode=mode)
     b.capture(steps_per_graph=2)
     b.replay(steps // 2)
     torch.cuda.synchronize()
@@ -56,19 +59,24 @@ def _check_batch(streams):
         if sb.kmc.boundary_condition.value != "ff-only":
             assert np.array_equal(sb.kmc.get_expansions(), ss.kmc.get_expansions())
     assert all(f == 0 for f in b.check_flags())
+    b.free()
     for s in batched + single:
         s.kmc.free()
 
 
 def test_batch_graph_equals_single_system_steps():
-    _check_batch(1)
+    _check_batch(1, "graph")
+
+
+def test_batched_kernels_equal_single_system_steps():
+    _check_batch(1, "kernels")
 
 
 @pytest.mark.skip(reason="recording the systems on extra torch side streams crashed the host process inside "
                          "graph instantiation (torch.cuda.graph capture_end) on this ROCm/PyTorch stack; "
                          "KMCBatch raises for streams != 1")
 def test_batch_graph_on_side_streams():
-    _check_batch(3)
+    _check_batch(3, "graph")
 
 
 def test_batch_rejects_side_streams():
