@@ -356,6 +356,23 @@ __device__ __forceinline__ void sweep_images(const double* dx, double dy, double
     }
     sweep_group_cols<L, P0, NP, NI>(zr, ir2, u, base, ar, ai);
 }
+template <int L, int P0, int NP, int NI>
+__device__ __forceinline__ void sweep_images_xyz(const double* dx, const double* dy, const double* dz,
+                                                 const double* qe, double (*ar)[L + 1], double (*ai)[L + 1]) {
+    double zr[NI], ir2[NI], base[NI];
+    Cplx u[NI];
+#pragma unroll
+    for (int j = 0; j < NI; j++) {
+        double r2 = fma(dx[j], dx[j], fma(dy[j], dy[j], dz[j] * dz[j]));
+        r2 = qe[j] != 0.0 ? r2 : 1.0;
+        const double ir = rsqrt_fp64(r2);
+        ir2[j] = ir * ir;
+        zr[j] = dz[j] * ir2[j];
+        u[j] = Cplx{dx[j] * ir2[j], -dy[j] * ir2[j]};
+        base[j] = qe[j] * ir;
+    }
+    sweep_group_cols<L, P0, NP, NI>(zr, ir2, u, base, ar, ai);
+}
 
 __device__ __forceinline__ bool near1(int d) { return d < 2 && d > -2; }
 
@@ -377,6 +394,44 @@ __device__ __forceinline__ void sweep_group(const SweepArgs& a, int cell, int z0
     if (a.mode == 0) nsrc = a.rec->valid ? 2 : 0;
     else nsrc = a.src_end - a.src_begin;
 
+    if constexpr (PER && NP == 2) {
+        if (a.mode == 0 && nsrc == 2) {
+            // the move: each image of the old (-q) and the new (+q) position together
+            // (two independent recurrences per column; with one column pair per lane the
+            // extra registers would spill)
+            const MoveRec* rc = a.rec;
+            double dx0[2], dy0[2], dz0[2], sq[2];
+            int dcx[2], dcy[2], dcz[2];
+#pragma unroll
+            for (int s = 0; s < 2; s++) {
+                const double* p = s == 0 ? rc->old_pos : rc->new_pos;
+                const int sc = s == 0 ? rc->old_cell : rc->new_cell;
+                sq[s] = s == 0 ? -1.0 * rc->q : rc->q;
+                dx0[s] = p[0] - cx; dy0[s] = p[1] - cy; dz0[s] = p[2] - cz;
+                dcx[s] = sc % n - tx; dcy[s] = (sc / n) % n - ty; dcz[s] = sc / (n * n) - tz;
+            }
+#pragma unroll 1
+            for (int oz = z0; oz <= z1; oz++) {
+#pragma unroll 1
+                for (int oy = -1; oy <= 1; oy++) {
+#pragma unroll 1
+                    for (int ox = -1; ox <= 1; ox++) {
+                        double dx[2], dy[2], dz[2], qe[2];
+#pragma unroll
+                        for (int s = 0; s < 2; s++) {
+                            dx[s] = fma((double)ox, E, dx0[s]);
+                            dy[s] = fma((double)oy, E, dy0[s]);
+                            dz[s] = fma((double)oz, E, dz0[s]);
+                            const bool nr = near1(dcx[s] + ox * n) && near1(dcy[s] + oy * n) && near1(dcz[s] + oz * n);
+                            qe[s] = nr ? 0.0 : sq[s];
+                        }
+                        sweep_images_xyz<L, P0, NP, 2>(dx, dy, dz, qe, ar, ai);
+                    }
+                }
+            }
+            return;
+        }
+    }
     for (int s = 0; s < nsrc; s++) {
         double sx, sy, sz, sq;
         int sc;
