@@ -40,6 +40,8 @@ EXPORTS = (
     "kmc_propose_with_dats", "kmc_accept", "kmc_step", "kmc_materialize_moves",
     "kmc_check_flags", "kmc_set_tiles", "kmc_sweep_hop", "kmc_time_kernel", "kmc_eval_field",
     "kmc_batch_create", "kmc_batch_destroy", "kmc_batch_set_system", "kmc_batch_prepare", "kmc_batch_step",
+    "kmc_particle_count", "kmc_set_particle_count", "kmc_charge_energies", "kmc_group_energy",
+    "kmc_propose_extract", "kmc_propose_inject", "kmc_extract", "kmc_inject", "kmc_discover_sites",
 )
 
 
@@ -116,6 +118,15 @@ def lib():
         "kmc_set_tiles": (ctypes.c_int, [P, I32, I32, I32, I32]),
         "kmc_sweep_hop": (ctypes.c_int, [P, P, D, P, P]),
         "kmc_time_kernel": (ctypes.c_int, [P, P, I32, I32, I32, P]),
+        "kmc_particle_count": (ctypes.c_int, [P, P]),
+        "kmc_set_particle_count": (ctypes.c_int, [P, I64]),
+        "kmc_charge_energies": (ctypes.c_int, [P, P, I64, P, P]),
+        "kmc_group_energy": (ctypes.c_int, [P, P, I64, I64, P, P, P]),
+        "kmc_propose_extract": (ctypes.c_int, [P, P, I64, I64, P, I32, P]),
+        "kmc_propose_inject": (ctypes.c_int, [P, P, I64, P, P, P]),
+        "kmc_extract": (ctypes.c_int, [P, P, I64, P, P]),
+        "kmc_inject": (ctypes.c_int, [P, P, I64, P, P, P]),
+        "kmc_discover_sites": (ctypes.c_int, [P, I64, P, I64, P, I64, P, D, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

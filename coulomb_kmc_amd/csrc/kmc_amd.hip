@@ -32,6 +32,7 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <unordered_map>
 
 #include "../../include/kmc_amd.h"
 
@@ -2061,6 +2062,256 @@ __global__ void k_eval_combine(int n, const double* ud, const double* ui, int nf
 }
 
 // ============================================================================
+// charge energies, energies of small groups, injection and extraction of charges
+// (coulomb_kmc/kmc_inject_extract.py:82-424)
+// ============================================================================
+// charges by local id as evaluation entries (x, y, z, sign*q) in their leaf, with the id
+// itself excluded from the near field (the old_* arrays of kmc_inject_extract.py:97-113).
+// A bad id raises err bit 1 and leaves an inert zero-charge entry.
+__global__ void k_gather_ids(Geom g, int n, const int64_t* ids, int64_t np, const double4* ppos4,
+                             const int32_t* pcell, double sign, double4* pos4, int32_t* cell, int32_t* excl, int* err) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t id = ids[i];
+    if (id < 0 || id >= np) {
+        atomicOr(err, 1);
+        pos4[i] = make_double4(g.c0, g.c0, g.c0, 0.0);
+        cell[i] = 0;
+        excl[i] = -1;
+        return;
+    }
+    double4 p = ppos4[id];
+    p.w *= sign;
+    pos4[i] = p;
+    cell[i] = pcell[id];
+    excl[i] = (int32_t)id;
+}
+
+// points with charges as evaluation entries, no exclusion
+__global__ void k_pack_charges(Geom g, int n, const double* pos, const double* q, double4* pos4, int32_t* cell,
+                               int32_t* excl) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+    pos4[i] = make_double4(x, y, z, q[i]);
+    const int nc = g.ncps;
+    cell[i] = bin_axis(x, g.E, g.w, nc) + nc * (bin_axis(y, g.E, g.w, nc) + nc * bin_axis(z, g.E, g.w, nc));
+    if (excl) excl[i] = -1;
+}
+
+// eu * (near field + leaf expansion, both already carrying the charge, + q * long-range
+// potential): old_energy_d + old_energy_i + old_energy_l of kmc_inject_extract.py:115-123
+// (near/far/long-range parts: kmc_local.py:374-396, kmc_octal.py:363-375,
+// kmc_full_long_range.py:123-140)
+__global__ void k_charge_energy(int n, const double4* pos4, const double* ud, const double* ui, int nfar,
+                                const double* lr, double eu, double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double v = 0.0;
+    if (ud) {
+        v = ud[i];
+        for (int j = 0; j < nfar; j++) v += ui[(size_t)j * n + i];
+    }
+    if (lr) v = fma(pos4[i].w, lr[i], v);
+    out[i] = eu * v;
+}
+
+// column m of the packed multipole of charge p.w at p about the origin, added to Mb
+// (the vector v(x) of k_eval_lr scaled by the charge: M = sum_i q_i v(x_i))
+__device__ void multipole_column_add(int L, int m, double4 p, double* Mb) {
+    const double r2 = p.x * p.x + p.y * p.y + p.z * p.z;
+    double s = 1.0;
+    Cplx C = {1.0, 0.0};
+    for (int k = 1; k <= m; k++) {
+        s *= -sqrt((2.0 * k - 1.0) / (2.0 * k));
+        C = Cplx{C.re * p.x + C.im * p.y, C.im * p.x - C.re * p.y};
+    }
+    double b2 = 0.0, b1 = s;
+    for (int l = m; l < L; l++) {
+        if (l > m) {
+            const double b = rec_alpha(l, m) * p.z * b1 - rec_beta(l, m) * r2 * b2;
+            b2 = b1;
+            b1 = b;
+        }
+        const int h = hidx(l, m);
+        Mb[2 * h] = fma(p.w * b1, C.re, Mb[2 * h]);
+        Mb[2 * h + 1] = fma(p.w * b1, C.im, Mb[2 * h + 1]);
+    }
+}
+
+// energy of each group of m charges alone in the domain (compute_energy,
+// kmc_inject_extract.py:145-165; the direct sums it delegates to are restated from
+// kmc_direct.py:150-300): every pair once; for '27' / pbc half of every pair and self
+// pair over the 26 neighbour images; for pbc / ff-only 0.5 Mb.(G Mb) of the group's own
+// level-0 multipole.  One workgroup per group; gdat == null: no long-range part.
+__global__ void __launch_bounds__(256) k_group_energy(int m, const double4* __restrict__ pos4, double E, int bc,
+                                                      int L, int nh, const double* gdat, const int64_t* gptr,
+                                                      const int32_t* gidx, double eu, double* out) {
+    extern __shared__ double Mb[];
+    __shared__ double part[256];
+    const double4* p = pos4 + (size_t)blockIdx.x * m;
+    const int tid = threadIdx.x;
+    double acc = 0.0;
+    if (bc != 3) {
+        const bool img = bc == 1 || bc == 2;
+        const long long mm = (long long)m * m;
+        for (long long t = tid; t < mm; t += blockDim.x) {
+            const int i = (int)(t / m), j = (int)(t % m);
+            const double4 a = p[i], b = p[j];
+            const double qq = a.w * b.w;
+            const double dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+            if (j > i) acc += qq / sqrt(dx * dx + dy * dy + dz * dz);
+            if (img) {
+                double s = 0.0;
+                for (int k = 0; k < 27; k++) {
+                    if (k == 13) continue;
+                    const double ex = fma((double)(k % 3 - 1), E, dx);
+                    const double ey = fma((double)((k / 3) % 3 - 1), E, dy);
+                    const double ez = fma((double)(k / 9 - 1), E, dz);
+                    s += 1.0 / sqrt(ex * ex + ey * ey + ez * ez);
+                }
+                acc = fma(0.5 * qq, s, acc);
+            }
+        }
+    }
+    if (gdat) {
+        for (int k = tid; k < 2 * nh; k += blockDim.x) Mb[k] = 0.0;
+        __syncthreads();
+        if (tid < L)
+            for (int i = 0; i < m; i++) multipole_column_add(L, tid, p[i], Mb);
+        __syncthreads();
+        for (int row = tid; row < 2 * nh; row += blockDim.x) {
+            double y = 0.0;
+            for (int64_t k = gptr[row]; k < gptr[row + 1]; k++) y = fma(gdat[k], Mb[gidx[k]], y);
+            acc = fma(0.5 * Mb[row], y, acc);
+        }
+    }
+    part[tid] = acc;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (tid < o) part[tid] += part[tid + o];
+        __syncthreads();
+    }
+    if (tid == 0) out[blockIdx.x] = eu * part[0];
+}
+
+// propose_extract per row (kmc_inject_extract.py:178-213): energy of the row's charges
+// alone (0 without self energy) minus the sum of their energies
+__global__ void k_extract_rows(int nrows, int m, const double* ge, const double* ce, double* out) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    double v = ge ? ge[r] : 0.0;
+    for (int k = 0; k < m; k++) v -= ce[(size_t)r * m + k];
+    out[r] = v;
+}
+
+// propose_inject (kmc_inject_extract.py:216-237): group energy + sum_i eu q_i phi(x_i)
+__global__ void __launch_bounds__(256) k_inject_sum(int n, const double* ce, const double* ge, double* out) {
+    __shared__ double part[256];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += ce[i];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = ge[0] + part[0];
+}
+
+__global__ void k_add_energy(const double* dE, double* energy) { *energy += *dE; }
+
+// extraction from the cell tables and the particle arrays, one charge after another in
+// descending id order: the cell's last occupant takes the freed slot (the store side of
+// kmc_local.py:180-271), then the charge with the highest id takes the freed id (its
+// slot entry renamed), so ids stay 0 .. n-1.  Flags bit 1 when an id is not found once.
+__global__ void k_extract_store(int k, const int64_t* ids_desc, int n0, int* occ, double4* slot_pos, int32_t* slot_id,
+                                int occmax, double4* ppos4, int32_t* pcell, int* flags) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int n = n0;
+    for (int t = 0; t < k; t++) {
+        const int id = (int)ids_desc[t];
+        const int c = pcell[id];
+        const size_t b = (size_t)c * occmax;
+        const int oc = occ[c];
+        int idx = -1, found = 0;
+        for (int s = 0; s < oc; s++)
+            if (slot_id[b + s] == id) { idx = s; found++; }
+        if (found != 1) { atomicOr(flags, 1); return; }
+        slot_id[b + idx] = slot_id[b + oc - 1];
+        slot_pos[b + idx] = slot_pos[b + oc - 1];
+        slot_id[b + oc - 1] = -1;
+        occ[c] = oc - 1;
+        const int last = n - 1;
+        if (id != last) {
+            const int lc = pcell[last];
+            const size_t lb = (size_t)lc * occmax;
+            const int lo = occ[lc];
+            int li = -1, lf = 0;
+            for (int s = 0; s < lo; s++)
+                if (slot_id[lb + s] == last) { li = s; lf++; }
+            if (lf != 1) { atomicOr(flags, 1); return; }
+            slot_id[lb + li] = id;
+            ppos4[id] = ppos4[last];
+            pcell[id] = lc;
+        }
+        n = last;
+    }
+}
+
+// injection: new charges take ids n0 .. n0+k-1 and are appended to their cells
+// (the store side of kmc_local.py:274-359).  Flags bit 2 on a full cell.
+__global__ void k_inject_store(int k, int n0, const double4* src, const int32_t* src_cell, int* occ, double4* slot_pos,
+                               int32_t* slot_id, int occmax, double4* ppos4, int32_t* pcell, int* flags) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int t = 0; t < k; t++) {
+        const int c = src_cell[t];
+        const int o = occ[c];
+        if (o >= occmax) { atomicOr(flags, 2); return; }
+        const size_t sidx = (size_t)c * occmax + o;
+        slot_id[sidx] = n0 + t;
+        slot_pos[sidx] = src[t];
+        occ[c] = o + 1;
+        ppos4[n0 + t] = src[t];
+        pcell[n0 + t] = c;
+    }
+}
+
+// M += sum of per-block partial multipoles
+__global__ void k_reduce_rows_add(int nblk, int nrow, const double* partial, double* M) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrow) return;
+    double s = 0.0;
+    for (int b = 0; b < nblk; b++) s += partial[(size_t)b * nrow + r];
+    M[r] += s;
+}
+
+__global__ void k_zero_i64(int64_t n, int64_t* p) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// DiscoverInjectExtract (coulomb_kmc/kmc_inject_extract.py:439-533): each charge counts
+// itself on every inject site within squared distance tol2 and records the first
+// extract site within tol2 (1-based; 0: none)
+__global__ void k_discover_sites(int64_t n, const double* pos, int ni, const double* isites, int ne,
+                                 const double* esites, double tol2, unsigned long long* iflags, int64_t* eflag) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+    for (int s = 0; s < ni; s++) {
+        const double dx = isites[3 * s] - px, dy = isites[3 * s + 1] - py, dz = isites[3 * s + 2] - pz;
+        if (dx * dx + dy * dy + dz * dz < tol2) atomicAdd(&iflags[s], 1ull);
+    }
+    int64_t e = 0;
+    for (int s = 0; s < ne; s++) {
+        const double dx = esites[3 * s] - px, dy = esites[3 * s + 1] - py, dz = esites[3 * s + 2] - pz;
+        if (dx * dx + dy * dy + dz * dz < tol2) { e = s + 1; break; }
+    }
+    eflag[i] = e;
+}
+
+// ============================================================================
 // order dispatch tables
 // ============================================================================
 typedef void (*SweepLaunch)(SweepArgs, int, hipStream_t, int, int, int);
@@ -2256,6 +2507,11 @@ struct kmc_ctx {
     hipStream_t side = nullptr;
     hipEvent_t ev_fork[2] = {nullptr, nullptr}, ev_join[2] = {nullptr, nullptr};
     bool have_particles = false, have_lr = false, initialised = false;
+    // the host mirrors below lag the device after device-side steps
+    bool mirrors_stale = false;
+    // bumped whenever device buffers are reallocated or the charge count changes
+    // (batches recorded against the old layout refuse to step)
+    int64_t generation = 0;
     OrderFns fns{};
 
     // particles
@@ -2565,6 +2821,161 @@ static int grow_store(kmc_ctx* c, int newmax, hipStream_t s) {
     c->slot_id.p = ni;
     c->slot_id.n = nslots;
     c->occmax = newmax;
+    c->generation++;
+    return KMC_OK;
+}
+
+// grow a device array to hold `want` elements, keeping the first `used`
+template <typename T>
+static int grow_keep(kmc_ctx* c, DBuf<T>& b, size_t used, size_t want, hipStream_t s) {
+    if (want <= b.n && b.p) return KMC_OK;
+    const size_t cap = std::max(want, b.n + b.n / 2);
+    T* np = nullptr;
+    HIPCHK(hipMalloc((void**)&np, cap * sizeof(T)));
+    if (used) HIPCHK(hipMemcpyAsync(np, b.p, used * sizeof(T), hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    b.release();
+    b.p = np;
+    b.n = cap;
+    c->generation++;
+    return KMC_OK;
+}
+
+// host mirrors of positions, charges, leaves and occupancies from the device (after
+// device-side steps moved charges)
+static int refresh_mirrors(kmc_ctx* c, hipStream_t s) {
+    if (!c->mirrors_stale) return KMC_OK;
+    const int64_t n = c->n;
+    std::vector<double4> p4(n);
+    c->h_pos.resize(3 * n);
+    c->h_q.resize(n);
+    c->h_cell.resize(n);
+    c->h_occ.resize(c->ncells);
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipMemcpy(p4.data(), c->ppos4.p, n * sizeof(double4), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(c->h_cell.data(), c->pcell.p, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(c->h_occ.data(), c->occ.p, c->ncells * sizeof(int), hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; i++) {
+        c->h_pos[3 * i] = p4[i].x;
+        c->h_pos[3 * i + 1] = p4[i].y;
+        c->h_pos[3 * i + 2] = p4[i].z;
+        c->h_q[i] = p4[i].w;
+    }
+    c->mirrors_stale = false;
+    return KMC_OK;
+}
+
+// P2L of charges src[0..k) (charge in .w) into every well-separated leaf, with their
+// 27 images for '27' / pbc, in source chunks
+static void sweep_sources(kmc_ctx* c, hipStream_t s, const double4* src, const int32_t* src_cell, int k) {
+    const int chunk = 2048;
+    for (int b = 0; b < k; b += chunk) {
+        SweepArgs sa;
+        sa.g = c->geom();
+        sa.lexp = c->lexp.p;
+        sa.ncells = (int)c->ncells;
+        sa.mode = 1;
+        sa.rec = nullptr;
+        sa.src_pos4 = src;
+        sa.src_cell = src_cell;
+        sa.src_begin = b;
+        sa.src_end = std::min(k, b + chunk);
+        c->fns.sweep(sa, (int)c->ncells, s, c->sweep_block, c->sweep_pairs, c->sweep_split);
+    }
+}
+
+// M += multipole of charges src[0..k), then g = G M and E_lr (kmc_full_long_range.py:270-303)
+static int lr_add_sources(kmc_ctx* c, hipStream_t s, const double4* src, int k) {
+    const int chunk = 1024;
+    const int nblk = (k + chunk - 1) / chunk;
+    const int nrow = 2 * c->nh;
+    DBuf<double> partial;
+    HIPCHK(partial.ensure((size_t)nblk * nrow));
+    c->fns.mp(k, chunk, src, partial.p, nblk, s);
+    hipLaunchKernelGGL(k_reduce_rows_add, dim3((nrow + 255) / 256), dim3(256), 0, s, nblk, nrow, partial.p, c->M.p);
+    hipLaunchKernelGGL(k_lr_refresh, dim3(1), dim3(256), 0, s, nrow, c->gdat.p, c->gptr.p, c->gidx.p, c->M.p,
+                       c->gvec.p, c->elr.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    return KMC_OK;
+}
+
+// eu-scaled energies of n evaluation entries: near field (without the excluded id),
+// leaf expansion and long-range term.  The caller synchronises before the scratch
+// buffers go out of scope.
+static int entry_energies(kmc_ctx* c, hipStream_t s, int n, const double4* pos4, const int32_t* cell,
+                          const int32_t* excl, double* out, DBuf<double>& ud, DBuf<double>& ui, DBuf<double>& lr) {
+    const int nfar = (c->L + 1) / 2;
+    const bool near_far = c->bc != KMC_BC_FF_ONLY;
+    const bool lrt = uses_lr(c);
+    if (near_far) {
+        HIPCHK(ud.ensure(n));
+        HIPCHK(ui.ensure((size_t)n * nfar));
+        launch_near(eval_args(c, n, pos4, cell, excl, ud.p), c->near_lanes, false, s);
+        c->fns.far(eval_args(c, n, pos4, cell, excl, ui.p), s);
+    }
+    if (lrt) {
+        HIPCHK(lr.ensure(n));
+        hipLaunchKernelGGL(k_eval_lr, dim3((n + 255) / 256), dim3(256), 0, s, c->L, n, pos4, c->gvec.p, lr.p);
+    }
+    hipLaunchKernelGGL(k_charge_energy, dim3((n + 255) / 256), dim3(256), 0, s, n, pos4, near_far ? ud.p : nullptr,
+                       near_far ? ui.p : nullptr, nfar, lrt ? lr.p : nullptr, c->eu, out);
+    HIPCHK(hipGetLastError());
+    return KMC_OK;
+}
+
+static void launch_group_energy(kmc_ctx* c, hipStream_t s, int ngroups, int m, const double4* pos4, double* out) {
+    hipLaunchKernelGGL(k_group_energy, dim3(ngroups), dim3(256), (size_t)2 * c->nh * sizeof(double), s, m, pos4, c->E,
+                       c->bc, c->L, c->nh, uses_lr(c) ? c->gdat.p : nullptr, c->gptr.p, c->gidx.p, c->eu, out);
+}
+
+// propose_extract of nrows rows of m charges given by device ids (synchronises)
+static int propose_extract_core(kmc_ctx* c, hipStream_t s, int nrows, int m, const int64_t* d_ids, bool self_energy,
+                                double* d_out) {
+    const int n = nrows * m;
+    DBuf<double4> pos4;
+    DBuf<int32_t> cell, excl;
+    DBuf<double> ce, ge, ud, ui, lr;
+    HIPCHK(pos4.ensure(n));
+    HIPCHK(cell.ensure(n));
+    HIPCHK(excl.ensure(n));
+    HIPCHK(ce.ensure(n));
+    HIPCHK(ge.ensure(nrows));
+    HIPCHK(hipMemsetAsync(c->err.p, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_gather_ids, dim3((n + 255) / 256), dim3(256), 0, s, c->geom(), n, d_ids, c->n, c->ppos4.p,
+                       c->pcell.p, 1.0, pos4.p, cell.p, excl.p, c->err.p);
+    int rc = entry_energies(c, s, n, pos4.p, cell.p, excl.p, ce.p, ud, ui, lr);
+    if (rc) return rc;
+    if (self_energy) launch_group_energy(c, s, nrows, m, pos4.p, ge.p);
+    hipLaunchKernelGGL(k_extract_rows, dim3((nrows + 255) / 256), dim3(256), 0, s, nrows, m,
+                       self_energy ? ge.p : nullptr, ce.p, d_out);
+    HIPCHK(hipGetLastError());
+    int herr = 0;
+    HIPCHK(hipMemcpyAsync(&herr, c->err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (herr) return set_err(KMC_E_ARG, "charge id out of range");
+    return KMC_OK;
+}
+
+// propose_inject of n charges (device positions [n][3], charges [n]) into d_out[0]
+// (synchronises)
+static int propose_inject_core(kmc_ctx* c, hipStream_t s, int n, const double* d_pos, const double* d_q, double* d_out) {
+    DBuf<double4> pos4;
+    DBuf<int32_t> cell, excl;
+    DBuf<double> ce, ge, ud, ui, lr;
+    HIPCHK(pos4.ensure(n));
+    HIPCHK(cell.ensure(n));
+    HIPCHK(excl.ensure(n));
+    HIPCHK(ce.ensure(n));
+    HIPCHK(ge.ensure(1));
+    hipLaunchKernelGGL(k_pack_charges, dim3((n + 255) / 256), dim3(256), 0, s, c->geom(), n, d_pos, d_q, pos4.p, cell.p,
+                       excl.p);
+    int rc = entry_energies(c, s, n, pos4.p, cell.p, excl.p, ce.p, ud, ui, lr);
+    if (rc) return rc;
+    launch_group_energy(c, s, 1, n, pos4.p, ge.p);
+    hipLaunchKernelGGL(k_inject_sum, dim3(1), dim3(256), 0, s, n, ce.p, ge.p, d_out);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
     return KMC_OK;
 }
 
@@ -2726,6 +3137,8 @@ int kmc_set_particles(kmc_ctx* c, const double* h_pos, const double* h_q) {
     HIPCHK(hipMemset(c->gvec.p, 0, 2 * c->nh * sizeof(double)));
     c->have_particles = true;
     c->initialised = false;
+    c->mirrors_stale = false;
+    c->generation++;
     return KMC_OK;
 }
 
@@ -2902,22 +3315,9 @@ int kmc_initialise(kmc_ctx* c, void* stream, double* h_energy) {
         int rc = run_fmm_init(c, s);
         if (rc) return rc;
     } else if (c->bc != KMC_BC_FF_ONLY) {
-        // P2L of every charge into every well-separated leaf, in source chunks
-        const int chunk = 2048;
-        for (int b = 0; b < n; b += chunk) {
-            SweepArgs sa;
-            sa.g = c->geom();
-            sa.lexp = c->lexp.p;
-            sa.ncells = (int)c->ncells;
-            sa.mode = 1;
-            sa.rec = nullptr;
-            sa.src_pos4 = c->ppos4.p;
-            sa.src_cell = c->pcell.p;
-            sa.src_begin = b;
-            sa.src_end = std::min(n, b + chunk);
-            c->fns.sweep(sa, (int)c->ncells, s, c->sweep_block, c->sweep_pairs, c->sweep_split);
-            HIPCHK(hipGetLastError());
-        }
+        // P2L of every charge into every well-separated leaf
+        sweep_sources(c, s, c->ppos4.p, c->pcell.p, n);
+        HIPCHK(hipGetLastError());
     }
     // level-0 multipole M
     {
@@ -3134,6 +3534,8 @@ int kmc_accept(kmc_ctx* c, void* stream, int64_t pid, const double* h_new_pos, d
     if (pid < 0 || pid >= c->n || !h_new_pos) return set_err(KMC_E_ARG, "bad move");
     hipStream_t s = (hipStream_t)stream;
     // host mirrors: leaf of the new position and capacity of its slots
+    int mrc = refresh_mirrors(c, s);
+    if (mrc) return mrc;
     const int n = c->ncps;
     const int newc = bin_axis(h_new_pos[0], c->E, c->w, n) +
                      n * (bin_axis(h_new_pos[1], c->E, c->w, n) + n * bin_axis(h_new_pos[2], c->E, c->w, n));
@@ -3185,6 +3587,7 @@ int kmc_step(kmc_ctx* c, void* stream, int64_t P, int64_t T, const int64_t* d_pi
     hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, s, sa);
     rc = run_accept_tail(c, s, d_counter);
     if (rc) return rc;
+    c->mirrors_stale = true;
     HIPCHK(hipGetLastError());
     return KMC_OK;
 }
@@ -3346,6 +3749,273 @@ int kmc_sweep_hop(kmc_ctx* c, void* stream, double q, const double* h_old, const
     return KMC_OK;
 }
 
+
+int kmc_set_particle_count(kmc_ctx* c, int64_t n) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    if (n < 1 || n >= (int64_t)1 << 31) return set_err(KMC_E_ARG, "n out of range");
+    c->n = n;
+    c->have_particles = false;
+    c->initialised = false;
+    c->generation++;
+    return KMC_OK;
+}
+
+int kmc_particle_count(kmc_ctx* c, int64_t* out) {
+    if (check_ctx(c) || !out) return set_err(KMC_E_ARG, "null argument");
+    *out = c->n;
+    return KMC_OK;
+}
+
+int kmc_charge_energies(kmc_ctx* c, void* stream, int64_t n, const int64_t* d_ids, double* d_out) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    if (!c->initialised) return set_err(KMC_E_STATE, "Run initialise before this call");
+    if (n < 0 || n >= (int64_t)1 << 30) return set_err(KMC_E_ARG, "bad charge count");
+    if (n == 0) return KMC_OK;
+    if (!d_ids || !d_out) return set_err(KMC_E_ARG, "null pointer");
+    hipStream_t s = (hipStream_t)stream;
+    const int ni = (int)n;
+    DBuf<double4> pos4;
+    DBuf<int32_t> cell, excl;
+    DBuf<double> ud, ui, lr;
+    HIPCHK(pos4.ensure(ni));
+    HIPCHK(cell.ensure(ni));
+    HIPCHK(excl.ensure(ni));
+    HIPCHK(hipMemsetAsync(c->err.p, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_gather_ids, dim3((ni + 255) / 256), dim3(256), 0, s, c->geom(), ni, d_ids, c->n, c->ppos4.p,
+                       c->pcell.p, 1.0, pos4.p, cell.p, excl.p, c->err.p);
+    int rc = entry_energies(c, s, ni, pos4.p, cell.p, excl.p, d_out, ud, ui, lr);
+    if (rc) return rc;
+    int herr = 0;
+    HIPCHK(hipMemcpyAsync(&herr, c->err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (herr) return set_err(KMC_E_ARG, "charge id out of range");
+    return KMC_OK;
+}
+
+int kmc_group_energy(kmc_ctx* c, void* stream, int64_t ngroups, int64_t m, const double* d_pos, const double* d_q,
+                     double* d_out) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    if (!c->have_particles) return set_err(KMC_E_STATE, "set particles first (kmc_set_particles)");
+    if ((c->bc == KMC_BC_PBC || c->bc == KMC_BC_FF_ONLY) && !c->have_lr)
+        return set_err(KMC_E_STATE, "set the long-range operator first (pbc / ff-only)");
+    if (ngroups < 0 || m < 0 || ngroups * m >= (int64_t)1 << 30 || ngroups >= (int64_t)1 << 31)
+        return set_err(KMC_E_ARG, "bad group sizes");
+    if (ngroups == 0) return KMC_OK;
+    if (!d_pos || !d_q || !d_out) return set_err(KMC_E_ARG, "null pointer");
+    hipStream_t s = (hipStream_t)stream;
+    const int n = (int)(ngroups * m);
+    DBuf<double4> pos4;
+    DBuf<int32_t> cell;
+    HIPCHK(pos4.ensure(std::max(n, 1)));
+    HIPCHK(cell.ensure(std::max(n, 1)));
+    if (n > 0)
+        hipLaunchKernelGGL(k_pack_charges, dim3((n + 255) / 256), dim3(256), 0, s, c->geom(), n, d_pos, d_q, pos4.p,
+                           cell.p, nullptr);
+    launch_group_energy(c, s, (int)ngroups, (int)m, pos4.p, d_out);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    return KMC_OK;
+}
+
+int kmc_propose_extract(kmc_ctx* c, void* stream, int64_t nrows, int64_t m, const int64_t* d_ids,
+                        int32_t self_energy, double* d_out) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    if (!c->initialised) return set_err(KMC_E_STATE, "Run initialise before this call");
+    if (nrows < 0 || m < 0 || nrows * m >= (int64_t)1 << 30) return set_err(KMC_E_ARG, "bad id array shape");
+    if (nrows == 0) return KMC_OK;
+    if (!d_out || (m > 0 && !d_ids)) return set_err(KMC_E_ARG, "null pointer");
+    hipStream_t s = (hipStream_t)stream;
+    if (m == 0) {
+        hipLaunchKernelGGL(k_extract_rows, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0, s, (int)nrows, 0,
+                           nullptr, nullptr, d_out);
+        HIPCHK(hipStreamSynchronize(s));
+        return KMC_OK;
+    }
+    return propose_extract_core(c, s, (int)nrows, (int)m, d_ids, self_energy != 0, d_out);
+}
+
+int kmc_propose_inject(kmc_ctx* c, void* stream, int64_t n, const double* d_pos, const double* d_q, double* d_out) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    if (!c->initialised) return set_err(KMC_E_STATE, "Run initialise before this call");
+    if (n < 1 || n >= (int64_t)1 << 30) return set_err(KMC_E_ARG, "bad charge count");
+    if (!d_pos || !d_q || !d_out) return set_err(KMC_E_ARG, "null pointer");
+    return propose_inject_core(c, (hipStream_t)stream, (int)n, d_pos, d_q, d_out);
+}
+
+int kmc_extract(kmc_ctx* c, void* stream, int64_t k, const int64_t* h_ids, double* h_energy) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    if (!c->initialised) return set_err(KMC_E_STATE, "Run initialise before this call");
+    if (k < 0) return set_err(KMC_E_ARG, "bad charge count");
+    hipStream_t s = (hipStream_t)stream;
+    if (k > 0) {
+        if (!h_ids) return set_err(KMC_E_ARG, "null pointer");
+        if (k >= c->n) return set_err(KMC_E_ARG, "cannot extract every charge of the system");
+        std::vector<int64_t> desc(h_ids, h_ids + k);
+        std::sort(desc.begin(), desc.end(), [](int64_t a, int64_t b) { return a > b; });
+        for (int64_t t = 0; t < k; t++) {
+            if (desc[t] < 0 || desc[t] >= c->n) return set_err(KMC_E_ARG, "charge id out of range");
+            if (t > 0 && desc[t] == desc[t - 1]) return set_err(KMC_E_ARG, "charge ids to extract must be unique");
+        }
+        int rc = refresh_mirrors(c, s);
+        if (rc) return rc;
+        const int n0 = (int)c->n, kk = (int)k;
+        DBuf<int64_t> d_ids, d_desc;
+        DBuf<double> dE;
+        DBuf<double4> src;
+        DBuf<int32_t> scell, sexcl;
+        HIPCHK(d_ids.ensure(kk));
+        HIPCHK(d_desc.ensure(kk));
+        HIPCHK(dE.ensure(1));
+        HIPCHK(src.ensure(kk));
+        HIPCHK(scell.ensure(kk));
+        HIPCHK(sexcl.ensure(kk));
+        HIPCHK(hipMemcpyAsync(d_ids.p, h_ids, kk * sizeof(int64_t), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d_desc.p, desc.data(), kk * sizeof(int64_t), hipMemcpyHostToDevice, s));
+        // energy change of removing the charges as one group (kmc_inject_extract.py:240-251)
+        rc = propose_extract_core(c, s, 1, kk, d_ids.p, true, dE.p);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_add_energy, dim3(1), dim3(1), 0, s, dE.p, c->energy.p);
+        // the charges with negated sign as sources: leaf expansions
+        // (accept_local_exp in extraction mode, kmc_octal.py:1091-1217) and level-0 multipole
+        hipLaunchKernelGGL(k_gather_ids, dim3((kk + 255) / 256), dim3(256), 0, s, c->geom(), kk, d_ids.p, c->n,
+                           c->ppos4.p, c->pcell.p, -1.0, src.p, scell.p, sexcl.p, c->err.p);
+        if (c->bc != KMC_BC_FF_ONLY) sweep_sources(c, s, src.p, scell.p, kk);
+        if (uses_lr(c)) {
+            rc = lr_add_sources(c, s, src.p, kk);
+            if (rc) return rc;
+        }
+        hipLaunchKernelGGL(k_extract_store, dim3(1), dim3(64), 0, s, kk, d_desc.p, n0, c->occ.p, c->slot_pos.p,
+                           c->slot_id.p, c->occmax, c->ppos4.p, c->pcell.p, c->flags.p);
+        HIPCHK(hipGetLastError());
+        int fl = 0;
+        HIPCHK(hipMemcpyAsync(&fl, c->flags.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (fl) {
+            hipMemsetAsync(c->flags.p, 0, sizeof(int), s);
+            return set_err(KMC_E_STORE, "Index was not unique, this is an error.");
+        }
+        // host mirrors, the same compaction
+        int n = n0;
+        for (int t = 0; t < kk; t++) {
+            const int id = (int)desc[t];
+            c->h_occ[c->h_cell[id]]--;
+            const int last = n - 1;
+            if (id != last) {
+                for (int d = 0; d < 3; d++) c->h_pos[3 * id + d] = c->h_pos[3 * last + d];
+                c->h_q[id] = c->h_q[last];
+                c->h_cell[id] = c->h_cell[last];
+            }
+            n = last;
+        }
+        c->h_pos.resize(3 * (size_t)n);
+        c->h_q.resize(n);
+        c->h_cell.resize(n);
+        c->n = n;
+        c->generation++;
+    }
+    if (h_energy) HIPCHK(hipMemcpy(h_energy, c->energy.p, sizeof(double), hipMemcpyDeviceToHost));
+    return KMC_OK;
+}
+
+int kmc_inject(kmc_ctx* c, void* stream, int64_t k, const double* h_pos, const double* h_q, double* h_energy) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    if (!c->initialised) return set_err(KMC_E_STATE, "Run initialise before this call");
+    if (k < 0 || c->n + k >= (int64_t)1 << 31) return set_err(KMC_E_ARG, "bad charge count");
+    hipStream_t s = (hipStream_t)stream;
+    if (k > 0) {
+        if (!h_pos || !h_q) return set_err(KMC_E_ARG, "null pointer");
+        const double half = 0.5 * c->E;
+        for (int64_t i = 0; i < 3 * k; i++)
+            if (!(h_pos[i] >= -half && h_pos[i] <= half))
+                return set_err(KMC_E_BAD_MOVE, "Injected position is outside simulation domain.");
+        int rc = refresh_mirrors(c, s);
+        if (rc) return rc;
+        const int n0 = (int)c->n, kk = (int)k;
+        // leaves of the new charges and room in their cells
+        std::vector<int32_t> cells(kk);
+        std::unordered_map<int32_t, int> add;
+        int need = 0;
+        for (int i = 0; i < kk; i++) {
+            const int nc = c->ncps;
+            cells[i] = bin_axis(h_pos[3 * i], c->E, c->w, nc) +
+                       nc * (bin_axis(h_pos[3 * i + 1], c->E, c->w, nc) + nc * bin_axis(h_pos[3 * i + 2], c->E, c->w, nc));
+            need = std::max(need, c->h_occ[cells[i]] + ++add[cells[i]]);
+        }
+        DBuf<double> d_pos, d_q, dE;
+        DBuf<double4> src;
+        DBuf<int32_t> scell;
+        HIPCHK(d_pos.ensure(3 * (size_t)kk));
+        HIPCHK(d_q.ensure(kk));
+        HIPCHK(dE.ensure(1));
+        HIPCHK(src.ensure(kk));
+        HIPCHK(scell.ensure(kk));
+        HIPCHK(hipMemcpyAsync(d_pos.p, h_pos, 3 * kk * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d_q.p, h_q, kk * sizeof(double), hipMemcpyHostToDevice, s));
+        // energy change (kmc_inject_extract.py:304-349)
+        rc = propose_inject_core(c, s, kk, d_pos.p, d_q.p, dE.p);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_add_energy, dim3(1), dim3(1), 0, s, dE.p, c->energy.p);
+        if (need > c->occmax) {
+            rc = grow_store(c, need + std::max(4, c->occ_headroom), s);
+            if (rc) return rc;
+        }
+        rc = grow_keep(c, c->ppos4, n0, (size_t)n0 + kk, s);
+        if (rc) return rc;
+        rc = grow_keep(c, c->pcell, n0, (size_t)n0 + kk, s);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_pack_charges, dim3((kk + 255) / 256), dim3(256), 0, s, c->geom(), kk, d_pos.p, d_q.p, src.p,
+                           scell.p, nullptr);
+        hipLaunchKernelGGL(k_inject_store, dim3(1), dim3(64), 0, s, kk, n0, src.p, scell.p, c->occ.p, c->slot_pos.p,
+                           c->slot_id.p, c->occmax, c->ppos4.p, c->pcell.p, c->flags.p);
+        // leaf expansions (accept_local_exp in injection mode, kmc_octal.py:1091-1217) and
+        // level-0 multipole
+        if (c->bc != KMC_BC_FF_ONLY) sweep_sources(c, s, src.p, scell.p, kk);
+        if (uses_lr(c)) {
+            rc = lr_add_sources(c, s, src.p, kk);
+            if (rc) return rc;
+        }
+        HIPCHK(hipGetLastError());
+        int fl = 0;
+        HIPCHK(hipMemcpyAsync(&fl, c->flags.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (fl) {
+            hipMemsetAsync(c->flags.p, 0, sizeof(int), s);
+            return set_err(KMC_E_STORE, "cell store overflow while injecting");
+        }
+        for (int i = 0; i < kk; i++) {
+            for (int d = 0; d < 3; d++) c->h_pos.push_back(h_pos[3 * i + d]);
+            c->h_q.push_back(h_q[i]);
+            c->h_cell.push_back(cells[i]);
+            c->h_occ[cells[i]]++;
+        }
+        c->n = n0 + kk;
+        c->generation++;
+    }
+    if (h_energy) HIPCHK(hipMemcpy(h_energy, c->energy.p, sizeof(double), hipMemcpyDeviceToHost));
+    return KMC_OK;
+}
+
+int kmc_discover_sites(void* stream, int64_t n, const double* d_pos, int64_t n_inject, const double* d_inject_sites,
+                       int64_t n_extract, const double* d_extract_sites, double tol2, int64_t* d_inject_flags,
+                       int64_t* d_extract_flags) {
+    if (n < 0 || n_inject < 0 || n_extract < 0 || n_inject >= (int64_t)1 << 30 || n_extract >= (int64_t)1 << 30)
+        return set_err(KMC_E_ARG, "bad sizes");
+    if ((n > 0 && (!d_pos || !d_extract_flags)) || (n_inject > 0 && (!d_inject_sites || !d_inject_flags)) ||
+        (n_extract > 0 && !d_extract_sites))
+        return set_err(KMC_E_ARG, "null pointer");
+    hipStream_t s = (hipStream_t)stream;
+    if (n_inject > 0)
+        hipLaunchKernelGGL(k_zero_i64, dim3((unsigned)((n_inject + 255) / 256)), dim3(256), 0, s, n_inject,
+                           d_inject_flags);
+    if (n > 0)
+        hipLaunchKernelGGL(k_discover_sites, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, d_pos,
+                           (int)n_inject, d_inject_sites, (int)n_extract, d_extract_sites, tol2,
+                           (unsigned long long*)d_inject_flags, d_extract_flags);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    return KMC_OK;
+}
+
 }  // extern "C"
 
 // ============================================================================
@@ -3360,6 +4030,7 @@ struct kmc_batch {
         const double* uniform = nullptr;
         int64_t* counter = nullptr;
         int64_t* selected = nullptr;
+        int64_t generation = -1;       // context layout the argument tables were built on
         int64_t* pid = nullptr;
         int64_t* es = nullptr;
         double* new_pos = nullptr;
@@ -3533,6 +4204,7 @@ int kmc_batch_prepare(kmc_batch* b, double kT, int32_t precision) {
         HIPCHK(upload(g->tail, tail[gi]));
     }
     b->all = all;
+    for (kmc_batch::Sys& y : b->sys) y.generation = y.c->generation;
     b->ready = true;
     return KMC_OK;
 }
@@ -3540,6 +4212,11 @@ int kmc_batch_prepare(kmc_batch* b, double kT, int32_t precision) {
 int kmc_batch_step(kmc_batch* b, void* stream) {
     if (!b) return set_err(KMC_E_ARG, "null batch");
     if (!b->ready) return set_err(KMC_E_STATE, "call kmc_batch_prepare before kmc_batch_step");
+    for (kmc_batch::Sys& y : b->sys) {
+        if (y.generation != y.c->generation)
+            return set_err(KMC_E_STATE, "a system changed size or buffers since kmc_batch_prepare; prepare again");
+        y.c->mirrors_stale = true;
+    }
     hipStream_t s = (hipStream_t)stream;
     const BatchShape& a = b->all;
     const int n = a.n;

@@ -27,6 +27,7 @@ import numpy as np
 from . import _native
 from .common import BCType, Domain, leaf_of
 from . import lattice
+from .kmc_inject_extract import InjectorExtractor
 
 REAL = ctypes.c_double
 INT64 = ctypes.c_int64
@@ -72,7 +73,7 @@ def default_levels(N):
     return r
 
 
-class KMCFMM:
+class KMCFMM(InjectorExtractor):
     """
     Electrostatic energy differences for kinetic Monte Carlo with the FMM-KMC
     algorithm, on one MI355X.
@@ -107,7 +108,12 @@ class KMCFMM:
         self.domain = domain
         self.positions = positions
         self.charges = charges
+        # the caller's objects (inject accepts them as keys after positions / charges
+        # have been replaced by host arrays)
+        self._user_positions = positions
+        self._user_charges = charges
         pos = _as_array(positions, 3, np.float64)
+        self._N_given = N is not None
         self.N = int(N) if N is not None else int(pos.shape[0])
         self.extent = float(ext[0])
         self.R = int(r) if r is not None else default_levels(self.N)
@@ -175,6 +181,13 @@ class KMCFMM:
     def initialise(self):
         """Initial solve (coulomb_kmc/kmc_fmm.py:947-993); sets ``energy``."""
         t0 = time.time()
+        if not self._N_given:
+            # the charge set may have changed size since construction (as ppmd dats do)
+            self.N = int(self._pos().shape[0])
+        n_ctx = ctypes.c_int64(0)
+        _native.check(self._lib.kmc_particle_count(self._ctx, ctypes.byref(n_ctx)), "kmc_particle_count")
+        if n_ctx.value != self.N:
+            _native.check(self._lib.kmc_set_particle_count(self._ctx, self.N), "kmc_set_particle_count")
         pos = np.ascontiguousarray(self._pos()[:self.N], dtype=np.float64)
         q = np.ascontiguousarray(self._q()[:self.N], dtype=np.float64)
         _native.check(self._lib.kmc_set_particles(self._ctx, pos.ctypes.data_as(ctypes.c_void_p),

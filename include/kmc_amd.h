@@ -176,6 +176,83 @@ int kmc_materialize_moves(kmc_ctx* ctx, void* stream, int64_t nsteps, int64_t P,
  */
 int kmc_eval_field(kmc_ctx* ctx, void* stream, int64_t n, const double* d_points, double* d_out);
 
+/*
+ * Charge count of the context and a new one for the next kmc_set_particles (the count
+ * changes with kmc_extract / kmc_inject; set it before re-initialising with another
+ * set of charges).  kmc_set_particle_count drops the particles: set them and
+ * initialise again.
+ */
+int kmc_particle_count(kmc_ctx* ctx, int64_t* out);
+int kmc_set_particle_count(kmc_ctx* ctx, int64_t n);
+
+/*
+ * Energy of each charge d_ids[n] (device, local ids) with all others, energy_unit
+ * applied: near field without the charge itself, its leaf's local expansion and, for
+ * pbc / ff-only, the long-range term (KMCFMM.get_energy,
+ * coulomb_kmc/kmc_inject_extract.py:82-124, per charge before its row sum; parts from
+ * kmc_local.py:374-396, kmc_octal.py:363-375, kmc_full_long_range.py:123-140).
+ * Synchronises the stream.
+ */
+int kmc_charge_energies(kmc_ctx* ctx, void* stream, int64_t n, const int64_t* d_ids, double* d_out);
+
+/*
+ * Energy of each of ngroups groups of m charges alone in the domain, energy_unit
+ * applied (KMCFMM.compute_energy, coulomb_kmc/kmc_inject_extract.py:145-165, with the
+ * direct sums of coulomb_kmc/kmc_direct.py:150-300: pairs; '27' / pbc: half of every
+ * pair and self pair over the 26 neighbour images; pbc / ff-only: 0.5 M.(G M) of the
+ * group's multipole).  Device d_pos[ngroups*m][3], d_q[ngroups*m] -> d_out[ngroups].
+ */
+int kmc_group_energy(kmc_ctx* ctx, void* stream, int64_t ngroups, int64_t m, const double* d_pos,
+                     const double* d_q, double* d_out);
+
+/*
+ * KMCFMM.propose_extract (coulomb_kmc/kmc_inject_extract.py:178-213): for each row of
+ * d_ids[nrows][m] (device local ids) the energy change of removing the row's charges:
+ * (self_energy ? energy of the row alone : 0) - sum of the row's charge energies.
+ * d_out[nrows].  Synchronises the stream.
+ */
+int kmc_propose_extract(kmc_ctx* ctx, void* stream, int64_t nrows, int64_t m, const int64_t* d_ids,
+                        int32_t self_energy, double* d_out);
+
+/*
+ * KMCFMM.propose_inject (coulomb_kmc/kmc_inject_extract.py:216-237): energy change of
+ * adding n charges d_pos[n][3], d_q[n] (device): their energy alone plus
+ * energy_unit * sum q_i phi(x_i).  d_out[0].  Synchronises the stream.
+ */
+int kmc_propose_inject(kmc_ctx* ctx, void* stream, int64_t n, const double* d_pos, const double* d_q,
+                       double* d_out);
+
+/*
+ * KMCFMM.extract (coulomb_kmc/kmc_inject_extract.py:240-301): removes the k charges
+ * h_ids[k] (host, unique local ids; not every charge): energy += propose_extract of
+ * the ids as one row, the leaf expansions and the level-0 multipole lose the charges
+ * (kmc_octal.py:1091-1217 extraction mode, kmc_full_long_range.py:270-284), the cell
+ * tables drop them (kmc_local.py:180-271).  Ids stay 0 .. n-k-1: taking the ids in
+ * descending order, the charge with the highest id moves into each freed id.
+ * *h_energy (optional) receives the new energy.  Changes the charge count.
+ */
+int kmc_extract(kmc_ctx* ctx, void* stream, int64_t k, const int64_t* h_ids, double* h_energy);
+
+/*
+ * KMCFMM.inject (coulomb_kmc/kmc_inject_extract.py:304-424): adds k charges h_pos[k][3]
+ * (inside the domain), h_q[k] (host) with ids n .. n+k-1: energy += propose_inject,
+ * cell tables (kmc_local.py:274-359), leaf expansions (kmc_octal.py:1091-1217
+ * injection mode) and level-0 multipole (kmc_full_long_range.py:287-303).  Grows the
+ * cell tables and particle arrays when needed.  Changes the charge count.
+ */
+int kmc_inject(kmc_ctx* ctx, void* stream, int64_t k, const double* h_pos, const double* h_q, double* h_energy);
+
+/*
+ * DiscoverInjectExtract (coulomb_kmc/kmc_inject_extract.py:439-533), context free:
+ * d_inject_flags[s] = number of charges d_pos[n][3] within squared distance tol2 of
+ * inject site s; d_extract_flags[i] = 1 + the first extract site within tol2 of charge
+ * i, 0 if none (all device arrays).  The reference compares squared distances with
+ * 1e-5.  Synchronises the stream.
+ */
+int kmc_discover_sites(void* stream, int64_t n, const double* d_pos, int64_t n_inject,
+                       const double* d_inject_sites, int64_t n_extract, const double* d_extract_sites,
+                       double tol2, int64_t* d_inject_flags, int64_t* d_extract_flags);
+
 /* Flags raised by device-side steps (store overflow, bad moves); cleared by the call. */
 int kmc_check_flags(kmc_ctx* ctx, void* stream, int32_t* h_flags);
 

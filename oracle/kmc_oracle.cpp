@@ -632,6 +632,79 @@ struct Oracle {
         return e;
     }
 
+    // ---------------------------------------------------------------- inject / extract
+    // kmc_inject_extract.py:82-124 (get_energy, per charge before the row sum):
+    // old_energy_d (kmc_local.py:374-396, own id excluded), old_energy_i
+    // (kmc_octal.py:363-375) and old_energy_l = q * local_eval(lrc(M), x)
+    // (kmc_full_long_range.py:123-140, 901-960), whose bilinear form is the one
+    // lr_apply_dot restates with the factor 1/2 removed.  eu applied.
+    double charge_energy(int64_t id) const {
+        const double* p = &pos[3 * id];
+        double e = 0.0;
+        if (bc != BC_FF) {
+            const int64_t lc = store_cell(cell[id]);
+            e += direct(p, q[id], lc, id) + indirect(p, q[id], lc);
+        }
+        if ((bc == BC_PBC || bc == BC_FF) && have_lr) e += lr_point(p, q[id]);
+        return e * eu;
+    }
+    double lr_point(const double* p, double qq) const {
+        std::vector<double> ex(2 * L * L, 0.0);
+        double r, th, ph;
+        spherical(p[0], p[1], p[2], &r, &th, &ph);
+        dot_vec(B, qq, r, th, ph, ex.data());
+        return 2.0 * lr_apply_dot(M.data(), ex.data());
+    }
+    // q * phi(x) at a point that holds no charge of the system (eval_field pieces,
+    // kmc_fmm.py:544-567), eu applied
+    double point_energy(const double* p, double qq) const {
+        double e = 0.0;
+        if (bc != BC_FF) {
+            int64_t c3[3];
+            bin(p, c3);
+            const int64_t lc = store_cell(lin(c3));
+            e += direct(p, qq, lc, -1) + indirect(p, qq, lc);
+        }
+        if ((bc == BC_PBC || bc == BC_FF) && have_lr) e += lr_point(p, qq);
+        return e * eu;
+    }
+    // compute_energy (kmc_inject_extract.py:145-165) with the direct sums of
+    // kmc_direct.py:150-300: nearest_direct (pairs once, plus half of every pair and
+    // self pair over the 26 neighbour images for '27' / pbc) and pbc_direct (half the
+    // linear operator and dipole correction applied to the group's own M and E)
+    double group_energy(int64_t m, const double* gp, const double* gq) const {
+        double e = 0.0;
+        if (bc != BC_FF) {
+            for (int64_t i = 0; i < m; i++) {
+                for (int64_t j = i + 1; j < m; j++) {
+                    const double dx = gp[3 * i] - gp[3 * j], dy = gp[3 * i + 1] - gp[3 * j + 1],
+                                 dz = gp[3 * i + 2] - gp[3 * j + 2];
+                    e += gq[i] * gq[j] / std::sqrt(dx * dx + dy * dy + dz * dz);
+                }
+                if (bc == BC_27 || bc == BC_PBC)
+                    for (int64_t j = 0; j < m; j++)
+                        for (int k = 0; k < 27; k++) {
+                            if (OFF[k][0] == 0 && OFF[k][1] == 0 && OFF[k][2] == 0) continue;
+                            const double dx = gp[3 * j] - gp[3 * i] + OFF[k][0] * E;
+                            const double dy = gp[3 * j + 1] - gp[3 * i + 1] + OFF[k][1] * E;
+                            const double dz = gp[3 * j + 2] - gp[3 * i + 2] + OFF[k][2] * E;
+                            e += 0.5 * gq[i] * gq[j] / std::sqrt(dx * dx + dy * dy + dz * dz);
+                        }
+            }
+        }
+        if ((bc == BC_PBC || bc == BC_FF) && have_lr) {
+            std::vector<double> gm(2 * L * L, 0.0), ge(2 * L * L, 0.0);
+            for (int64_t i = 0; i < m; i++) {
+                double r, th, ph;
+                spherical(gp[3 * i], gp[3 * i + 1], gp[3 * i + 2], &r, &th, &ph);
+                p2m(B, gq[i], r, th, ph, gm.data());
+                dot_vec(B, gq[i], r, th, ph, ge.data());
+            }
+            e += lr_apply_dot(gm.data(), ge.data());
+        }
+        return e * eu;
+    }
+
     // kmc_fmm.py:811-912
     int accept(int64_t gid, const double* np_, double* dE_out) {
         const double op[3] = {pos[3 * gid], pos[3 * gid + 1], pos[3 * gid + 2]};
@@ -807,6 +880,21 @@ void orc_accept_sweep_only(void* h, double q, const double* np_, const double* o
     o->bin(np_, n3);
     o->bin(op, o3);
     o->accept_sweep(q, np_, op, o->lin(n3), o->lin(o3));
+}
+
+// per-charge energies (get_energy before its row sum), eu applied
+void orc_charge_energy(void* h, int64_t n, const int64_t* ids, double* out) {
+    Oracle* o = (Oracle*)h;
+    for (int64_t i = 0; i < n; i++) out[i] = o->charge_energy(ids[i]);
+}
+// q_i phi(x_i) at points without charges of the system, eu applied
+void orc_point_energy(void* h, int64_t n, const double* p, const double* qq, double* out) {
+    Oracle* o = (Oracle*)h;
+    for (int64_t i = 0; i < n; i++) out[i] = o->point_energy(&p[3 * i], qq[i]);
+}
+// energy of m charges alone in the domain, eu applied
+double orc_group_energy(void* h, int64_t m, const double* p, const double* qq) {
+    return ((Oracle*)h)->group_energy(m, p, qq);
 }
 
 // basis primitives in the reference layout (for closed-form checks)

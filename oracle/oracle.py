@@ -68,6 +68,10 @@ def lib():
         L.orc_p2l.argtypes = [I32, D, P, P]
         L.orc_p2m.argtypes = [I32, D, P, P]
         L.orc_dot_vec.argtypes = [I32, D, P, P]
+        L.orc_charge_energy.argtypes = [P, I64, P, P]
+        L.orc_point_energy.argtypes = [P, I64, P, P, P]
+        L.orc_group_energy.restype = D
+        L.orc_group_energy.argtypes = [P, I64, P, P]
         _lib = L
     return _lib
 
@@ -239,6 +243,42 @@ class Oracle:
         if err:
             raise RuntimeError("accept failed ({})".format(err))
         return float(dE[0])
+
+    # inject / extract (coulomb_kmc/kmc_inject_extract.py:82-237)
+    def charge_energy(self, ids):
+        """Per-charge energies (get_energy before the row sum), eu applied."""
+        ids = _i64(ids).ravel()
+        out = np.zeros(ids.size)
+        lib().orc_charge_energy(self.h, ids.size, _p(ids), _p(out))
+        return out
+
+    def point_energy(self, pos, q):
+        """q_i phi(x_i) at points holding no charge of the system, eu applied."""
+        pos = _f64(pos).reshape(-1, 3)
+        q = _f64(q).ravel()
+        out = np.zeros(q.size)
+        lib().orc_point_energy(self.h, q.size, _p(pos), _p(q), _p(out))
+        return out
+
+    def group_energy(self, pos, q):
+        """compute_energy: the charges alone in the domain, eu applied."""
+        pos = _f64(pos).reshape(-1, 3)
+        q = _f64(q).ravel()
+        return float(lib().orc_group_energy(self.h, q.size, _p(pos), _p(q)))
+
+    def propose_extract(self, ids, self_energy=True):
+        """kmc_inject_extract.py:178-213 for rows of ids."""
+        ids = np.atleast_2d(_i64(ids))
+        cur = self.positions()
+        out = np.zeros(ids.shape[0])
+        for r, row in enumerate(ids):
+            ge = self.group_energy(cur[row], self.q0[row]) if self_energy else 0.0
+            out[r] = ge - self.charge_energy(row).sum()
+        return out
+
+    def propose_inject(self, pos, q):
+        """kmc_inject_extract.py:216-237."""
+        return self.group_energy(pos, q) + float(self.point_energy(pos, q).sum())
 
     def accept_sweep_only(self, q, new_pos, old_pos):
         lib().orc_accept_sweep_only(self.h, float(q), _p(_f64(new_pos)), _p(_f64(old_pos)))
