@@ -42,6 +42,7 @@ EXPORTS = (
     "kmc_batch_create", "kmc_batch_destroy", "kmc_batch_set_system", "kmc_batch_prepare", "kmc_batch_step",
     "kmc_particle_count", "kmc_set_particle_count", "kmc_charge_energies", "kmc_group_energy",
     "kmc_propose_extract", "kmc_propose_inject", "kmc_extract", "kmc_inject", "kmc_discover_sites",
+    "kmc_lattice_step",
 )
 
 
@@ -127,6 +128,7 @@ def lib():
         "kmc_extract": (ctypes.c_int, [P, P, I64, P, P]),
         "kmc_inject": (ctypes.c_int, [P, P, I64, P, P, P]),
         "kmc_discover_sites": (ctypes.c_int, [P, I64, P, I64, P, I64, P, D, P, P]),
+        "kmc_lattice_step": (ctypes.c_int, [P, P, I64, P, D, I32, P, P, P, I64, P, D, I32, P, P, I32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

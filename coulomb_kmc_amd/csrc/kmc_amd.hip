@@ -931,8 +931,10 @@ struct CombineArgs {
     const int64_t* rate_loc;     // output index, or null (identity)
     double* out;
     double kT;
-    double* rates;               // Metropolis rates min(1, exp(-dE/kT)) per proposal, or null
+    double* rates;               // rates per proposal (rate_rule), or null
     double* bsum;                // their sums over blocks of 64 proposals
+    const int64_t* mask;         // proposals with mask <= 0 get rate 0, or null
+    int rate_rule;               // 0: min(1, exp(-dE/kT)) (Metropolis), 1: exp(-dE/kT)
 };
 
 __device__ __forceinline__ double sum_parts(const double* v, int nparts, size_t stride, size_t i) {
@@ -941,9 +943,10 @@ __device__ __forceinline__ double sum_parts(const double* v, int nparts, size_t 
     return s;
 }
 
-__device__ __forceinline__ double rate_of(double de, double kT) {
+__device__ __forceinline__ double rate_of(double de, double kT, int rule) {
     if (!(de == de) || isinf(de)) return 0.0;
     const double w = exp(-de / kT);
+    if (rule == 1) return isinf(w) ? 0.0 : w;
     return w < 1.0 ? w : 1.0;
 }
 
@@ -995,7 +998,7 @@ __device__ __forceinline__ void combine_body(const CombineArgs& a) {
     if (t < a.T) {
         const double de = combine_one<MIXED>(a, t);
         if (a.rates) {
-            rate = rate_of(de, a.kT);
+            rate = (a.mask && a.mask[t] <= 0) ? 0.0 : rate_of(de, a.kT, a.rate_rule);
             a.rates[t] = rate;
         }
     }
@@ -2312,6 +2315,84 @@ __global__ void k_discover_sites(int64_t n, const double* pos, int ni, const dou
 }
 
 // ============================================================================
+// lattice bookkeeping on the device (Algorithm 2 of the reference's examples:
+// docs/source/example_bookkeeping.rst, examples/propose_with_dats_example.py:217-312)
+// ============================================================================
+// M proposals per charge: position + offset, wrapped into the domain as the example's
+// prop_pos kernel does (below -E/2: +E, above E/2: -E), mask 1.  wrap == 0: no
+// wrapping, proposals outside the domain get mask 0.
+__global__ void k_lattice_proposals(int n, int M, double E, int wrap, const double4* ppos4, const double* offsets,
+                                    double* new_pos, int64_t* masks) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * M) return;
+    const int i = t / M, m = t - i * M;
+    const double4 p = ppos4[i];
+    const double lo = -0.5 * E, hi = 0.5 * E;
+    double v[3] = {p.x + offsets[3 * m], p.y + offsets[3 * m + 1], p.z + offsets[3 * m + 2]};
+    int64_t ok = 1;
+    for (int d = 0; d < 3; d++) {
+        if (wrap) {
+            if (v[d] < lo) v[d] += E;
+            if (v[d] > hi) v[d] -= E;
+        } else if (v[d] < lo || v[d] > hi) {
+            ok = 0;
+        }
+        new_pos[3 * (size_t)t + d] = v[d];
+    }
+    masks[t] = ok;
+}
+
+// overlap masking (the example's exclude PairLoop): proposal t of charge i gets mask 0
+// when another charge lies within squared distance tol2 of it.  Periodic (wrap):
+// minimum-image distance, as the PairLoop sees neighbours across the boundary.
+// The charges are found in the 27 leaves around the proposal's leaf (tol2 <= w^2).
+__global__ void k_overlap_mask(Geom g, int n, int M, double tol2, int wrap, const double* new_pos, const int* occ,
+                               const double4* slot_pos, const int32_t* slot_id, int occmax, int64_t* masks) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * M) return;
+    if (masks[t] <= 0) return;
+    const int i = t / M;
+    const double x = new_pos[3 * (size_t)t], y = new_pos[3 * (size_t)t + 1], z = new_pos[3 * (size_t)t + 2];
+    const int nc = g.ncps;
+    const int cx = bin_axis(x, g.E, g.w, nc), cy = bin_axis(y, g.E, g.w, nc), cz = bin_axis(z, g.E, g.w, nc);
+    for (int k = 0; k < 27; k++) {
+        int ox, oy, oz;
+        stencil(k, ox, oy, oz);
+        int jx = cx + ox, jy = cy + oy, jz = cz + oz;
+        if (wrap) {
+            jx = (jx + nc) % nc;
+            jy = (jy + nc) % nc;
+            jz = (jz + nc) % nc;
+        } else if (jx < 0 || jy < 0 || jz < 0 || jx >= nc || jy >= nc || jz >= nc) {
+            continue;
+        }
+        const int jc = jx + nc * (jy + nc * jz);
+        const size_t b = (size_t)jc * occmax;
+        const int o = occ[jc];
+        for (int s = 0; s < o; s++) {
+            if (slot_id[b + s] == i) continue;
+            const double4 q = slot_pos[b + s];
+            double dx = q.x - x, dy = q.y - y, dz = q.z - z;
+            if (wrap) {
+                dx -= g.E * rint(dx / g.E);
+                dy -= g.E * rint(dy / g.E);
+                dz -= g.E * rint(dz / g.E);
+            }
+            if (dx * dx + dy * dy + dz * dz < tol2) {
+                masks[t] = 0;
+                return;
+            }
+        }
+    }
+}
+
+__global__ void k_lattice_index(int n, int M, int64_t* pid, int64_t* es) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) pid[i] = i;
+    if (i <= n) es[i] = (int64_t)i * M;
+}
+
+// ============================================================================
 // order dispatch tables
 // ============================================================================
 typedef void (*SweepLaunch)(SweepArgs, int, hipStream_t, int, int, int);
@@ -2544,6 +2625,11 @@ struct kmc_ctx {
     // propose_with_dats work buffers
     DBuf<int64_t> d_cnt, d_has, d_slot, d_off, d_pid, d_es, d_rate, d_tot;
     DBuf<double> d_npos, d_nq;
+    // lattice steps: every charge proposes the same M offsets
+    DBuf<int64_t> lat_pid, lat_es;
+    DBuf<double> lat_off;
+    std::vector<double> h_lat_off;
+    int64_t lat_n = -1;
 
     Geom geom() const { return Geom{E, w, -0.5 * E + 0.5 * w, ncps, bc}; }
 };
@@ -2652,6 +2738,8 @@ static CombineArgs combine_args(kmc_ctx* c, int64_t P, int64_t T, double* d_out,
     ca.kT = kT;
     ca.rates = rates;
     ca.bsum = rates ? c->bsum.p : nullptr;
+    ca.mask = nullptr;
+    ca.rate_rule = 0;
     return ca;
 }
 
@@ -2740,7 +2828,8 @@ static int join_side(kmc_ctx* c, hipStream_t s, int i) {
 }
 
 static int run_propose_core(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, double* d_out, const int64_t* rate_loc,
-                            bool mixed, double kT = 0.0, double* rates = nullptr) {
+                            bool mixed, double kT = 0.0, double* rates = nullptr, const int64_t* mask = nullptr,
+                            int rate_rule = 0) {
     const int ne = (int)(T + P);
     c->last_P = P;
     c->last_T = T;
@@ -2762,7 +2851,9 @@ static int run_propose_core(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, dou
         int rc = join_side(c, s, 0);
         if (rc) return rc;
     }
-    const CombineArgs ca = combine_args(c, P, T, d_out, rate_loc, kT, rates);
+    CombineArgs ca = combine_args(c, P, T, d_out, rate_loc, kT, rates);
+    ca.mask = mask;
+    ca.rate_rule = rate_rule;
     if (T > 0) {
         if (mixed) hipLaunchKernelGGL(k_combine<true>, dim3((T + 63) / 64), dim3(64), 0, s, ca);
         else hipLaunchKernelGGL(k_combine<false>, dim3((T + 63) / 64), dim3(64), 0, s, ca);
@@ -3992,6 +4083,61 @@ int kmc_inject(kmc_ctx* c, void* stream, int64_t k, const double* h_pos, const d
         c->generation++;
     }
     if (h_energy) HIPCHK(hipMemcpy(h_energy, c->energy.p, sizeof(double), hipMemcpyDeviceToHost));
+    return KMC_OK;
+}
+
+int kmc_lattice_step(kmc_ctx* c, void* stream, int64_t M, const double* h_offsets, double tol2, int32_t wrap,
+                     double* d_new_pos, int64_t* d_masks, double* d_dE, int64_t nsteps, const double* d_uniform,
+                     double kT, int32_t rate_rule, int64_t* d_counter, int64_t* d_selected, int32_t precision) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    if (!c->initialised) return set_err(KMC_E_STATE, "Run initialise before this call");
+    const int64_t P = c->n, T = c->n * M;
+    if (M < 1 || M > 4096 || T >= (int64_t)1 << 30) return set_err(KMC_E_ARG, "bad number of offsets");
+    if (!h_offsets || !d_new_pos || !d_masks || !d_dE || !d_uniform || !d_counter || !d_selected)
+        return set_err(KMC_E_ARG, "null pointer");
+    if (!(tol2 >= 0.0) || tol2 > c->w * c->w) return set_err(KMC_E_ARG, "overlap tolerance must lie in [0, leaf width^2]");
+    if (nsteps < 1 || !(kT > 0.0) || (rate_rule != 0 && rate_rule != 1))
+        return set_err(KMC_E_ARG, "bad nsteps, kT or rate rule");
+    for (int64_t k = 0; k < 3 * M; k++)
+        if (!(fabs(h_offsets[k]) <= c->max_move))
+            return set_err(KMC_E_BAD_MOVE, "offset component beyond max_move");
+    hipStream_t s = (hipStream_t)stream;
+    const int n = (int)P, nt = (int)T;
+    // sizing (allocates: the first call for a charge count and offset set runs eagerly,
+    // later calls may be captured)
+    const bool same_off = c->h_lat_off.size() == (size_t)(3 * M) &&
+                          std::equal(c->h_lat_off.begin(), c->h_lat_off.end(), h_offsets);
+    if (!step_sized(c, P, T) || c->lat_n != P || !same_off) {
+        int rc = ensure_eval(c, T, P);
+        if (rc) return rc;
+        if (uses_lr(c)) {
+            rc = ensure_lr_work(c, P);
+            if (rc) return rc;
+        }
+        HIPCHK(c->lat_pid.ensure(P));
+        HIPCHK(c->lat_es.ensure(P + 1));
+        HIPCHK(c->lat_off.ensure(3 * M));
+        hipLaunchKernelGGL(k_lattice_index, dim3((n + 256) / 256), dim3(256), 0, s, n, (int)M, c->lat_pid.p,
+                           c->lat_es.p);
+        HIPCHK(hipMemcpyAsync(c->lat_off.p, h_offsets, 3 * M * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        c->h_lat_off.assign(h_offsets, h_offsets + 3 * M);
+        c->lat_n = P;
+    }
+    hipLaunchKernelGGL(k_lattice_proposals, dim3((nt + 255) / 256), dim3(256), 0, s, n, (int)M, c->E, wrap,
+                       c->ppos4.p, c->lat_off.p, d_new_pos, d_masks);
+    hipLaunchKernelGGL(k_overlap_mask, dim3((nt + 255) / 256), dim3(256), 0, s, c->geom(), n, (int)M, tol2, wrap,
+                       d_new_pos, c->occ.p, c->slot_pos.p, c->slot_id.p, c->occmax, d_masks);
+    int rc = launch_setup(c, s, P, T, c->lat_pid.p, c->lat_es.p, d_new_pos, nullptr);
+    if (rc) return rc;
+    rc = run_propose_core(c, s, P, T, d_dE, nullptr, precision == KMC_PREC_MIXED, kT, c->rates.p, d_masks, rate_rule);
+    if (rc) return rc;
+    const SelectArgs sa = select_args(c, P, T, c->lat_pid.p, d_dE, nsteps, d_uniform, kT, d_counter, d_selected);
+    hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, s, sa);
+    rc = run_accept_tail(c, s, d_counter);
+    if (rc) return rc;
+    c->mirrors_stale = true;
+    HIPCHK(hipGetLastError());
     return KMC_OK;
 }
 

@@ -253,6 +253,24 @@ int kmc_discover_sites(void* stream, int64_t n, const double* d_pos, int64_t n_i
                        const double* d_inject_sites, int64_t n_extract, const double* d_extract_sites,
                        double tol2, int64_t* d_inject_flags, int64_t* d_extract_flags);
 
+/*
+ * One lattice KMC step on the device, Algorithm 2 of the reference's examples
+ * (docs/source/example_bookkeeping.rst; examples/propose_with_dats_example.py:217-450):
+ * every charge proposes position + h_offsets[m] (host [M][3], each component within
+ * max_move), wrapped into the domain (wrap != 0; else proposals outside the domain
+ * get mask 0) -> d_new_pos[N*M][3], d_masks[N*M] = 1; proposals within squared
+ * distance tol2 (<= leaf width^2; the example's TOL is 0.01) of another charge get
+ * mask 0 (minimum image when wrapping); all N*M proposals are evaluated -> d_dE;
+ * rates exp(-dE/kT) (rate_rule 1, the example's rate kernel) or min(1, exp(-dE/kT))
+ * (rate_rule 0), 0 where masked; selection and accept as kmc_step, with proposal index
+ * i*M + m written to d_selected[*d_counter % nsteps].  The first call for a charge count
+ * and offset set sizes the work buffers (eager); later calls allocate nothing and may be
+ * captured in a graph.  No host synchronisation.
+ */
+int kmc_lattice_step(kmc_ctx* ctx, void* stream, int64_t M, const double* h_offsets, double tol2, int32_t wrap,
+                     double* d_new_pos, int64_t* d_masks, double* d_dE, int64_t nsteps, const double* d_uniform,
+                     double kT, int32_t rate_rule, int64_t* d_counter, int64_t* d_selected, int32_t precision);
+
 /* Flags raised by device-side steps (store overflow, bad moves); cleared by the call. */
 int kmc_check_flags(kmc_ctx* ctx, void* stream, int32_t* h_flags);
 
