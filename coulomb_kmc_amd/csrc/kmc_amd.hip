@@ -610,13 +610,27 @@ __device__ __forceinline__ void sweep_fused_body(const SweepArgs& a) {
         if ((lane >> 5) == half) sweep_stage_dispatch<L, NP, false, 0>(stage + (lane & 31) * ST, group, ar, ai);
         __syncthreads();
         double2* base = a.lexp + (size_t)(cell0 + 32 * half) * NH;
-        for (int e = tid; e < 32 * NH; e += nthr) {
-            const int j = e / NH, h = e - j * NH;
-            const double2 sv = stage[j * ST + h];
-            double2 v = base[e];
-            v.x += sv.x;
-            v.y += sv.y;
-            base[e] = v;
+        // eight loads in flight per thread before their adds and stores
+        constexpr int U = 8;
+#pragma unroll 1
+        for (int e0 = tid; e0 < 32 * NH; e0 += nthr * U) {
+            double2 v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int e = e0 + u * nthr;
+                if (e < 32 * NH) v[u] = base[e];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int e = e0 + u * nthr;
+                if (e < 32 * NH) {
+                    const int j = e / NH, h = e - j * NH;
+                    const double2 sv = stage[j * ST + h];
+                    v[u].x += sv.x;
+                    v[u].y += sv.y;
+                    base[e] = v[u];
+                }
+            }
         }
     }
 }
@@ -833,6 +847,225 @@ __global__ void __launch_bounds__(256) kb_near_cells(const EvalArgs* __restrict_
     near_cells_body<MIXED>(args[blockIdx.z]);
 }
 
+// near field of the propose paths, grouped by moving charge: one wave per moving
+// charge p takes its old position (entry T + p, its own id excluded) and its proposals
+// (entries es[p] .. es[p+1]-1) together, up to 16 points per pass.  The leaves within
+// one of the points' leaves form a box around the old position's leaf (at most 5 per
+// side); the wave lists the box's slots leaf by leaf (prefix sums in LDS) and each lane
+// reads its share of them once and adds each to every point whose 27-leaf stencil holds
+// its leaf.  Per pair the image shift and the arithmetic are those of the per-point
+// kernels (only the order of the sums differs).  A pass whose points span more than
+// 3 leaves along an axis is evaluated point by point over each point's own 27 leaves.
+__device__ __forceinline__ double readlane_d(double v, int k) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffll), k);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+__device__ __forceinline__ int floordiv(int a, int n) { return a >= 0 ? a / n : -((-a + n - 1) / n); }
+
+constexpr int NEAR_GROUP_POINTS = 16;
+constexpr int NEAR_BOX = 5;
+
+template <bool MIXED>
+__global__ void __launch_bounds__(256) k_near_groups(EvalArgs a, int P, int T, const int64_t* __restrict__ es) {
+    constexpr int KP = NEAR_GROUP_POINTS;
+    constexpr int NB3 = NEAR_BOX * NEAR_BOX * NEAR_BOX;
+    constexpr int SPL = 6;    // slots per lane per round
+    __shared__ int s_start[4][NB3 + 1];
+    __shared__ int s_cell[4][NB3];
+    __shared__ int s_shift[4][NB3];
+    __shared__ double s_pu[4][KP][3];    // point positions in the box frame
+    __shared__ int4 s_pk[4][KP];         // first member leaf of the box per axis, excluded id
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int p = blockIdx.x * 4 + wv;
+    if (p >= P) return;
+    const int n = a.g.ncps;
+    const double E = a.g.E;
+    const bool periodic = a.g.bc != KMC_BC_FREE_SPACE;
+    const int t0 = (int)es[p];
+    const int K = 1 + (int)es[p + 1] - t0;
+    const int c0 = a.cell[T + p];
+    const int l0x = c0 % n, l0y = (c0 / n) % n, l0z = c0 / (n * n);
+    for (int pass = 0; pass < K; pass += KP) {
+        const int kn = min(KP, K - pass);
+        // point of this lane (lanes < kn): position, leaf offset a from the old leaf,
+        // unwrap count u, excluded id
+        const int kk = pass + lane;
+        const int e = lane < kn ? (kk == 0 ? T + p : t0 + kk - 1) : T + p;
+        const double4 pt = a.pos4[e];
+        const int ce = a.cell[e];
+        const int ex = a.excl[e];
+        int ax = ce % n - l0x, ay = (ce / n) % n - l0y, az = ce / (n * n) - l0z;
+        if (periodic) {
+            if (2 * ax > n) ax -= n; else if (2 * ax < -n) ax += n;
+            if (2 * ay > n) ay -= n; else if (2 * ay < -n) ay += n;
+            if (2 * az > n) az -= n; else if (2 * az < -n) az += n;
+        }
+        const int ux = floordiv(l0x + ax, n), uy = floordiv(l0y + ay, n), uz = floordiv(l0z + az, n);
+        int mnx = lane < kn ? ax : 1 << 20, mny = lane < kn ? ay : 1 << 20, mnz = lane < kn ? az : 1 << 20;
+        int mxx = lane < kn ? ax : -(1 << 20), mxy = lane < kn ? ay : -(1 << 20), mxz = lane < kn ? az : -(1 << 20);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            mnx = min(mnx, __shfl_xor(mnx, o)); mny = min(mny, __shfl_xor(mny, o)); mnz = min(mnz, __shfl_xor(mnz, o));
+            mxx = max(mxx, __shfl_xor(mxx, o)); mxy = max(mxy, __shfl_xor(mxy, o)); mxz = max(mxz, __shfl_xor(mxz, o));
+        }
+        const int bx = mxx - mnx + 3, by = mxy - mny + 3, bz = mxz - mnz + 3;
+        if (bx <= NEAR_BOX && by <= NEAR_BOX && bz <= NEAR_BOX) {
+            const int lox = mnx - 1, loy = mny - 1, loz = mnz - 1;
+            const int ncb = bx * by * bz;
+            // box leaves: slot counts, leaf ids, image shifts; exclusive scan of the counts
+            int cntv[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int cb = lane + 64 * h;
+                int cnt = 0;
+                if (cb < ncb) {
+                    const int ix = cb % bx, iy = (cb / bx) % by, iz = cb / (bx * by);
+                    const int gx = l0x + lox + ix, gy = l0y + loy + iy, gz = l0z + loz + iz;
+                    const int sx = floordiv(gx, n), sy = floordiv(gy, n), sz = floordiv(gz, n);
+                    const bool inside = periodic || (sx == 0 && sy == 0 && sz == 0);
+                    const int leaf = (gx - sx * n) + n * ((gy - sy * n) + n * (gz - sz * n));
+                    cnt = inside ? min(a.occ[leaf], a.occmax) : 0;
+                    s_cell[wv][cb] = inside ? leaf : 0;
+                    s_shift[wv][cb] = (sx + 8) | ((sy + 8) << 4) | ((sz + 8) << 8);
+                }
+                cntv[h] = cnt;
+            }
+            int tot = 0;
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                int v = cntv[h];
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int y = __shfl_up(v, o);
+                    if (lane >= o) v += y;
+                }
+                const int cb = lane + 64 * h;
+                if (cb < ncb) s_start[wv][cb] = tot + v - cntv[h];
+                tot += __shfl(v, 63);
+            }
+            if (lane == 0) s_start[wv][ncb] = tot;
+            if (lane < kn) {
+                // box frame: a particle of box leaf b sits at its position + s_b E, the point
+                // at its position + u E; the point's stencil is box leaves a-1 .. a+1
+                s_pu[wv][lane][0] = pt.x + (double)ux * E;
+                s_pu[wv][lane][1] = pt.y + (double)uy * E;
+                s_pu[wv][lane][2] = pt.z + (double)uz * E;
+                s_pk[wv][lane] = make_int4(ax - 1 - lox, ay - 1 - loy, az - 1 - loz, ex);
+            }
+            wave_sync();
+            const int S = tot;
+            double acc[KP];
+#pragma unroll
+            for (int k = 0; k < KP; k++) acc[k] = 0.0;
+            for (int base = 0; base < S; base += 64 * SPL) {
+                double qx[SPL], qy[SPL], qz[SPL], qw[SPL];
+                int id[SPL], ixj[SPL], iyj[SPL], izj[SPL];
+#pragma unroll
+                for (int j = 0; j < SPL; j++) {
+                    const int g0 = base + 64 * j + lane;
+                    const bool ok = g0 < S;
+                    const int g = ok ? g0 : S - 1;
+                    int lo = 0, hi = ncb - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (s_start[wv][mid] <= g) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    const int cb = lo;
+                    const size_t sidx = (size_t)s_cell[wv][cb] * a.occmax + (g - s_start[wv][cb]);
+                    const double4 q = a.slot_pos[sidx];
+                    const int sc = s_shift[wv][cb];
+                    qx[j] = q.x + (double)((sc & 15) - 8) * E;
+                    qy[j] = q.y + (double)(((sc >> 4) & 15) - 8) * E;
+                    qz[j] = q.z + (double)(((sc >> 8) & 15) - 8) * E;
+                    qw[j] = q.w;
+                    id[j] = a.slot_id[sidx];
+                    // slots past the list: a leaf index no stencil holds
+                    ixj[j] = ok ? cb % bx : 64;
+                    iyj[j] = (cb / bx) % by;
+                    izj[j] = cb / (bx * by);
+                }
+#pragma unroll
+                for (int k = 0; k < KP; k++) {
+                    if (k < kn) {
+                        const double pux = s_pu[wv][k][0], puy = s_pu[wv][k][1], puz = s_pu[wv][k][2];
+                        const int4 pk = s_pk[wv][k];
+#pragma unroll
+                        for (int j = 0; j < SPL; j++) {
+                            const bool member = (unsigned)(ixj[j] - pk.x) <= 2u && (unsigned)(iyj[j] - pk.y) <= 2u &&
+                                                (unsigned)(izj[j] - pk.z) <= 2u;
+                            const double dx = pux - qx[j], dy = puy - qy[j], dz = puz - qz[j];
+                            double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                            const bool use = member && (pk.w < 0 || id[j] != pk.w);
+                            r2 = use ? r2 : 1.0;
+                            const double contrib = qw[j] * rsqrt_sel<MIXED>(r2);
+                            acc[k] += use ? contrib : 0.0;
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < KP; k++) {
+                if (k < kn) {
+                    double v = acc[k];
+#pragma unroll
+                    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+                    if (lane == k) a.out[e] = v * pt.w;
+                }
+            }
+            wave_sync();
+        } else {
+            // points spread over more leaves: each point over its own 27 leaves, lane k < 27
+            // one leaf, its slots four at a time
+            for (int k = 0; k < kn; k++) {
+                const int ek = __builtin_amdgcn_readlane(e, k);
+                const double px0 = readlane_d(pt.x, k), py0 = readlane_d(pt.y, k), pz0 = readlane_d(pt.z, k);
+                const int ck = a.cell[ek];
+                const int exk = __builtin_amdgcn_readlane(ex, k);
+                double v = 0.0;
+                if (lane < 27) {
+                    const int tx = ck % n, ty = (ck / n) % n, tz = ck / (n * n);
+                    int ox, oy, oz;
+                    stencil(lane, ox, oy, oz);
+                    int nx = tx + ox, ny = ty + oy, nz = tz + oz;
+                    double shx = 0.0, shy = 0.0, shz = 0.0;
+                    bool okc = true;
+                    if (periodic) {
+                        if (nx < 0) { nx += n; shx = -a.g.E; } else if (nx >= n) { nx -= n; shx = a.g.E; }
+                        if (ny < 0) { ny += n; shy = -a.g.E; } else if (ny >= n) { ny -= n; shy = a.g.E; }
+                        if (nz < 0) { nz += n; shz = -a.g.E; } else if (nz >= n) { nz -= n; shz = a.g.E; }
+                    } else if (nx < 0 || nx >= n || ny < 0 || ny >= n || nz < 0 || nz >= n) {
+                        okc = false;
+                    }
+                    if (okc) {
+                        const int nc = nx + n * (ny + n * nz);
+                        const int oc = min(a.occ[nc], a.occmax);
+                        const size_t b0 = (size_t)nc * a.occmax;
+                        for (int sl = 0; sl < oc; sl++) {
+                            const double4 q = a.slot_pos[b0 + sl];
+                            const double dx = (px0 - shx) - q.x, dy = (py0 - shy) - q.y, dz = (pz0 - shz) - q.z;
+                            double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                            const bool use = exk < 0 || a.slot_id[b0 + sl] != exk;
+                            r2 = use ? r2 : 1.0;
+                            const double contrib = q.w * rsqrt_sel<MIXED>(r2);
+                            v += use ? contrib : 0.0;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+                if (lane == k) a.out[e] = v * pt.w;
+            }
+        }
+    }
+}
+
 
 // ============================================================================
 // proposal setup (tuple form) and validation
@@ -852,6 +1085,7 @@ struct SetupArgs {
     int32_t* eexcl;
     int32_t* owner;
     int* err;
+    int64_t* es_out;     // copy of es[0..P] (read by the grouped near-field kernel)
 };
 
 // owner p of proposal t: es[p] <= t < es[p+1]; always returns a value in [0, P)
@@ -879,6 +1113,8 @@ __device__ __forceinline__ void setup_body(const SetupArgs& a) {
         if (a.es[x] > a.es[x + 1]) bad++;
         if (x == 0 && a.es[0] != 0) bad++;
         if (x == a.P - 1 && a.es[a.P] != a.T) bad++;
+        a.es_out[x] = a.es[x];
+        if (x == a.P - 1) a.es_out[a.P] = a.es[a.P];
     }
     if (x < a.T) {
         const int p = owner_of(x, a.es, a.P);
@@ -2522,7 +2758,7 @@ static bool order_fns(int L, OrderFns* f) {
 template <bool MIXED>
 static void launch_near_t(EvalArgs a, int G, hipStream_t s) {
     if (a.n <= 0) return;
-    if (G == 27) {
+    if (G == 27 || G == 0) {
         const long long threads = (long long)a.n * 32;
         hipLaunchKernelGGL((k_near_cells<MIXED>), dim3((int)((threads + 255) / 256)), dim3(256), 0, s, a);
         return;
@@ -2541,6 +2777,17 @@ static void launch_near_t(EvalArgs a, int G, hipStream_t s) {
 static void launch_near(EvalArgs a, int G, bool mixed, hipStream_t s) {
     if (mixed) launch_near_t<true>(a, G, s);
     else launch_near_t<false>(a, G, s);
+}
+// evaluation entries of a propose (T proposals, then P old positions, grouped by es):
+// the grouped kernel when selected (near_lanes 0), else the per-point kernels
+static void launch_near_prop(EvalArgs a, int G, bool mixed, int P, int T, const int64_t* es, hipStream_t s) {
+    if (G == 0 && es && P > 0) {
+        const int blocks = (P + 3) / 4;
+        if (mixed) hipLaunchKernelGGL(k_near_groups<true>, dim3(blocks), dim3(256), 0, s, a, P, T, es);
+        else hipLaunchKernelGGL(k_near_groups<false>, dim3(blocks), dim3(256), 0, s, a, P, T, es);
+        return;
+    }
+    launch_near(a, G, mixed, s);
 }
 
 // ============================================================================
@@ -2621,6 +2868,7 @@ struct kmc_ctx {
     DBuf<int32_t> ecell, eexcl, owner;
     DBuf<double> UD, UI, lrd, dE1, rates, bsum;
     DBuf<int64_t> one_pid, one_es;
+    DBuf<int64_t> es_copy;            // es of the last propose set-up
     DBuf<double> one_pos;
     // propose_with_dats work buffers
     DBuf<int64_t> d_cnt, d_has, d_slot, d_off, d_pid, d_es, d_rate, d_tot;
@@ -2650,6 +2898,7 @@ static int ensure_eval(kmc_ctx* c, int64_t T, int64_t P) {
     HIPCHK(c->lrd.ensure((size_t)12 * std::max<int64_t>(T, 1)));
     HIPCHK(c->rates.ensure((size_t)std::max<int64_t>(T, 1)));
     HIPCHK(c->bsum.ensure((size_t)std::max<int64_t>((T + 63) / 64, 1)));
+    HIPCHK(c->es_copy.ensure((size_t)P + 1));
     return KMC_OK;
 }
 
@@ -2714,6 +2963,7 @@ static SetupArgs setup_args(kmc_ctx* c, int64_t P, int64_t T, const int64_t* d_p
     a.eexcl = c->eexcl.p;
     a.owner = c->owner.p;
     a.err = c->err.p;
+    a.es_out = c->es_copy.p;
     return a;
 }
 
@@ -2782,7 +3032,7 @@ static bool step_sized(const kmc_ctx* c, int64_t P, int64_t T) {
     const bool lr_sized =
         !c->have_lr || ((size_t)12 * P * lr_nbmax(c->L) <= c->aold.n && (size_t)12 * P <= c->cold.n);
     return (size_t)(T + P) <= c->epos4.n && (size_t)T <= c->owner.n && (size_t)T <= c->rates.n &&
-           (size_t)((T + 63) / 64) <= c->bsum.n && lr_sized;
+           (size_t)((T + 63) / 64) <= c->bsum.n && (size_t)(P + 1) <= c->es_copy.n && lr_sized;
 }
 
 static SelectArgs select_args(kmc_ctx* c, int64_t P, int64_t T, const int64_t* d_pid, double* d_dE, int64_t nsteps,
@@ -2843,7 +3093,7 @@ static int run_propose_core(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, dou
     }
     if (c->bc != KMC_BC_FF_ONLY) {
         EvalArgs an = eval_args(c, ne, c->epos4.p, c->ecell.p, c->eexcl.p, c->UD.p);
-        launch_near(an, c->near_lanes, mixed, s);
+        launch_near_prop(an, c->near_lanes, mixed, (int)P, (int)T, c->es_copy.p, s);
         EvalArgs af = eval_args(c, ne, c->epos4.p, c->ecell.p, c->eexcl.p, c->UI.p);
         c->fns.far(af, s);
     }
@@ -3164,9 +3414,10 @@ int kmc_destroy(kmc_ctx* c) {
 
 int kmc_set_tiles(kmc_ctx* c, int32_t near_lanes, int32_t sweep_block, int32_t sweep_pairs, int32_t sweep_split) {
     if (check_ctx(c)) return KMC_E_ARG;
-    if (near_lanes != 1 && near_lanes != 4 && near_lanes != 8 && near_lanes != 16 && near_lanes != 27 &&
-        near_lanes != 32 && near_lanes != 64)
-        return set_err(KMC_E_ARG, "near_lanes must be one of 1,4,8,16,27,32,64 (27: one lane per neighbour leaf)");
+    if (near_lanes != 0 && near_lanes != 1 && near_lanes != 4 && near_lanes != 8 && near_lanes != 16 &&
+        near_lanes != 27 && near_lanes != 32 && near_lanes != 64)
+        return set_err(KMC_E_ARG, "near_lanes must be one of 0,1,4,8,16,27,32,64 (27: one lane per neighbour leaf, "
+                                  "0: the proposals of a moving charge together)");
     if (sweep_pairs < 1 || sweep_pairs > 2) return set_err(KMC_E_ARG, "sweep_pairs must be 1 or 2");
     if (sweep_block != 0 && (sweep_block < 64 || sweep_block > 256 || sweep_block % 64))
         return set_err(KMC_E_ARG, "sweep_block must be 0 (fused), 64, 128, 192 or 256");
@@ -3754,7 +4005,7 @@ int kmc_time_kernel(kmc_ctx* c, void* stream, int32_t which, int32_t reps, int32
             c->fns.sweep(sa, (int)c->ncells, s, c->sweep_block, c->sweep_pairs, c->sweep_split);
         } else if (which == 1) {
             EvalArgs an = eval_args(c, (int)(T + P), c->epos4.p, c->ecell.p, c->eexcl.p, c->UD.p);
-            launch_near(an, c->near_lanes, mixed, s);
+            launch_near_prop(an, c->near_lanes, mixed, (int)P, (int)T, c->es_copy.p, s);
         } else if (which == 2) {
             EvalArgs af = eval_args(c, (int)(T + P), c->epos4.p, c->ecell.p, c->eexcl.p, c->UI.p);
             c->fns.far(af, s);

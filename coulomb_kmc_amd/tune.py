@@ -24,7 +24,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(_HERE, "tiles.json")
 
-NEAR_LANES = (1, 4, 8, 16, 27, 32, 64)
+NEAR_LANES = (0, 1, 4, 8, 16, 27, 32, 64)
 # (sweep_pairs, sweep_block, sweep_split)
 SWEEP_SHAPES = ((1, 0, 1), (2, 0, 1), (1, 64, 1), (1, 256, 1), (2, 64, 1), (2, 256, 1), (1, 64, 3), (2, 64, 3))
 DEFAULT = {"near_lanes": 27, "sweep_block": 0, "sweep_pairs": 2, "sweep_split": 1}

@@ -188,6 +188,33 @@ def test_every_tile_shape_matches_oracle(bc, L):
     kmc.free()
 
 
+@pytest.mark.parametrize("bc", BCS)
+@pytest.mark.parametrize("r", (2, 4))
+def test_grouped_near_field(bc, r):
+    """The grouped near-field kernel (near_lanes 0: a moving charge's old position and
+    proposals over one box of leaves) against the per-point kernel on the same state
+    (same pairs, other summation order) and against the oracle: 1 to 21 proposals per
+    charge (two passes above 15), proposals up to 2.7 leaves away (the per-point
+    fallback), two leaves per side (periodic images of the same leaf)."""
+    kmc, orc, pos, q, rng = make_pair(bc, N=300, E=6.0, r=r, L=8, max_move=2.0, seed=40 + r)
+    pids = rng.choice(len(q), 30, replace=False)
+    moves = []
+    for i, pid in enumerate(pids):
+        moves += list(ball_moves(rng, pos, [pid], 1 + (i * 7) % 21, 2.0, 6.0, bc))
+    moves = tuple(moves)
+    for prec, tol in (("fp64", 1e-10), ("mixed", 1e-8)):
+        kmc.precision = prec
+        kmc.set_tiles(near_lanes=27)
+        ref = np.concatenate(kmc.propose(moves)) - kmc.energy
+        kmc.set_tiles(near_lanes=0)
+        g = np.concatenate(kmc.propose(moves)) - kmc.energy
+        c = np.concatenate(orc.propose(moves)) - orc.energy
+        _print("grouped near", bc, r, prec, rel_err(g, ref).max(), rel_err(g, c).max())
+        assert rel_err(g, ref).max() <= 1e-11
+        assert rel_err(g, c).max() <= tol
+    kmc.free()
+
+
 def test_accept_equals_reinitialise():
     """Sweep after a move == fresh initialise of the moved system (direct P2L, exact)."""
     from coulomb_kmc_amd import KMCFMM
