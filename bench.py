@@ -380,10 +380,10 @@ def main():
     d_dE2 = torch.zeros(T, dtype=torch.float64, device="cuda")
 
     # ---- dE error of mixed vs FP64 on the initial state, 65536 proposals (or fewer)
-    nerr = max(1, 65536 // T)
+    nerr = max(1, min(65536 // T, nstream))
     errs_abs, errs_rel, ref_vals = [], [], []
     for b in range(nerr):
-        counter.fill_(nstream - 1 - b)
+        counter.fill_(nstream - 1 - b)   # the last rows of the move stream, 0 <= row < nstream
         kmc.materialize_moves(P, k, t_pids, t_disp, counter, d_pid, d_es, d_np)
         kmc.propose_device(d_pid, d_es, d_np, d_dE, precision="fp64")
         kmc.propose_device(d_pid, d_es, d_np, d_dE2, precision="mixed")
@@ -414,7 +414,7 @@ def main():
         t_cpu, n_cpu, cmax = 0.0, 0, 0.0
         gpos = pos0.copy()
         picks = []
-        while n_cpu < 1 or (t_cpu < args.cpu_seconds and n_cpu < 50):
+        while n_cpu < 1 or (t_cpu < args.cpu_seconds and n_cpu < min(50, nstream - 1)):
             s = n_cpu
             moves = host_moves(orc.positions(), pids_all[s], disp_all[s], k, E, bc)
             es = np.arange(P + 1, dtype=np.int64) * k
@@ -438,6 +438,24 @@ def main():
                "sample": "{} steps ({} proposals + 1 accept each) of this workload from the same state and move "
                          "stream, {:.1f} s, oracle/kmc_oracle.cpp (FP64, OpenMP, reference data structures)"
                          .format(n_cpu, T, t_cpu)}
+        # one thread, propose only, on the first moving charges of the next step of the stream
+        P1 = max(1, min(P, 2048))
+        moves1 = host_moves(orc.positions(), pids_all[n_cpu][:P1], disp_all[n_cpu][:P1 * k], k, E, bc)
+        O.set_threads(1)
+        tc = time.time()
+        orc.propose_arrays(pids_all[n_cpu][:P1], np.arange(P1 + 1, dtype=np.int64) * k,
+                           np.concatenate([m[1] for m in moves1]))
+        t1 = time.time() - tc
+        O.set_threads(nth)
+        cpu["single_thread"] = {"value": P1 * k / t1, "unit": "proposals/s",
+                                "sample": "{} proposals ({} moving charges x {}) of the next step, propose only, "
+                                          "{:.2f} s".format(P1 * k, P1, k, t1)}
+        try:
+            with open("/proc/cpuinfo") as fh:
+                cpu["cpu_model"] = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")),
+                                        "unknown")
+        except OSError:
+            cpu["cpu_model"] = "unknown"
         cpu_parity = {"fp64_dE_max_rel_err_step0": cmax, "cpu_picks": picks}
         log("cpu baseline", cpu)
 

@@ -1654,6 +1654,13 @@ __global__ void __launch_bounds__(256) kb_accept_tail(const TailArgs* __restrict
 // ============================================================================
 // Metropolis selection on the device (one block)
 // ============================================================================
+// row of a cyclic per-step table for a step counter: counter mod nsteps in [0, nsteps)
+// for every counter value, negative ones included
+__device__ __forceinline__ int64_t step_of(int64_t counter, int64_t nsteps) {
+    const int64_t r = counter % nsteps;
+    return r < 0 ? r + nsteps : r;
+}
+
 struct SelectArgs {
     Geom g;
     int T;
@@ -1723,7 +1730,7 @@ __device__ __forceinline__ void select_body(const SelectArgs& a) {
     const double prev = __shfl_up(incl, 1, 64);
     __syncthreads();
     const double lo = lane > 0 ? prev : (wid > 0 ? wlast[wid - 1] : 0.0);
-    const int64_t step = *a.counter % a.nsteps;
+    const int64_t step = step_of(*a.counter, a.nsteps);
     const double total = wlast[nw - 1];
     const double target = a.uniform[step] * total;
     if (target >= lo && target < incl && b1 > b0) {
@@ -1863,7 +1870,7 @@ __device__ __forceinline__ void materialize_body(const MatArgs& a) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int P = a.P, k = a.k;
     const int T = P * k;
-    const int64_t step = *a.counter % a.nsteps;
+    const int64_t step = step_of(*a.counter, a.nsteps);
     if (t <= P && t >= 0 && t < P + 1) a.es[t] = (int64_t)t * k;
     if (t >= T) return;
     const int p = t / k;

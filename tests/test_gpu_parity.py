@@ -215,6 +215,28 @@ def test_grouped_near_field(bc, r):
     kmc.free()
 
 
+def test_step_counter_below_zero_wraps():
+    """The synthetic move stream and the selection take row (counter mod nsteps) of
+    their per-step tables, counted from 0 also for counters below zero (a counter of -1
+    once addressed the row before the table)."""
+    import torch
+    kmc, orc, pos, q, rng = make_pair("pbc", N=200, E=5.0, r=3, L=8)
+    nsteps, P, k = 5, 4, 2
+    pids = np.stack([rng.choice(200, P, replace=False) for _ in range(nsteps)]).astype(np.int64)
+    disp = rng.uniform(-0.3, 0.3, (nsteps, P * k, 3))
+    t_pids = torch.as_tensor(pids).cuda()
+    t_disp = torch.as_tensor(disp).cuda()
+    d_pid = torch.zeros(P, dtype=torch.int64, device="cuda")
+    d_es = torch.zeros(P + 1, dtype=torch.int64, device="cuda")
+    d_np = torch.zeros(P * k * 3, dtype=torch.float64, device="cuda")
+    for c in (-1, -7, 12):
+        counter = torch.full((1,), c, dtype=torch.int64, device="cuda")
+        kmc.materialize_moves(P, k, t_pids, t_disp, counter, d_pid, d_es, d_np)
+        assert np.array_equal(d_pid.cpu().numpy(), pids[c % nsteps]), c
+    assert kmc.check_flags() == 0
+    kmc.free()
+
+
 def test_accept_equals_reinitialise():
     """Sweep after a move == fresh initialise of the moved system (direct P2L, exact)."""
     from coulomb_kmc_amd import KMCFMM
