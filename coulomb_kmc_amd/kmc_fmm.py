@@ -273,6 +273,10 @@ class KMCFMM(InjectorExtractor):
         n = self.N
         smc = np.ascontiguousarray(np.asarray(site_max_counts, dtype=np.int64).ravel())
         sites = np.ascontiguousarray(_as_array(current_sites, 1, np.int64)[:n].ravel())
+        if masks.shape[0] < n or ppos.shape[0] < n or diffs.shape[0] < n or sites.shape[0] < n:
+            raise RuntimeError("propose_with_dats: the dats hold fewer than N rows")
+        if n and (sites.min() < 0 or sites.max() >= smc.size):
+            raise RuntimeError("propose_with_dats: a current site is not an index of site_max_counts")
         d_smc = self._dev(smc, torch.int64)
         d_sites = self._dev(sites, torch.int64)
         d_ppos = self._dev(ppos[:n], torch.float64)

@@ -125,6 +125,7 @@ int kmc_propose(kmc_ctx* ctx, void* stream, int64_t P, int64_t T, const int64_t*
  * prop_charges[n][max_prop] or NULL; writes prop_energy_diffs[n][max_prop] at every
  * masked slot taken (mask > 0, at most site_max_counts[site] per charge).
  * *h_total_moves (optional) receives the number of proposals evaluated.
+ * Every current_sites[i] must index d_site_max_counts (not checked on the device).
  */
 int kmc_propose_with_dats(kmc_ctx* ctx, void* stream, int64_t max_prop, const int64_t* d_site_max_counts,
                           const int64_t* d_current_sites, const double* d_prop_positions,
