@@ -887,8 +887,10 @@ __global__ void __launch_bounds__(256) k_near_groups(EvalArgs a, int P, int T, c
     const int n = a.g.ncps;
     const double E = a.g.E;
     const bool periodic = a.g.bc != KMC_BC_FREE_SPACE;
-    const int t0 = (int)es[p];
-    const int K = 1 + (int)es[p + 1] - t0;
+    // es as k_setup validates it, clamped so that no entry index leaves [0, T + P)
+    const int t0 = (int)min((int64_t)T, max((int64_t)0, es[p]));
+    const int t1 = (int)min((int64_t)T, max((int64_t)t0, es[p + 1]));
+    const int K = 1 + t1 - t0;
     const int c0 = a.cell[T + p];
     const int l0x = c0 % n, l0y = (c0 / n) % n, l0z = c0 / (n * n);
     for (int pass = 0; pass < K; pass += KP) {
