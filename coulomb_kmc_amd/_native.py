@@ -42,7 +42,7 @@ EXPORTS = (
     "kmc_batch_create", "kmc_batch_destroy", "kmc_batch_set_system", "kmc_batch_prepare", "kmc_batch_step",
     "kmc_particle_count", "kmc_set_particle_count", "kmc_charge_energies", "kmc_group_energy",
     "kmc_propose_extract", "kmc_propose_inject", "kmc_extract", "kmc_inject", "kmc_discover_sites",
-    "kmc_lattice_step",
+    "kmc_lattice_step", "kmc_accept_movedata",
 )
 
 
@@ -129,6 +129,7 @@ def lib():
         "kmc_inject": (ctypes.c_int, [P, P, I64, P, P, P]),
         "kmc_discover_sites": (ctypes.c_int, [P, I64, P, I64, P, I64, P, D, P, P]),
         "kmc_lattice_step": (ctypes.c_int, [P, P, I64, P, D, I32, P, P, P, I64, P, D, I32, P, P, I32]),
+        "kmc_accept_movedata": (ctypes.c_int, [P, P, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -3921,6 +3921,26 @@ int kmc_accept(kmc_ctx* c, void* stream, int64_t pid, const double* h_new_pos, d
     return KMC_OK;
 }
 
+int kmc_accept_movedata(kmc_ctx* c, void* stream, const int64_t* h_movedata, double* h_energy) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    if (!c->initialised) return set_err(KMC_E_STATE, "Run initialise before this call");
+    if (!h_movedata) return set_err(KMC_E_ARG, "null move record");
+    double rd[7];
+    memcpy(rd, h_movedata, sizeof(rd));
+    const int64_t pid = h_movedata[7];
+    if (pid < 0 || pid >= c->n) return set_err(KMC_E_ARG, "move record: charge id out of range");
+    int rc = refresh_mirrors(c, (hipStream_t)stream);
+    if (rc) return rc;
+    const int n = c->ncps;
+    const int newc = bin_axis(rd[3], c->E, c->w, n) + n * (bin_axis(rd[4], c->E, c->w, n) + n * bin_axis(rd[5], c->E, c->w, n));
+    for (int d = 0; d < 3; d++)
+        if (rd[d] != c->h_pos[3 * pid + d]) return set_err(KMC_E_ARG, "move record: old position is not the charge's");
+    if (rd[6] != c->h_q[pid]) return set_err(KMC_E_ARG, "move record: charge is not the charge's");
+    if (h_movedata[8] != c->h_cell[pid] || h_movedata[9] != newc)
+        return set_err(KMC_E_ARG, "move record: cells do not match the positions");
+    return kmc_accept(c, stream, pid, rd + 3, h_energy);
+}
+
 int kmc_step(kmc_ctx* c, void* stream, int64_t P, int64_t T, const int64_t* d_pid, const int64_t* d_es,
              const double* d_new_pos, double* d_dE, int64_t nsteps, const double* d_uniform, double kT,
              int64_t* d_counter, int64_t* d_selected, int32_t precision) {

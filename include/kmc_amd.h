@@ -143,6 +143,15 @@ int kmc_propose_with_dats(kmc_ctx* ctx, void* stream, int64_t max_prop, const in
 int kmc_accept(kmc_ctx* ctx, void* stream, int64_t pid, const double* h_new_pos, double* h_energy);
 
 /*
+ * The accept of KMCFMM._accept's move record (coulomb_kmc/kmc_fmm.py:811-836):
+ * movedata[10] int64, words 0..6 the doubles old position, new position, charge, then
+ * the charge's id (the local id here), old leaf and new leaf (linear, x fastest).  The
+ * record must match the charge's current position, charge and leaf and the new
+ * position's leaf; then as kmc_accept.
+ */
+int kmc_accept_movedata(kmc_ctx* ctx, void* stream, const int64_t* h_movedata, double* h_energy);
+
+/*
  * One whole KMC step on the device, no host synchronisation (graph capturable
  * once the context's work buffers have been sized by a first call):
  * proposals T from d_pid/d_es/d_new_pos as kmc_propose, then Metropolis selection

@@ -237,6 +237,35 @@ def test_step_counter_below_zero_wraps():
     kmc.free()
 
 
+def test_accept_movedata_equals_accept():
+    """kmc_accept_movedata (the reference's 10-word move record, kmc_fmm.py:811-836)
+    does what kmc_accept does, and refuses a record that does not match the state."""
+    import ctypes
+    a, orc, pos, q, rng = make_pair("pbc", N=200, E=5.0, r=3, L=8, seed=31)
+    b, _, _, _, _ = make_pair("pbc", N=200, E=5.0, r=3, L=8, seed=31)
+    n = a._ncps
+    pid = 11
+    newp = np.clip(pos[pid] + np.array([0.4, -0.3, 0.2]), -2.5, 2.5)
+    c3o, c3n = a.leaf_of(pos[pid:pid + 1])[0], a.leaf_of(newp[None, :])[0]
+    md = np.zeros(10, np.int64)
+    rd = md[:7].view(np.float64)
+    rd[0:3], rd[3:6], rd[6] = pos[pid], newp, q[pid]
+    md[7] = pid
+    md[8] = c3o[0] + n * (c3o[1] + n * c3o[2])
+    md[9] = c3n[0] + n * (c3n[1] + n * c3n[2])
+    e = ctypes.c_double(0.0)
+    bad = md.copy()
+    bad[9] = (md[9] + 1) % n ** 3
+    assert a._lib.kmc_accept_movedata(a.handle, a._stream(), bad.ctypes.data_as(ctypes.c_void_p), ctypes.byref(e)) != 0
+    assert a._lib.kmc_accept_movedata(a.handle, a._stream(), md.ctypes.data_as(ctypes.c_void_p), ctypes.byref(e)) == 0
+    b.accept((pid, newp))
+    assert e.value == b.energy
+    assert np.array_equal(a.get_expansions(), b.get_expansions())
+    assert np.array_equal(a.get_device_particles()[0], b.get_device_particles()[0])
+    a.free()
+    b.free()
+
+
 def test_accept_equals_reinitialise():
     """Sweep after a move == fresh initialise of the moved system (direct P2L, exact)."""
     from coulomb_kmc_amd import KMCFMM
