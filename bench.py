@@ -187,7 +187,8 @@ def traffic_from_pmc(workload, fetch_dir, write_dir):
 
 BATCH = dict(workload="batch: 256 systems, N log-uniform [2000, 200000] at density 0.1, half free-space half pbc, "
                       "8-16 charges per leaf, L in {8, 12, 16}, 256x8 proposals + 1 accept per system per step",
-             systems=256, nmin=2000, nmax=200000, density=0.1, P=256, k=8, radius=1.0, seed=1003, steps=20, warmup=4)
+             systems=256, nmin=2000, nmax=200000, density=0.1, P=256, k=8, radius=1.0, seed=1003, steps=20, warmup=4,
+             near_lanes=8)
 
 
 def batch_systems(spec):
@@ -229,8 +230,10 @@ def run_batch(args):
         k = KMCFMM(x["pos"].copy(), x["q"].reshape(-1, 1), x["E"], r=x["r"], l=x["L"], boundary_condition=x["bc"],
                    max_move=x["radius"], precision="mixed", init_method="fmm")
         k.initialise()
+        k.set_tiles(near_lanes=near_lanes)
         return k
 
+    near_lanes = args.near_lanes or spec["near_lanes"]
     nst = 16
     systems = []
     t0 = time.time()
@@ -282,7 +285,9 @@ def run_batch(args):
                    "proposals_per_step": T, "graph_steps": G, "streams": args.batch_streams,
                    "mode": args.batch_mode,
                    "tiles": {"L{}_C{}_T{}".format(*k): v for k, v in tiles.items()} or
-                            "batched kernels: fused sweep, 2 column pairs per lane, 32 near lanes per point"},
+                            "batched kernels: fused sweep, 2 column pairs per lane, near field " +
+                            ("one lane per neighbour leaf" if near_lanes in (0, 27) else
+                             "{} lanes per point".format(near_lanes))},
         "accepts_per_s": len(systems) * steps / (ms * 1e-3),
         "initialise_s": t_init,
         "flags": flags,

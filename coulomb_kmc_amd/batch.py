@@ -10,7 +10,9 @@ Metropolis and accepted.  Two ways to run it:
   whole batch (once per expansion order for the order-specialised kernels), the
   system being blockIdx.z of a table of per-system kernel arguments, so a batch step
   is ~20 launches whatever the number of systems.  The accept sweep uses the fused
-  layout with two column pairs per lane, the near field 32 lanes per point.
+  layout with two column pairs per lane; the near field runs G lanes per point when
+  every system selects the same G with set_tiles(near_lanes=G), else one lane per
+  neighbour leaf.
 * ``mode="graph"``: the systems' own kmc_step calls one after the other, with each
   system's tiles from the autotuner (per order / leaf count / proposal count class).
 

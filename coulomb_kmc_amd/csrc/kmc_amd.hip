@@ -736,7 +736,7 @@ __global__ void __launch_bounds__(512) kb_far(const EvalArgs* __restrict__ args)
 // shuffle reduction inside the lane group.
 // ============================================================================
 template <int G, bool MIXED>
-__global__ void __launch_bounds__(256) k_near(EvalArgs a) {
+__device__ __forceinline__ void near_body(const EvalArgs& a) {
     const int gid = (blockIdx.x * blockDim.x + threadIdx.x);
     const int i = gid / G;
     const int lane = gid % G;
@@ -780,6 +780,13 @@ __global__ void __launch_bounds__(256) k_near(EvalArgs a) {
         for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, G);
     }
     if (active && lane == 0) a.out[i] = acc * a.pos4[i].w;
+}
+template <int G, bool MIXED>
+__global__ void __launch_bounds__(256) k_near(EvalArgs a) { near_body<G, MIXED>(a); }
+// batched form: system blockIdx.z of an argument table (grid sized for the largest)
+template <int G, bool MIXED>
+__global__ void __launch_bounds__(256) kb_near(const EvalArgs* __restrict__ args) {
+    near_body<G, MIXED>(args[blockIdx.z]);
 }
 
 // near field, one lane per neighbour leaf: 32 lanes per evaluation point, lanes
@@ -4478,6 +4485,7 @@ struct kmc_batch {
     double kT = 1.0;
     bool mixed = true;
     bool ready = false;
+    int near_lanes = 27;  // the systems' common per-point lane count (1..64), else 27: k_near_cells' layout
     BatchShape all{};
     DBuf<MatArgs> mat;
     DBuf<SetupArgs> setup;
@@ -4563,6 +4571,12 @@ int kmc_batch_prepare(kmc_batch* b, double kT, int32_t precision) {
     std::vector<std::vector<SweepArgs>> sweep;
     std::vector<std::vector<TailArgs>> tail;
     BatchShape all{0, 0, 0, 0, 0};
+    // near field: the per-point kernel when every system selects the same lane count
+    // (so a batch step sums in the order of the systems' own steps), else one lane per leaf
+    b->near_lanes = b->sys.empty() ? 27 : b->sys[0].c->near_lanes;
+    for (const kmc_batch::Sys& y : b->sys)
+        if (y.c->near_lanes != b->near_lanes) b->near_lanes = 27;
+    if (b->near_lanes == 0) b->near_lanes = 27;
     for (size_t i = 0; i < b->sys.size(); i++) {
         kmc_batch::Sys& y = b->sys[i];
         if (!y.set) return set_err(KMC_E_STATE, "batch system " + std::to_string(i) + " has no move stream");
@@ -4635,6 +4649,21 @@ int kmc_batch_prepare(kmc_batch* b, double kT, int32_t precision) {
     return KMC_OK;
 }
 
+template <bool MIXED>
+static void launch_batch_near(int G, int max_ne, int n, const EvalArgs* args, hipStream_t s) {
+    const long long threads = (long long)max_ne * (G == 27 ? 32 : G);
+    const dim3 grid((unsigned)((threads + 255) / 256), 1, n);
+    switch (G) {
+        case 1: hipLaunchKernelGGL((kb_near<1, MIXED>), grid, dim3(256), 0, s, args); break;
+        case 4: hipLaunchKernelGGL((kb_near<4, MIXED>), grid, dim3(256), 0, s, args); break;
+        case 8: hipLaunchKernelGGL((kb_near<8, MIXED>), grid, dim3(256), 0, s, args); break;
+        case 16: hipLaunchKernelGGL((kb_near<16, MIXED>), grid, dim3(256), 0, s, args); break;
+        case 32: hipLaunchKernelGGL((kb_near<32, MIXED>), grid, dim3(256), 0, s, args); break;
+        case 64: hipLaunchKernelGGL((kb_near<64, MIXED>), grid, dim3(256), 0, s, args); break;
+        default: hipLaunchKernelGGL(kb_near_cells<MIXED>, grid, dim3(256), 0, s, args); break;
+    }
+}
+
 int kmc_batch_step(kmc_batch* b, void* stream) {
     if (!b) return set_err(KMC_E_ARG, "null batch");
     if (!b->ready) return set_err(KMC_E_STATE, "call kmc_batch_prepare before kmc_batch_step");
@@ -4649,13 +4678,8 @@ int kmc_batch_step(kmc_batch* b, void* stream) {
     const int mat_threads = std::max(a.max_T, a.max_P + 1);
     hipLaunchKernelGGL(kb_materialize, dim3((mat_threads + 255) / 256, 1, n), dim3(256), 0, s, b->mat.p);
     hipLaunchKernelGGL(kb_setup, dim3((std::max(a.max_T, a.max_P) + 255) / 256, 1, n), dim3(256), 0, s, b->setup.p);
-    const long long near_threads = (long long)a.max_ne * 32;
-    if (b->mixed)
-        hipLaunchKernelGGL(kb_near_cells<true>, dim3((unsigned)((near_threads + 255) / 256), 1, n), dim3(256), 0, s,
-                           b->nearA.p);
-    else
-        hipLaunchKernelGGL(kb_near_cells<false>, dim3((unsigned)((near_threads + 255) / 256), 1, n), dim3(256), 0, s,
-                           b->nearA.p);
+    if (b->mixed) launch_batch_near<true>(b->near_lanes, a.max_ne, n, b->nearA.p, s);
+    else launch_batch_near<false>(b->near_lanes, a.max_ne, n, b->nearA.p, s);
     for (kmc_batch::Group* g : b->groups) {
         g->fns.bfar(g->far.p, g->shape, s);
         if (g->lr_shape.n > 0) g->fns.blr(g->lr.p, g->lr_shape, s);

@@ -34,14 +34,14 @@ def _systems(seed):
     return out
 
 
-def _check_batch(streams, mode="graph"):
+def _check_batch(streams, mode="graph", near_lanes=27):
     import torch
     from coulomb_kmc_amd.batch import KMCBatch
     steps = 4
     batched = _systems(5)
     single = _systems(5)
     for s in batched + single:   # the batched kernels' shapes
-        s.kmc.set_tiles(near_lanes=27, sweep_block=0, sweep_pairs=2, sweep_split=1)
+        s.kmc.set_tiles(near_lanes=near_lanes, sweep_block=0, sweep_pairs=2, sweep_split=1)
     b = KMCBatch(batched, kT=1.0, precision="mixed", streams=streams, mode=mode)
     b.capture(steps_per_graph=2)
     b.replay(steps // 2)
@@ -70,6 +70,12 @@ def test_batch_graph_equals_single_system_steps():
 
 def test_batched_kernels_equal_single_system_steps():
     _check_batch(1, "kernels")
+
+
+@pytest.mark.parametrize("near_lanes", [4, 8, 16])
+def test_batched_per_point_near_equals_single_system_steps(near_lanes):
+    # every system selects the same per-point lane count: the batch uses kb_near<G>
+    _check_batch(1, "kernels", near_lanes)
 
 
 @pytest.mark.skip(reason="recording the systems on extra torch side streams crashed the host process inside "
