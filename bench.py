@@ -101,6 +101,18 @@ def metropolis_pick(dE, u, kT=1.0):
 COPY_PROBE_BYTES = 512 * 2 ** 20
 
 
+def measured_peaks(nbytes=2 ** 30, reps=10):
+    """The roofline denominators measured on this device (kmc_probe_peaks): a streaming
+    copy of nbytes (GB/s of read + write) and a loop of independent FP64 FMA chains."""
+    import ctypes
+    import torch
+    from coulomb_kmc_amd import _native
+    out = (ctypes.c_double * 2)()
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    _native.check(_native.lib().kmc_probe_peaks(st, nbytes, reps, out), "kmc_probe_peaks")
+    return {"copy_GBs": out[0], "copy_bytes": nbytes, "fp64_fma_TFLOPs": out[1]}
+
+
 def probe_traffic(kmc, cfg, pids_all, disp_all):
     """Launches for the HBM-traffic counters: one propose (sizes the evaluation arrays),
     5 accept sweeps, then 5 device copies of COPY_PROBE_BYTES (calibration of the
@@ -416,7 +428,7 @@ def main():
         orc = O.Oracle(pos0, q, E, r, L, bc=bc, max_move=cfg["radius"],
                        lr=lr_operator_reference_layout(L, E) if bc in ("pbc", "ff-only") else None)
         orc.init_from_expansions(kmc.get_expansions(), kmc.energy)
-        t_cpu, n_cpu, cmax = 0.0, 0, 0.0
+        t_cpu, n_cpu, cmax, cabs, cmed = 0.0, 0, 0.0, 0.0, 0.0
         gpos = pos0.copy()
         picks = []
         while n_cpu < 1 or (t_cpu < args.cpu_seconds and n_cpu < min(50, nstream - 1)):
@@ -437,6 +449,8 @@ def main():
                 kmc.propose_device(d_pid, d_es, d_np, d_dE, precision="fp64")
                 g = d_dE.cpu().numpy()
                 cmax = float(np.max(np.abs(g - dc) / np.maximum(np.abs(dc), 1e-3)))
+                cabs = float(np.max(np.abs(g - dc)))
+                cmed = float(np.median(np.abs(g - dc) / np.maximum(np.abs(dc), 1e-3)))
             n_cpu += 1
         cpu = {"value": n_cpu * T / t_cpu, "unit": "proposals/s", "cores": nth, "kind": "port",
                "accepts_per_s": n_cpu / t_cpu,
@@ -461,7 +475,8 @@ def main():
                                         "unknown")
         except OSError:
             cpu["cpu_model"] = "unknown"
-        cpu_parity = {"fp64_dE_max_rel_err_step0": cmax, "cpu_picks": picks}
+        cpu_parity = {"fp64_dE_max_rel_err_step0": cmax, "fp64_dE_median_rel_err_step0": cmed,
+                      "fp64_dE_max_abs_err_step0": cabs, "rel_err_floor": 1e-3, "cpu_picks": picks}
         log("cpu baseline", cpu)
 
     # ---- timed runs: graph of G steps, replayed
@@ -550,6 +565,11 @@ def main():
     traffic = tune.read_traffic(ROOT, "k_sweep", cfg)
     if traffic is not None:
         roof["traffic"] = traffic
+    peaks = measured_peaks()
+    log("measured peaks", peaks)
+    roof["measured_peaks"] = peaks
+    roof["frac_of_measured"] = ach_tflops / peaks["fp64_fma_TFLOPs"]
+    roof["hbm_frac_of_measured"] = ach_gbs / peaks["copy_GBs"]
 
     mixed, fp64 = results["mixed"], results["fp64"]
     out = {

@@ -4062,6 +4062,73 @@ int kmc_time_kernel(kmc_ctx* c, void* stream, int32_t which, int32_t reps, int32
     return KMC_OK;
 }
 
+// roofline denominators measured on the device: a streaming copy (HBM read + write)
+// and a loop of independent FP64 FMA chains (8 per lane, 2 flops each)
+__global__ void __launch_bounds__(256) k_probe_copy(const double4* __restrict__ src, double4* __restrict__ dst,
+                                                     int64_t n4) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) dst[i] = src[i];
+}
+constexpr int PROBE_CHAINS = 8;
+__global__ void __launch_bounds__(256) k_probe_fma(double* __restrict__ out, int iters, double m, double c) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    double a[PROBE_CHAINS];
+#pragma unroll
+    for (int j = 0; j < PROBE_CHAINS; j++) a[j] = 1e-3 * (double)(gid + j);
+    for (int it = 0; it < iters; it++) {
+#pragma unroll
+        for (int j = 0; j < PROBE_CHAINS; j++) a[j] = fma(a[j], m, c);
+    }
+    double t = 0.0;
+#pragma unroll
+    for (int j = 0; j < PROBE_CHAINS; j++) t += a[j];
+    out[gid] = t;
+}
+
+int kmc_probe_peaks(void* stream, int64_t bytes, int32_t reps, double* h_out) {
+    if (bytes < (1 << 20) || bytes > ((int64_t)1 << 34) || reps < 1 || !h_out)
+        return set_err(KMC_E_ARG, "bytes must be in [1 MiB, 16 GiB], reps >= 1");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t n4 = bytes / (int64_t)sizeof(double4);
+    int dev = 0, ncu = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int copy_blocks = ncu * 16;
+    const int fma_blocks = ncu * 8, fma_iters = 4096;
+    DBuf<double4> src, dst;
+    DBuf<double> out;
+    HIPCHK(src.ensure(n4));
+    HIPCHK(dst.ensure(n4));
+    HIPCHK(out.ensure((size_t)fma_blocks * 256));
+    HIPCHK(hipMemsetAsync(src.p, 0, (size_t)n4 * sizeof(double4), s));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    float ms = 0.0f;
+    // warm-up, then timed
+    hipLaunchKernelGGL(k_probe_copy, dim3(copy_blocks), dim3(256), 0, s, src.p, dst.p, n4);
+    HIPCHK(hipEventRecord(e0, s));
+    for (int i = 0; i < reps; i++)
+        hipLaunchKernelGGL(k_probe_copy, dim3(copy_blocks), dim3(256), 0, s, src.p, dst.p, n4);
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    h_out[0] = 2.0 * (double)n4 * sizeof(double4) * reps / (ms * 1e-3) / 1e9;
+    hipLaunchKernelGGL(k_probe_fma, dim3(fma_blocks), dim3(256), 0, s, out.p, fma_iters, 0.999999, 1e-7);
+    HIPCHK(hipEventRecord(e0, s));
+    for (int i = 0; i < reps; i++)
+        hipLaunchKernelGGL(k_probe_fma, dim3(fma_blocks), dim3(256), 0, s, out.p, fma_iters, 0.999999, 1e-7);
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    const double flops = 2.0 * PROBE_CHAINS * (double)fma_iters * fma_blocks * 256.0 * reps;
+    h_out[1] = flops / (ms * 1e-3) / 1e12;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    HIPCHK(hipGetLastError());
+    return KMC_OK;
+}
+
 int kmc_eval_field(kmc_ctx* c, void* stream, int64_t n, const double* d_points, double* d_out) {
     if (check_ctx(c)) return KMC_E_ARG;
     if (!c->initialised) return set_err(KMC_E_STATE, "Run initialise before this call");

@@ -307,6 +307,13 @@ int kmc_sweep_hop(kmc_ctx* ctx, void* stream, double q, const double* h_old_pos,
  */
 int kmc_time_kernel(kmc_ctx* ctx, void* stream, int32_t which, int32_t reps, int32_t precision, double* h_ms);
 
+/* Measurement hook (no reference counterpart): the roofline denominators measured on
+ * the device, beside the published peaks.  Allocates two buffers of
+ * `bytes`, times `reps` launches of a streaming copy and of an FP64 FMA loop on
+ * `stream`; h_out[0] = copy GB/s (read + write bytes / time), h_out[1] = FP64
+ * TFLOP/s (FMA = 2 flops).  bytes in [1 MiB, 16 GiB]. */
+int kmc_probe_peaks(void* stream, int64_t bytes, int32_t reps, double* h_out);
+
 /*
  * Batches of independent systems (no reference counterpart: the reference steps one
  * system per process).  The step of every system in the batch is one launch per kernel
