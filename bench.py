@@ -318,7 +318,8 @@ def main():
     ap.add_argument("--steps", type=int, default=0, help="0: the workload's default")
     ap.add_argument("--warmup", type=int, default=-1, help="-1: the workload's default")
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS) + ["batch-256"])
-    ap.add_argument("--batch-streams", type=int, default=1, help="streams of the batched workload (1)")
+    ap.add_argument("--batch-streams", type=int, default=6,
+                    help="streams of the batched workload: per-group launches on streams - 1 side streams (6)")
     ap.add_argument("--batch-mode", default="kernels", choices=("kernels", "graph"),
                     help="batched kernels (one launch per kernel kind) or per-system steps in one graph")
     ap.add_argument("--graph-steps", type=int, default=10, help="steps captured per HIP graph")

@@ -17,10 +17,13 @@ Metropolis and accepted.  Two ways to run it:
   system's tiles from the autotuner (per order / leaf count / proposal count class).
 
 Either way ``capture`` records ``steps_per_graph`` batch steps into one HIP graph, a
-batch step is one graph launch from the host.  The systems are recorded on one
-stream: recording them on extra torch side streams (forked from and joined to the
-capturing stream) ended in a host crash inside graph instantiation on this
-ROCm/PyTorch stack, so that option raises.
+batch step is one graph launch from the host.  In kernels mode ``streams=k`` sends
+the per-group launches (far field, long range, sweep and cell-table tail of each
+(order, periodicity) group) to k - 1 library-owned side streams, forked from and
+joined to the step's stream with events inside kmc_batch_step, so the groups
+overlap.  Graph mode records the systems on one stream: recording them on extra
+torch side streams (forked from and joined to the capturing stream) ended in a host
+crash inside graph instantiation on this ROCm/PyTorch stack, so that option raises.
 """
 
 import ctypes
@@ -62,11 +65,15 @@ class BatchSystem:
 
 class KMCBatch:
     def __init__(self, systems, kT=1.0, precision="mixed", streams=1, mode="kernels"):
-        if int(streams) != 1:
-            raise NotImplementedError("batches are recorded on one stream (extra side streams crashed graph "
-                                      "instantiation on this ROCm/PyTorch stack)")
         if mode not in ("kernels", "graph"):
             raise ValueError("mode must be 'kernels' or 'graph'")
+        if mode == "graph" and int(streams) != 1:
+            raise NotImplementedError("graph-mode batches are recorded on one stream (recording the systems on "
+                                      "extra torch side streams crashed graph instantiation on this ROCm/PyTorch "
+                                      "stack)")
+        if not 1 <= int(streams) <= 16:
+            raise ValueError("streams must be in [1, 16]")
+        self.streams = int(streams)
         self.systems = list(systems)
         self.kT = float(kT)
         self.precision = precision
@@ -81,6 +88,8 @@ class KMCBatch:
             h = ctypes.c_void_p()
             _native.check(lib.kmc_batch_create(ctypes.byref(h), arr, n), "kmc_batch_create")
             self._batch = h
+            if self.streams > 1:
+                _native.check(lib.kmc_batch_set_streams(h, self.streams), "kmc_batch_set_streams")
             for i, s in enumerate(self.systems):
                 _native.check(lib.kmc_batch_set_system(
                     h, i, s.uniforms.numel(), s.P, s.k, s.pids_all.data_ptr(), s.disp_all.data_ptr(),

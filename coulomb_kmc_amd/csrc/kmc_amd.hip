@@ -4559,8 +4559,18 @@ struct kmc_batch {
     DBuf<EvalArgs> nearA;
     DBuf<CombineArgs> comb;
     DBuf<SelectArgs> sel;
+    // side streams for the per-group launches (far field, long range, sweep + tail of
+    // one (order, periodicity) group each), forked from and joined back to the step's
+    // stream with events so that a captured step keeps the dependencies
+    std::vector<hipStream_t> side;
+    hipEvent_t ev_fork[2] = {nullptr, nullptr};
+    std::vector<hipEvent_t> ev_join;   // two per side stream (far/long-range phase, sweep phase)
     ~kmc_batch() {
         for (Group* g : groups) delete g;
+        for (hipStream_t x : side) hipStreamDestroy(x);
+        for (hipEvent_t e : ev_join) hipEventDestroy(e);
+        for (hipEvent_t e : ev_fork)
+            if (e) hipEventDestroy(e);
     }
 };
 
@@ -4584,6 +4594,32 @@ int kmc_batch_create(kmc_batch** out, kmc_ctx* const* ctxs, int32_t n) {
 
 int kmc_batch_destroy(kmc_batch* b) {
     delete b;
+    return KMC_OK;
+}
+
+int kmc_batch_set_streams(kmc_batch* b, int32_t nstreams) {
+    if (!b) return set_err(KMC_E_ARG, "null batch");
+    if (nstreams < 1 || nstreams > 16) return set_err(KMC_E_ARG, "nstreams must be in [1, 16]");
+    while ((int)b->side.size() > nstreams - 1) {
+        HIPCHK(hipStreamDestroy(b->side.back()));
+        b->side.pop_back();
+        for (int k = 0; k < 2; k++) {
+            HIPCHK(hipEventDestroy(b->ev_join.back()));
+            b->ev_join.pop_back();
+        }
+    }
+    for (int i = 0; i < 2; i++)
+        if (!b->ev_fork[i]) HIPCHK(hipEventCreateWithFlags(&b->ev_fork[i], hipEventDisableTiming));
+    while ((int)b->side.size() < nstreams - 1) {
+        hipStream_t x = nullptr;
+        hipEvent_t e = nullptr;
+        HIPCHK(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+        b->side.push_back(x);
+        for (int k = 0; k < 2; k++) {   // one join event per side stream and phase
+            HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            b->ev_join.push_back(e);
+        }
+    }
     return KMC_OK;
 }
 
@@ -4731,6 +4767,28 @@ static void launch_batch_near(int G, int max_ne, int n, const EvalArgs* args, hi
     }
 }
 
+// launches fn for every group: group i on side stream (i - 1) mod nside when side
+// streams are set, the first group and the rest on s; the side streams wait for s
+// (event fork) and s waits for them (event join) before the next launch on s
+template <class F>
+static int batch_groups(kmc_batch* b, hipStream_t s, int phase, F fn) {
+    const int ns = (int)b->side.size();
+    if (ns == 0 || b->groups.size() < 2) {
+        for (kmc_batch::Group* g : b->groups) fn(g, s);
+        return KMC_OK;
+    }
+    HIPCHK(hipEventRecord(b->ev_fork[phase], s));
+    const int used = std::min(ns, (int)b->groups.size() - 1);
+    for (int i = 0; i < used; i++) HIPCHK(hipStreamWaitEvent(b->side[i], b->ev_fork[phase], 0));
+    for (size_t gi = 0; gi < b->groups.size(); gi++)
+        fn(b->groups[gi], gi == 0 ? s : b->side[(gi - 1) % used]);
+    for (int i = 0; i < used; i++) {
+        HIPCHK(hipEventRecord(b->ev_join[2 * i + phase], b->side[i]));
+        HIPCHK(hipStreamWaitEvent(s, b->ev_join[2 * i + phase], 0));
+    }
+    return KMC_OK;
+}
+
 int kmc_batch_step(kmc_batch* b, void* stream) {
     if (!b) return set_err(KMC_E_ARG, "null batch");
     if (!b->ready) return set_err(KMC_E_STATE, "call kmc_batch_prepare before kmc_batch_step");
@@ -4747,19 +4805,21 @@ int kmc_batch_step(kmc_batch* b, void* stream) {
     hipLaunchKernelGGL(kb_setup, dim3((std::max(a.max_T, a.max_P) + 255) / 256, 1, n), dim3(256), 0, s, b->setup.p);
     if (b->mixed) launch_batch_near<true>(b->near_lanes, a.max_ne, n, b->nearA.p, s);
     else launch_batch_near<false>(b->near_lanes, a.max_ne, n, b->nearA.p, s);
-    for (kmc_batch::Group* g : b->groups) {
-        g->fns.bfar(g->far.p, g->shape, s);
-        if (g->lr_shape.n > 0) g->fns.blr(g->lr.p, g->lr_shape, s);
-    }
+    int rc = batch_groups(b, s, 0, [](kmc_batch::Group* g, hipStream_t t) {
+        g->fns.bfar(g->far.p, g->shape, t);
+        if (g->lr_shape.n > 0) g->fns.blr(g->lr.p, g->lr_shape, t);
+    });
+    if (rc) return rc;
     if (b->mixed)
         hipLaunchKernelGGL(kb_combine<true>, dim3((a.max_T + 63) / 64, 1, n), dim3(64), 0, s, b->comb.p);
     else
         hipLaunchKernelGGL(kb_combine<false>, dim3((a.max_T + 63) / 64, 1, n), dim3(64), 0, s, b->comb.p);
     hipLaunchKernelGGL(kb_select, dim3(1, 1, n), dim3(1024), 0, s, b->sel.p);
-    for (kmc_batch::Group* g : b->groups) {
-        g->fns.bsweep(g->sweep.p, g->shape, g->per, s);
-        g->fns.btail(g->tail.p, g->shape, s);
-    }
+    rc = batch_groups(b, s, 1, [](kmc_batch::Group* g, hipStream_t t) {
+        g->fns.bsweep(g->sweep.p, g->shape, g->per, t);
+        g->fns.btail(g->tail.p, g->shape, t);
+    });
+    if (rc) return rc;
     HIPCHK(hipGetLastError());
     return KMC_OK;
 }

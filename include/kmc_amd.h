@@ -336,6 +336,13 @@ int kmc_batch_set_system(kmc_batch* batch, int32_t i, int64_t nsteps, int64_t P,
                          int64_t* d_pid, int64_t* d_es, double* d_new_pos, double* d_dE);
 int kmc_batch_prepare(kmc_batch* batch, double kT, int32_t precision);
 int kmc_batch_step(kmc_batch* batch, void* stream);
+/* Side streams of a batch (no reference counterpart): with nstreams > 1, the launches
+ * that run once per (expansion order, periodicity) group -- far field, long range,
+ * accept sweep and cell-table tail -- go to nstreams - 1 library-owned side streams
+ * (default priority) forked from and joined back to the step's stream with events, so
+ * the groups overlap and a captured step keeps its dependencies.  1 (the default):
+ * every launch on the step's stream.  nstreams in [1, 16]; call outside capture. */
+int kmc_batch_set_streams(kmc_batch* batch, int32_t nstreams);
 
 #ifdef __cplusplus
 }

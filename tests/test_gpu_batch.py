@@ -88,4 +88,12 @@ def test_batch_graph_on_side_streams():
 def test_batch_rejects_side_streams():
     from coulomb_kmc_amd.batch import KMCBatch
     with pytest.raises(NotImplementedError):
-        KMCBatch([], streams=2)
+        KMCBatch([], streams=2, mode="graph")
+    with pytest.raises(ValueError):
+        KMCBatch([], streams=17)
+
+
+@pytest.mark.parametrize("streams", [2, 4])
+def test_batched_kernels_on_side_streams_equal_single_system_steps(streams):
+    # the per-group launches on library side streams (event fork / join inside the step)
+    _check_batch(streams, "kernels", 8)
