@@ -33,11 +33,11 @@ METRIC = "Î”E proposals/s + accepts/s on 1 GPU, mixed vs FP64, at Î”E rel err â‰
 # workloads (r = levels, leaf grid 2^(r-1) per side); steps/warmup are the defaults
 WORKLOADS = {
     "free-1k": dict(workload="free-space N=1000 box 10 leaves 8^3 L=12, 64x4 proposals", N=1000, E=10.0, r=4,
-                    L=12, bc="free_space", P=64, k=4, radius=1.0, seed=1000, steps=100, warmup=10),
+                    L=12, bc="free_space", P=64, k=4, radius=1.0, seed=1000, steps=100, warmup=10, pipeline=True),
     "pbc-100k": dict(workload="pbc N=100000 box 100 leaves 32^3 L=12, 1024x8 proposals", N=100000, E=100.0, r=6,
-                     L=12, bc="pbc", P=1024, k=8, radius=1.5, seed=1001, steps=1000, warmup=50),
+                     L=12, bc="pbc", P=1024, k=8, radius=1.5, seed=1001, steps=1000, warmup=50, pipeline=False),
     "pbc-2m": dict(workload="pbc N=2000000 box 271.4 leaves 64^3 L=16, 32768x8 proposals", N=2000000, E=271.4,
-                   r=7, L=16, bc="pbc", P=32768, k=8, radius=1.5, seed=1004, steps=100, warmup=20),
+                   r=7, L=16, bc="pbc", P=32768, k=8, radius=1.5, seed=1004, steps=100, warmup=20, pipeline=False),
 }
 DEFAULT_WORKLOAD = "pbc-2m"
 
@@ -326,6 +326,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--near-lanes", type=int, default=0, help="0: autotuned")
+    ap.add_argument("--pipeline", default="auto", choices=("auto", "on", "off"),
+                    help="on: kmc_run_steps with each step's sweep beside the next step's near field and long "
+                         "range; off: kmc_materialize_moves + kmc_step per step; auto: the workload's choice "
+                         "(on for free-1k, where it measured faster; off for pbc-100k and pbc-2m, where the "
+                         "overlapped kernels slowed each other)")
     ap.add_argument("--probe-traffic", action="store_true",
                     help="only launch the accept sweep and a copy of known size (for rocprofv3 --pmc runs)")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
@@ -481,6 +486,7 @@ def main():
         log("cpu baseline", cpu)
 
     # ---- timed runs: graph of G steps, replayed
+    pipeline = cfg["pipeline"] if args.pipeline == "auto" else args.pipeline == "on"
     stream = torch.cuda.Stream()
     results = {}
     for prec in ("fp64", "mixed"):
@@ -492,9 +498,15 @@ def main():
         stream.synchronize()
         graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(graph, stream=stream):
-            for _ in range(G):
-                kmc.materialize_moves(P, k, t_pids, t_disp, counter, d_pid, d_es, d_np)
-                kmc.step_device(d_pid, d_es, d_np, d_dE, t_u, 1.0, counter, sel, precision=prec)
+            if not pipeline:
+                for _ in range(G):
+                    kmc.materialize_moves(P, k, t_pids, t_disp, counter, d_pid, d_es, d_np)
+                    kmc.step_device(d_pid, d_es, d_np, d_dE, t_u, 1.0, counter, sel, precision=prec)
+            else:
+                # G steps in one call: each step's accept sweep beside the next step's
+                # set-up, near field and long range
+                kmc.run_steps(G, P, k, t_pids, t_disp, t_u, 1.0, counter, sel, d_pid, d_es, d_np, d_dE,
+                              precision=prec, pipelined=True, stream=stream.cuda_stream)
         counter.fill_(0)
         torch.cuda.synchronize()
         with torch.cuda.stream(stream):
@@ -588,7 +600,8 @@ def main():
         "data": "synthetic (seeded uniform charges +-1, ball-displacement proposals, seeded Metropolis uniforms)",
         "config": {"workload": cfg["workload"], "N": N, "box": E, "leaves_per_side": 2 ** (r - 1), "r": r,
                    "L": L, "bc": bc, "proposals_per_step": T, "moving_charges_per_step": P,
-                   "ball_radius": cfg["radius"], "kT": 1.0, "graph_steps": G, "tiles": tiles},
+                   "ball_radius": cfg["radius"], "kT": 1.0, "graph_steps": G, "tiles": tiles,
+                   "pipelined": pipeline},
         "accepts_per_s": mixed["accepts_per_s"],
         "mixed": mixed,
         "fp64": fp64,

@@ -39,7 +39,7 @@ EXPORTS = (
     "kmc_get_particles", "kmc_get_energy", "kmc_set_energy", "kmc_ncells", "kmc_propose",
     "kmc_propose_with_dats", "kmc_accept", "kmc_step", "kmc_materialize_moves",
     "kmc_check_flags", "kmc_set_tiles", "kmc_sweep_hop", "kmc_time_kernel", "kmc_probe_peaks", "kmc_eval_field",
-    "kmc_batch_create", "kmc_batch_destroy", "kmc_batch_set_system", "kmc_batch_prepare", "kmc_batch_step", "kmc_batch_set_streams",
+    "kmc_batch_create", "kmc_batch_destroy", "kmc_batch_set_system", "kmc_batch_prepare", "kmc_batch_step", "kmc_batch_set_streams", "kmc_run_steps",
     "kmc_particle_count", "kmc_set_particle_count", "kmc_charge_energies", "kmc_group_energy",
     "kmc_propose_extract", "kmc_propose_inject", "kmc_extract", "kmc_inject", "kmc_discover_sites",
     "kmc_lattice_step", "kmc_accept_movedata",
@@ -117,6 +117,7 @@ def lib():
         "kmc_batch_prepare": (ctypes.c_int, [P, D, I32]),
         "kmc_batch_step": (ctypes.c_int, [P, P]),
         "kmc_batch_set_streams": (ctypes.c_int, [P, I32]),
+        "kmc_run_steps": (ctypes.c_int, [P, P, I64, I64, I64, I64, P, P, I64, P, D, P, P, P, P, P, P, I32, I32]),
         "kmc_set_tiles": (ctypes.c_int, [P, I32, I32, I32, I32]),
         "kmc_sweep_hop": (ctypes.c_int, [P, P, D, P, P]),
         "kmc_time_kernel": (ctypes.c_int, [P, P, I32, I32, I32, P]),

@@ -2850,6 +2850,10 @@ struct kmc_ctx {
     // joined back to the caller's stream with events (graph-capturable)
     hipStream_t side = nullptr;
     hipEvent_t ev_fork[2] = {nullptr, nullptr}, ev_join[2] = {nullptr, nullptr};
+    // second side stream for the accept sweep of a pipelined run of steps: the sweep of
+    // step i runs beside the set-up, near field and long range of step i + 1
+    hipStream_t side2 = nullptr;
+    hipEvent_t ev_fork2 = nullptr, ev_sweep = nullptr;
     bool have_particles = false, have_lr = false, initialised = false;
     // the host mirrors below lag the device after device-side steps
     bool mirrors_stale = false;
@@ -3095,7 +3099,7 @@ static int join_side(kmc_ctx* c, hipStream_t s, int i) {
 
 static int run_propose_core(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, double* d_out, const int64_t* rate_loc,
                             bool mixed, double kT = 0.0, double* rates = nullptr, const int64_t* mask = nullptr,
-                            int rate_rule = 0) {
+                            int rate_rule = 0, bool wait_sweep = false) {
     const int ne = (int)(T + P);
     c->last_P = P;
     c->last_T = T;
@@ -3110,6 +3114,8 @@ static int run_propose_core(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, dou
     if (c->bc != KMC_BC_FF_ONLY) {
         EvalArgs an = eval_args(c, ne, c->epos4.p, c->ecell.p, c->eexcl.p, c->UD.p);
         launch_near_prop(an, c->near_lanes, mixed, (int)P, (int)T, c->es_copy.p, s);
+        // the far field reads the leaf expansions: after the previous step's sweep
+        if (wait_sweep) HIPCHK(hipStreamWaitEvent(s, c->ev_sweep, 0));
         EvalArgs af = eval_args(c, ne, c->epos4.p, c->ecell.p, c->eexcl.p, c->UI.p);
         c->fns.far(af, s);
     }
@@ -3137,7 +3143,7 @@ static int launch_setup(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, const i
     return KMC_OK;
 }
 
-static int run_accept_tail(kmc_ctx* c, hipStream_t s, int64_t* d_counter) {
+static int run_accept_tail(kmc_ctx* c, hipStream_t s, int64_t* d_counter, bool defer_sweep = false) {
     const TailArgs ta = tail_args(c, d_counter);
     if (c->bc == KMC_BC_FF_ONLY) {
         c->fns.md(ta, s);
@@ -3150,7 +3156,16 @@ static int run_accept_tail(kmc_ctx* c, hipStream_t s, int64_t* d_counter) {
     if (rc) return rc;
     c->fns.md(ta, c->side);
     const SweepArgs sa = sweep_rec_args(c);
-    c->fns.sweep(sa, (int)c->ncells, s, c->sweep_block, c->sweep_pairs, c->sweep_split);
+    if (defer_sweep) {
+        // the sweep on side2; s waits only for the tail, and the next step's far field
+        // waits for ev_sweep (the caller joins ev_sweep into s after the last step)
+        HIPCHK(hipEventRecord(c->ev_fork2, s));
+        HIPCHK(hipStreamWaitEvent(c->side2, c->ev_fork2, 0));
+        c->fns.sweep(sa, (int)c->ncells, c->side2, c->sweep_block, c->sweep_pairs, c->sweep_split);
+        HIPCHK(hipEventRecord(c->ev_sweep, c->side2));
+    } else {
+        c->fns.sweep(sa, (int)c->ncells, s, c->sweep_block, c->sweep_pairs, c->sweep_split);
+    }
     rc = join_side(c, s, 1);
     if (rc) return rc;
     HIPCHK(hipGetLastError());
@@ -3386,7 +3401,11 @@ int kmc_create(kmc_ctx** out, int64_t n, double extent, int32_t r, int32_t L, in
         rc = set_err(KMC_E_HIP, "device allocation failed");
     }
     if (rc == KMC_OK) {
-        if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) rc = set_err(KMC_E_HIP, "stream");
+        if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+            hipStreamCreateWithFlags(&c->side2, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&c->ev_fork2, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&c->ev_sweep, hipEventDisableTiming) != hipSuccess)
+            rc = set_err(KMC_E_HIP, "stream");
         for (int i = 0; i < 2 && rc == KMC_OK; i++)
             if (hipEventCreateWithFlags(&c->ev_fork[i], hipEventDisableTiming) != hipSuccess ||
                 hipEventCreateWithFlags(&c->ev_join[i], hipEventDisableTiming) != hipSuccess)
@@ -3424,6 +3443,9 @@ int kmc_destroy(kmc_ctx* c) {
         if (c->ev_join[i]) hipEventDestroy(c->ev_join[i]);
     }
     if (c->side) hipStreamDestroy(c->side);
+    if (c->ev_fork2) hipEventDestroy(c->ev_fork2);
+    if (c->ev_sweep) hipEventDestroy(c->ev_sweep);
+    if (c->side2) hipStreamDestroy(c->side2);
     delete c;
     return KMC_OK;
 }
@@ -4001,6 +4023,39 @@ int kmc_materialize_moves(kmc_ctx* c, void* stream, int64_t nsteps, int64_t P, i
     const int64_t threads = std::max(T, P + 1);
     MatArgs ma = mat_args(c, nsteps, P, k, d_pids_all, d_disp_all, d_counter, d_pid, d_es, d_new_pos);
     hipLaunchKernelGGL(k_materialize, dim3((threads + 255) / 256), dim3(256), 0, s, ma);
+    HIPCHK(hipGetLastError());
+    return KMC_OK;
+}
+
+int kmc_run_steps(kmc_ctx* c, void* stream, int64_t nrun, int64_t nsteps_moves, int64_t P, int64_t k,
+                  const int64_t* d_pids_all, const double* d_disp_all, int64_t nsteps_uniform,
+                  const double* d_uniform, double kT, int64_t* d_counter, int64_t* d_selected, int64_t* d_pid,
+                  int64_t* d_es, double* d_new_pos, double* d_dE, int32_t precision, int32_t pipelined) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    if (!c->initialised) return set_err(KMC_E_STATE, "Run initialise before this call");
+    if (nrun < 0 || P < 1 || k < 1 || nsteps_moves < 1 || nsteps_uniform < 1 || P * k >= (int64_t)1 << 30)
+        return set_err(KMC_E_ARG, "bad counts");
+    const int64_t T = P * k;
+    if (!step_sized(c, P, T))
+        return set_err(KMC_E_STATE, "work buffers not sized: call kmc_propose once with this T before kmc_run_steps");
+    hipStream_t s = (hipStream_t)stream;
+    const bool pipe = pipelined != 0 && c->bc != KMC_BC_FF_ONLY;
+    const bool mixed = precision == KMC_PREC_MIXED;
+    const int64_t mthreads = std::max(T, P + 1);
+    const MatArgs ma = mat_args(c, nsteps_moves, P, k, d_pids_all, d_disp_all, d_counter, d_pid, d_es, d_new_pos);
+    const SelectArgs sa = select_args(c, P, T, d_pid, d_dE, nsteps_uniform, d_uniform, kT, d_counter, d_selected);
+    for (int64_t it = 0; it < nrun; it++) {
+        hipLaunchKernelGGL(k_materialize, dim3((mthreads + 255) / 256), dim3(256), 0, s, ma);
+        int rc = launch_setup(c, s, P, T, d_pid, d_es, d_new_pos, nullptr);
+        if (rc) return rc;
+        rc = run_propose_core(c, s, P, T, d_dE, nullptr, mixed, kT, c->rates.p, nullptr, 0, pipe && it > 0);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, s, sa);
+        rc = run_accept_tail(c, s, d_counter, pipe);
+        if (rc) return rc;
+    }
+    if (pipe && nrun > 0) HIPCHK(hipStreamWaitEvent(s, c->ev_sweep, 0));
+    c->mirrors_stale = true;
     HIPCHK(hipGetLastError());
     return KMC_OK;
 }

@@ -177,6 +177,21 @@ int kmc_materialize_moves(kmc_ctx* ctx, void* stream, int64_t nsteps, int64_t P,
                           int64_t* d_pid, int64_t* d_es, double* d_new_pos);
 
 /*
+ * nrun steps of the synthetic move stream in one call (no reference counterpart):
+ * each is kmc_materialize_moves (nsteps_moves table rows) followed by kmc_step
+ * (nsteps_uniform uniforms), with the same kernels and the same results.  With
+ * pipelined != 0 (not for ff-only) the accept sweep of step i runs on a second side
+ * stream beside the set-up, near field and long range of step i + 1, which read no
+ * leaf expansion; step i + 1's far field waits for it, and the call's stream waits
+ * for the last sweep before returning (so a captured call is joined).  Needs the work
+ * buffers sized by a kmc_propose of the same T.
+ */
+int kmc_run_steps(kmc_ctx* ctx, void* stream, int64_t nrun, int64_t nsteps_moves, int64_t P, int64_t k,
+                  const int64_t* d_pids_all, const double* d_disp_all, int64_t nsteps_uniform,
+                  const double* d_uniform, double kT, int64_t* d_counter, int64_t* d_selected, int64_t* d_pid,
+                  int64_t* d_es, double* d_new_pos, double* d_dE, int32_t precision, int32_t pipelined);
+
+/*
  * KMCFMM.eval_field (coulomb_kmc/kmc_fmm.py:544-567): potential of all charges at n
  * points d_points[n][3] (device) into d_out[n]: near field over the 27 leaves around
  * the point's leaf (coulomb_kmc/kmc_local.py:1085-1182), that leaf's local expansion
