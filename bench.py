@@ -33,11 +33,11 @@ METRIC = "Î”E proposals/s + accepts/s on 1 GPU, mixed vs FP64, at Î”E rel err â‰
 # workloads (r = levels, leaf grid 2^(r-1) per side); steps/warmup are the defaults
 WORKLOADS = {
     "free-1k": dict(workload="free-space N=1000 box 10 leaves 8^3 L=12, 64x4 proposals", N=1000, E=10.0, r=4,
-                    L=12, bc="free_space", P=64, k=4, radius=1.0, seed=1000, steps=100, warmup=10, pipeline=True),
+                    L=12, bc="free_space", P=64, k=4, radius=1.0, seed=1000, steps=100, warmup=10, pipeline=1),
     "pbc-100k": dict(workload="pbc N=100000 box 100 leaves 32^3 L=12, 1024x8 proposals", N=100000, E=100.0, r=6,
-                     L=12, bc="pbc", P=1024, k=8, radius=1.5, seed=1001, steps=1000, warmup=50, pipeline=False),
+                     L=12, bc="pbc", P=1024, k=8, radius=1.5, seed=1001, steps=1000, warmup=50, pipeline=2),
     "pbc-2m": dict(workload="pbc N=2000000 box 271.4 leaves 64^3 L=16, 32768x8 proposals", N=2000000, E=271.4,
-                   r=7, L=16, bc="pbc", P=32768, k=8, radius=1.5, seed=1004, steps=100, warmup=20, pipeline=False),
+                   r=7, L=16, bc="pbc", P=32768, k=8, radius=1.5, seed=1004, steps=100, warmup=20, pipeline=0),
 }
 DEFAULT_WORKLOAD = "pbc-2m"
 
@@ -326,11 +326,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--near-lanes", type=int, default=0, help="0: autotuned")
-    ap.add_argument("--pipeline", default="auto", choices=("auto", "on", "off"),
+    ap.add_argument("--pipeline", default="auto", choices=("auto", "on", "off", "setup"),
                     help="on: kmc_run_steps with each step's sweep beside the next step's near field and long "
-                         "range; off: kmc_materialize_moves + kmc_step per step; auto: the workload's choice "
-                         "(on for free-1k, where it measured faster; off for pbc-100k and pbc-2m, where the "
-                         "overlapped kernels slowed each other)")
+                         "range; setup: beside its set-up and long range only; off: kmc_materialize_moves + "
+                         "kmc_step per step; auto: the workload's choice (on for free-1k, setup for pbc-100k, "
+                         "where they measured fastest; off for pbc-2m, where neither gained)")
     ap.add_argument("--probe-traffic", action="store_true",
                     help="only launch the accept sweep and a copy of known size (for rocprofv3 --pmc runs)")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
@@ -486,7 +486,7 @@ def main():
         log("cpu baseline", cpu)
 
     # ---- timed runs: graph of G steps, replayed
-    pipeline = cfg["pipeline"] if args.pipeline == "auto" else args.pipeline == "on"
+    pipeline = {"auto": cfg["pipeline"], "on": 1, "off": 0, "setup": 2}[args.pipeline]
     stream = torch.cuda.Stream()
     results = {}
     for prec in ("fp64", "mixed"):
@@ -506,7 +506,7 @@ def main():
                 # G steps in one call: each step's accept sweep beside the next step's
                 # set-up, near field and long range
                 kmc.run_steps(G, P, k, t_pids, t_disp, t_u, 1.0, counter, sel, d_pid, d_es, d_np, d_dE,
-                              precision=prec, pipelined=True, stream=stream.cuda_stream)
+                              precision=prec, pipelined=int(pipeline), stream=stream.cuda_stream)
         counter.fill_(0)
         torch.cuda.synchronize()
         with torch.cuda.stream(stream):

@@ -3099,7 +3099,7 @@ static int join_side(kmc_ctx* c, hipStream_t s, int i) {
 
 static int run_propose_core(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, double* d_out, const int64_t* rate_loc,
                             bool mixed, double kT = 0.0, double* rates = nullptr, const int64_t* mask = nullptr,
-                            int rate_rule = 0, bool wait_sweep = false) {
+                            int rate_rule = 0, int wait_sweep = 0) {
     const int ne = (int)(T + P);
     c->last_P = P;
     c->last_T = T;
@@ -3112,10 +3112,13 @@ static int run_propose_core(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, dou
         c->fns.lr(lr_args(c, P, T), c->side);
     }
     if (c->bc != KMC_BC_FF_ONLY) {
+        // wait_sweep 2: the near field too waits for the previous step's sweep (only the
+        // set-up and the long range overlap it)
+        if (wait_sweep == 2) HIPCHK(hipStreamWaitEvent(s, c->ev_sweep, 0));
         EvalArgs an = eval_args(c, ne, c->epos4.p, c->ecell.p, c->eexcl.p, c->UD.p);
         launch_near_prop(an, c->near_lanes, mixed, (int)P, (int)T, c->es_copy.p, s);
         // the far field reads the leaf expansions: after the previous step's sweep
-        if (wait_sweep) HIPCHK(hipStreamWaitEvent(s, c->ev_sweep, 0));
+        if (wait_sweep == 1) HIPCHK(hipStreamWaitEvent(s, c->ev_sweep, 0));
         EvalArgs af = eval_args(c, ne, c->epos4.p, c->ecell.p, c->eexcl.p, c->UI.p);
         c->fns.far(af, s);
     }
@@ -4048,7 +4051,8 @@ int kmc_run_steps(kmc_ctx* c, void* stream, int64_t nrun, int64_t nsteps_moves, 
         hipLaunchKernelGGL(k_materialize, dim3((mthreads + 255) / 256), dim3(256), 0, s, ma);
         int rc = launch_setup(c, s, P, T, d_pid, d_es, d_new_pos, nullptr);
         if (rc) return rc;
-        rc = run_propose_core(c, s, P, T, d_dE, nullptr, mixed, kT, c->rates.p, nullptr, 0, pipe && it > 0);
+        rc = run_propose_core(c, s, P, T, d_dE, nullptr, mixed, kT, c->rates.p, nullptr, 0,
+                              (pipe && it > 0) ? (pipelined == 2 ? 2 : 1) : 0);
         if (rc) return rc;
         hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, s, sa);
         rc = run_accept_tail(c, s, d_counter, pipe);

@@ -416,8 +416,9 @@ class KMCFMM(InjectorExtractor):
 
     def run_steps(self, nrun, P, k, d_pids_all, d_disp_all, d_uniform, kT, d_counter, d_selected, d_pid, d_es,
                   d_new_pos, d_dE, precision=None, pipelined=True, stream=None):
-        """nrun x (materialize_moves + step_device) in one call; pipelined: the accept sweep
-        of a step overlaps the next step's set-up, near field and long range."""
+        """nrun x (materialize_moves + step_device) in one call; pipelined (True or 1): the
+        accept sweep of a step overlaps the next step's set-up, near field and long range;
+        2: only its set-up and long range."""
         s = stream if stream is not None else self._stream()
         nsteps = d_pids_all.numel() // int(P)
         assert d_disp_all.numel() >= nsteps * int(P) * int(k) * 3
@@ -429,7 +430,7 @@ class KMCFMM(InjectorExtractor):
             self._ctx, s, int(nrun), nsteps, int(P), int(k), d_pids_all.data_ptr(), d_disp_all.data_ptr(),
             d_uniform.numel(), d_uniform.data_ptr(), float(kT), d_counter.data_ptr(), d_selected.data_ptr(),
             d_pid.data_ptr(), d_es.data_ptr(), d_new_pos.data_ptr(), d_dE.data_ptr(), prec,
-            1 if pipelined else 0), "kmc_run_steps")
+            int(pipelined)), "kmc_run_steps")
 
     def materialize_moves(self, P, k, d_pids_all, d_disp_all, d_counter, d_pid, d_es, d_new_pos, stream=None):
         s = stream if stream is not None else self._stream()

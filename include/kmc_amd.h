@@ -183,8 +183,9 @@ int kmc_materialize_moves(kmc_ctx* ctx, void* stream, int64_t nsteps, int64_t P,
  * pipelined != 0 (not for ff-only) the accept sweep of step i runs on a second side
  * stream beside the set-up, near field and long range of step i + 1, which read no
  * leaf expansion; step i + 1's far field waits for it, and the call's stream waits
- * for the last sweep before returning (so a captured call is joined).  Needs the work
- * buffers sized by a kmc_propose of the same T.
+ * for the last sweep before returning (so a captured call is joined).  pipelined = 2:
+ * the near field waits for the sweep too (only set-up and long range overlap it).
+ * Needs the work buffers sized by a kmc_propose of the same T.
  */
 int kmc_run_steps(kmc_ctx* ctx, void* stream, int64_t nrun, int64_t nsteps_moves, int64_t P, int64_t k,
                   const int64_t* d_pids_all, const double* d_disp_all, int64_t nsteps_uniform,

@@ -72,7 +72,7 @@ def test_run_steps_equal_single_steps(bc):
                           ref_w["sel"])
     ref = _state(ref_k, ref_w)
     assert ref[0] == nrun
-    for pipelined in (False, True):
+    for pipelined in (0, 1, 2):
         kk, w = make()
         kk.run_steps(nrun, P, k, tabs["pids"], tabs["disp"], tabs["us"], 1.0, w["counter"], w["sel"], w["pid"],
                      w["es"], w["new"], w["dE"], pipelined=pipelined)
