@@ -56,6 +56,41 @@ def sweep_flops_per_source(L):
     return 9 * L * (L + 1) // 2 + 6 * (L - 1) + 30
 
 
+def kernel_rooflines(kt, N, ncells, L, P, T, bc, sweep_flops, sweep_bytes):
+    """Per-kernel achieved rate against the attainable one, min(FP64 peak, AI x HBM peak),
+    from the algorithmic work per launch (add/mul 1, FMA 2, rsqrt 1):
+    near: 10 flops per pair, 27 rho pairs per evaluation point (T + P points, rho = N /
+    leaves); bytes 32 per point in + 8 out, and 32 per charge of the leaves the points'
+    stencils cover (at most all leaves; taken as min(leaves, 27 P) leaves);
+    far: 9 nh flops per point; bytes 40 per point and 16 nh per leaf holding a point
+    (taken as min(leaves, P));
+    sweep: as in the roofline line.  The long-range kernels are bound by LDS, not by
+    HBM or the FP64 rate, and are reported as durations only."""
+    nh = L * (L + 1) // 2
+    rho = N / ncells
+    ne = T + P
+    work = {
+        "near": (ne * 10.0 * 27 * rho, ne * 40.0 + min(ncells, 27 * P) * rho * 32.0),
+        "far": (ne * 9.0 * nh, ne * 40.0 + min(ncells, P) * 16.0 * nh),
+        "sweep": (float(sweep_flops), float(sweep_bytes)),
+    }
+    if bc == "ff-only":
+        work.pop("near"), work.pop("far")
+    out = {}
+    for name, (fl, by) in work.items():
+        ms = kt.get(name)
+        if not ms:
+            continue
+        ai = fl / by
+        att = min(FP64_PEAK_TFLOPS, ai * HBM_PEAK_GBS / 1e3)
+        ach = fl / (ms * 1e-3) / 1e12
+        out[name] = {"launch_ms": ms, "flops": fl, "bytes": by, "flop_per_byte": ai,
+                     "bound": "fp64" if att >= FP64_PEAK_TFLOPS else "hbm",
+                     "attainable_TFLOPs": att, "achieved_TFLOPs": ach, "frac": ach / att,
+                     "hbm_GBs": by / (ms * 1e-3) / 1e9}
+    return out
+
+
 def make_system(cfg):
     rng = np.random.default_rng(cfg["seed"])
     N, E = cfg["N"], cfg["E"]
@@ -585,6 +620,7 @@ def main():
     roof["hbm_frac_of_measured"] = ach_gbs / peaks["copy_GBs"]
 
     mixed, fp64 = results["mixed"], results["fp64"]
+    kroof = kernel_rooflines(kt, N, ncells, L, P, T, bc, sweep_flops, sweep_bytes)
     out = {
         "metric": METRIC,
         "value": mixed["proposals_per_s"],
@@ -608,6 +644,7 @@ def main():
         "mixed_vs_fp64": mixed["proposals_per_s"] / fp64["proposals_per_s"],
         "dE_error_mixed_vs_fp64": derr,
         "kernel_ms": kt,
+        "kernel_roofline": kroof,
         "initialise_s": t_init,
         "roofline": roof,
         "cpu_baseline": cpu,
