@@ -37,3 +37,15 @@ def test_missing_library_raises(tmp_path, monkeypatch):
     monkeypatch.setattr(_native, "LIB_PATH", str(tmp_path / "absent.so"))
     with pytest.raises(ImportError):
         _native.lib()
+
+
+def test_probe_peaks_rejects_bad_arguments():
+    # the argument checks run before any device call
+    import ctypes
+    from coulomb_kmc_amd import _native
+    lib = _native.lib()
+    out = (ctypes.c_double * 2)()
+    assert lib.kmc_probe_peaks(None, 1024, 1, out) != 0          # below 1 MiB
+    assert lib.kmc_probe_peaks(None, 1 << 35, 1, out) != 0       # above 16 GiB
+    assert lib.kmc_probe_peaks(None, 1 << 20, 0, out) != 0       # no repetitions
+    assert lib.kmc_probe_peaks(None, 1 << 20, 1, None) != 0      # no output
