@@ -10,9 +10,12 @@ cumulative sum in proposal order, seeded uniform) and accepted (cell tables, eve
 leaf local expansion, the level-0 multipole).  Each step runs entirely on the
 device; the steps are replayed from a captured HIP graph.
 
-Prints one JSON line (see the "value" field: mixed-precision proposals/s of the
-whole step; accepts/s, the FP64 path, the dE error of mixed vs FP64, the roofline of
-the dominant kernel and the host-CPU baseline are in the same line).
+Prints one JSON line.  "value" is proposals/s of whole steps of the headline path:
+the mixed-precision path when both paths hold their dE bars (FP64 1e-10, mixed 1e-8,
+relative with |dE| floored at 1e-3) over all proposals of step 0 against the oracle's
+extended-precision reference on the same state, else the FP64 path ("accuracy" says
+which and why).  Accepts/s, both paths, their errors, the roofline of the dominant
+kernel and the host-CPU baseline are in the same line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload pbc-2m|pbc-100k|free-1k]
 """
@@ -437,38 +440,57 @@ def main():
     d_dE = torch.zeros(T, dtype=torch.float64, device="cuda")
     d_dE2 = torch.zeros(T, dtype=torch.float64, device="cuda")
 
-    # ---- dE error of mixed vs FP64 on the initial state, 65536 proposals (or fewer)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O   # the checker's error measure (and, below, the CPU baseline)
+
+    # ---- dE of mixed vs FP64 on the initial state, 65536 proposals (or fewer)
     nerr = max(1, min(65536 // T, nstream))
-    errs_abs, errs_rel, ref_vals = [], [], []
+    va, vm = [], []
     for b in range(nerr):
         counter.fill_(nstream - 1 - b)   # the last rows of the move stream, 0 <= row < nstream
         kmc.materialize_moves(P, k, t_pids, t_disp, counter, d_pid, d_es, d_np)
         kmc.propose_device(d_pid, d_es, d_np, d_dE, precision="fp64")
         kmc.propose_device(d_pid, d_es, d_np, d_dE2, precision="mixed")
-        a, m = d_dE.cpu().numpy(), d_dE2.cpu().numpy()
-        errs_abs.append(np.abs(m - a))
-        ref_vals.append(np.abs(a))
-    ea = np.concatenate(errs_abs)
-    rv = np.concatenate(ref_vals)
-    er = ea / np.maximum(rv, 1e-300)
-    big = rv >= 1e-6 * np.median(rv)
-    derr = {"proposals": int(ea.size), "max_abs": float(ea.max()), "median_abs": float(np.median(ea)),
-            "max_rel": float(er[big].max()), "median_rel": float(np.median(er[big])),
-            "rel_floor": "proposals with |dE| >= 1e-6 x median |dE| ({} of {})".format(int(big.sum()), ea.size)}
+        va.append(d_dE.cpu().numpy())
+        vm.append(d_dE2.cpu().numpy())
+    derr = O.de_error(np.concatenate(vm), np.concatenate(va))
     log("mixed vs fp64 dE error", derr)
+
+    # ---- step 0's proposals on both GPU paths (checked against the oracle's extended
+    # reference below)
+    counter.fill_(0)
+    kmc.materialize_moves(P, k, t_pids, t_disp, counter, d_pid, d_es, d_np)
+    kmc.propose_device(d_pid, d_es, d_np, d_dE, precision="fp64")
+    kmc.propose_device(d_pid, d_es, d_np, d_dE2, precision="mixed")
+    g64_step0, gmx_step0 = d_dE.cpu().numpy(), d_dE2.cpu().numpy()
+    accuracy = {"measure": "|dE - dE_ref| / max(|dE_ref|, {:g}) (energy units; kT = 1)".format(O.DE_FLOOR),
+                "bars": {"fp64": O.DE_BAR_FP64, "mixed": O.DE_BAR_MIXED}}
 
     # ---- CPU baseline (oracle, same state and move stream), bounded sample
     cpu = None
     cpu_parity = None
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O
         from coulomb_kmc_amd.kmc_fmm import lr_operator_reference_layout
         nth = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         O.set_threads(nth)
         orc = O.Oracle(pos0, q, E, r, L, bc=bc, max_move=cfg["radius"],
                        lr=lr_operator_reference_layout(L, E) if bc in ("pbc", "ff-only") else None)
         orc.init_from_expansions(kmc.get_expansions(), kmc.energy)
+        # extended-precision reference of step 0's proposals on the same state
+        moves0 = host_moves(orc.positions(), pids_all[0], disp_all[0], k, E, bc)
+        es0 = np.arange(P + 1, dtype=np.int64) * k
+        npos0 = np.concatenate([m[1] for m in moves0])
+        tc = time.time()
+        ext0 = orc.propose_check(pids_all[0], es0, npos0, extended=True)
+        t_ext = time.time() - tc
+        accuracy["reference"] = ("oracle checker in long double (64-bit mantissa) on the GPU's initial state, "
+                                 "long range in difference form; all {} proposals of step 0, {:.1f} s"
+                                 .format(T, t_ext))
+        accuracy["fp64"] = O.de_error(g64_step0, ext0)
+        accuracy["mixed"] = O.de_error(gmx_step0, ext0)
+        accuracy["fp64_pass"] = accuracy["fp64"]["max_rel"] <= O.DE_BAR_FP64
+        accuracy["mixed_pass"] = accuracy["mixed"]["max_rel"] <= O.DE_BAR_MIXED
+        log("accuracy vs extended reference", accuracy)
         t_cpu, n_cpu, cmax, cabs, cmed = 0.0, 0, 0.0, 0.0, 0.0
         gpos = pos0.copy()
         picks = []
@@ -489,9 +511,10 @@ def main():
                 kmc.materialize_moves(P, k, t_pids, t_disp, counter, d_pid, d_es, d_np)
                 kmc.propose_device(d_pid, d_es, d_np, d_dE, precision="fp64")
                 g = d_dE.cpu().numpy()
-                cmax = float(np.max(np.abs(g - dc) / np.maximum(np.abs(dc), 1e-3)))
-                cabs = float(np.max(np.abs(g - dc)))
-                cmed = float(np.median(np.abs(g - dc) / np.maximum(np.abs(dc), 1e-3)))
+                ref_order = O.de_error(g, dc)
+                cmax, cabs, cmed = ref_order["max_rel"], ref_order["max_abs"], ref_order["median_rel"]
+                # the reference-ordered FP64 restatement against the extended reference
+                accuracy["cpu_reference_order"] = O.de_error(dc, ext0)
             n_cpu += 1
         cpu = {"value": n_cpu * T / t_cpu, "unit": "proposals/s", "cores": nth, "kind": "port",
                "accepts_per_s": n_cpu / t_cpu,
@@ -516,8 +539,9 @@ def main():
                                         "unknown")
         except OSError:
             cpu["cpu_model"] = "unknown"
-        cpu_parity = {"fp64_dE_max_rel_err_step0": cmax, "fp64_dE_median_rel_err_step0": cmed,
-                      "fp64_dE_max_abs_err_step0": cabs, "rel_err_floor": 1e-3, "cpu_picks": picks}
+        cpu_parity = {"gpu_fp64_vs_cpu_reference_order_step0": {"max_rel": cmax, "median_rel": cmed,
+                                                                "max_abs": cabs, "rel_floor": O.DE_FLOOR},
+                      "cpu_picks": picks}
         log("cpu baseline", cpu)
 
     # ---- timed runs: graph of G steps, replayed
@@ -621,14 +645,26 @@ def main():
 
     mixed, fp64 = results["mixed"], results["fp64"]
     kroof = kernel_rooflines(kt, N, ncells, L, P, T, bc, sweep_flops, sweep_bytes)
+    # the headline path: mixed only when both paths hold their bars against the extended
+    # reference; otherwise the FP64 path (whose bar is the tighter one), or nothing
+    # checked at all without the oracle (--no-cpu)
+    checked = "mixed_pass" in accuracy
+    if checked and accuracy["mixed_pass"] and accuracy["fp64_pass"]:
+        head = "mixed"
+    elif checked and accuracy["fp64_pass"]:
+        head = "fp64"
+    else:
+        head = "unchecked" if not checked else "failed"
+    accuracy["headline_path"] = head
+    hv = mixed if head in ("mixed", "unchecked") else fp64
     out = {
         "metric": METRIC,
-        "value": mixed["proposals_per_s"],
+        "value": hv["proposals_per_s"] if head != "failed" else None,
         "unit": "proposals/s",
         "n_gpus": 1,
         "steps": steps,
         "warmup": warm,
-        "ms_per_step": mixed["ms_per_step"],
+        "ms_per_step": hv["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -638,7 +674,8 @@ def main():
                    "L": L, "bc": bc, "proposals_per_step": T, "moving_charges_per_step": P,
                    "ball_radius": cfg["radius"], "kT": 1.0, "graph_steps": G, "tiles": tiles,
                    "pipelined": pipeline},
-        "accepts_per_s": mixed["accepts_per_s"],
+        "accepts_per_s": hv["accepts_per_s"],
+        "accuracy": accuracy,
         "mixed": mixed,
         "fp64": fp64,
         "mixed_vs_fp64": mixed["proposals_per_s"] / fp64["proposals_per_s"],
@@ -648,7 +685,7 @@ def main():
         "initialise_s": t_init,
         "roofline": roof,
         "cpu_baseline": cpu,
-        "gpu_vs_cpu": (mixed["proposals_per_s"] / cpu["value"]) if cpu else None,
+        "gpu_vs_cpu": (hv["proposals_per_s"] / cpu["value"]) if cpu else None,
         "parity": cpu_parity,
     }
     print(json.dumps(out), flush=True)

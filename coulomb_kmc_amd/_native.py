@@ -10,16 +10,20 @@ import os
 import subprocess
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_SRC = os.path.join(_HERE, "csrc", "kmc_amd.hip")
+_CSRC = os.path.join(_HERE, "csrc")
+_SRC = os.path.join(_CSRC, "kmc_amd.hip")          # order-independent kernels + host code
+_ORDER_SRC = os.path.join(_CSRC, "kmc_order.hip")  # compiled once per expansion order
+ORDERS = tuple(range(2, 21))
 LIB_PATH = os.path.join(_HERE, "libkmc_amd.so")
+_OBJ_DIR = os.path.join(_HERE, "_build")
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
     "-fPIC",
-    "-shared",
     "-std=c++17",
     "-Wno-unused-value",
+    "-Wno-unused-function",
     # packed FP32 instructions are disabled for all device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
@@ -27,6 +31,7 @@ HIPCC_FLAGS = [
 BC = {"free_space": 0, "27": 1, "pbc": 2, "ff-only": 3}
 PREC = {"fp64": 0, "mixed": 1}
 
+KMC_E_ARG = 1
 KMC_E_BAD_MOVE = 4
 KMC_E_STORE = 5
 KMC_E_STATE = 3
@@ -53,14 +58,36 @@ def hipcc():
     return "hipcc"
 
 
-def build(verbose=False):
-    """Compile csrc/kmc_amd.hip into libkmc_amd.so (always recompiles)."""
-    tmp = LIB_PATH + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-o", tmp, _SRC]
+def _compile_jobs():
+    jobs = [([_SRC], os.path.join(_OBJ_DIR, "kmc_amd.o"))]
+    for L in ORDERS:
+        jobs.append((["-DKMC_ORDER={}".format(L), _ORDER_SRC], os.path.join(_OBJ_DIR, "kmc_order_{}.o".format(L))))
+    return jobs
+
+
+def _run(cmd):
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         msg = "\n".join(l for l in res.stderr.splitlines() if "not a recognized feature" not in l)
-        raise RuntimeError("hipcc failed:\n" + msg[-4000:])
+        raise RuntimeError("hipcc failed: {}\n{}".format(" ".join(cmd[-2:]), msg[-4000:]))
+
+
+def build(verbose=False, jobs=None):
+    """Compile csrc/ into libkmc_amd.so (always recompiles): kmc_amd.hip and one object
+    of kmc_order.hip per expansion order, in parallel, then one shared-library link."""
+    from concurrent.futures import ThreadPoolExecutor
+    os.makedirs(_OBJ_DIR, exist_ok=True)
+    cc = hipcc()
+    todo = _compile_jobs()
+    # the order-independent object is the longest single compile: start it first
+    nproc = jobs or max(1, min(16, os.cpu_count() or 1))
+    with ThreadPoolExecutor(max_workers=nproc) as ex:
+        futs = [ex.submit(_run, [cc] + HIPCC_FLAGS + ["-c"] + args[:-1] + ["-o", obj, args[-1]])
+                for args, obj in todo]
+        for f in futs:
+            f.result()
+    tmp = LIB_PATH + ".tmp"
+    _run([cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + [obj for _, obj in todo])
     os.replace(tmp, LIB_PATH)
     if verbose:
         print("built", LIB_PATH)

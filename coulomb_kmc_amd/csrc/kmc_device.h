@@ -1,0 +1,1958 @@
+// Device code shared by the library's translation units: recurrence tables, basis
+// routines, kernel argument structs, the kernels templated on the expansion order L
+// and their launchers.  kmc_amd.hip holds the kernels that do not depend on L and
+// the host code; kmc_order.hip instantiates the L-templated launchers for one order
+// per object, so the orders compile in parallel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+#include <math.h>
+#include <string>
+#include <vector>
+#include <algorithm>
+#include <unordered_map>
+
+#include "../../include/kmc_amd.h"
+
+// ============================================================================
+// errors
+// ============================================================================
+static thread_local std::string g_err;
+static int set_err(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+#define HIPCHK(x)                                                                         \
+    do {                                                                                  \
+        hipError_t e_ = (x);                                                              \
+        if (e_ != hipSuccess)                                                             \
+            return set_err(KMC_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+// ============================================================================
+// compile-time recurrence tables
+// ============================================================================
+constexpr int KMIN_L = 2;
+constexpr int KMAX_L = 20;
+
+constexpr double c_sqrt(double x) {
+    if (x <= 0.0) return 0.0;
+    double g = x > 1.0 ? x : 1.0;
+    for (int i = 0; i < 200; i++) {
+        const double n = 0.5 * (g + x / g);
+        if (n == g) break;
+        g = n;
+    }
+    return g;
+}
+struct RecTab {
+    double a[KMAX_L + 1][KMAX_L + 1];   // alpha_lm = (2l-1)/sqrt(l^2-m^2)
+    double b[KMAX_L + 1][KMAX_L + 1];   // beta_lm  = sqrt(((l-1)^2-m^2)/(l^2-m^2))
+    double s[KMAX_L + 1];               // s_m = (-1)^m (2m-1)!!/sqrt((2m)!)
+};
+constexpr RecTab make_rectab() {
+    RecTab t{};
+    t.s[0] = 1.0;
+    for (int m = 1; m <= KMAX_L; m++) t.s[m] = -t.s[m - 1] * c_sqrt((2.0 * m - 1.0) / (2.0 * m));
+    for (int m = 0; m <= KMAX_L; m++)
+        for (int l = m + 1; l <= KMAX_L; l++) {
+            const double d = (double)(l * l - m * m);
+            t.a[l][m] = (2.0 * l - 1.0) / c_sqrt(d);
+            const double e = (double)((l - 1) * (l - 1) - m * m);
+            t.b[l][m] = e > 0.0 ? c_sqrt(e / d) : 0.0;
+        }
+    return t;
+}
+static constexpr RecTab kRec = make_rectab();
+
+__host__ __device__ constexpr int hidx(int l, int m) { return l * (l + 1) / 2 + m; }
+__host__ __device__ constexpr int nhalf(int L) { return L * (L + 1) / 2; }
+
+// ============================================================================
+// small device helpers
+// ============================================================================
+struct Cplx {
+    double re, im;
+};
+__device__ __forceinline__ Cplx cmul(Cplx a, Cplx b) {
+    return {fma(a.re, b.re, -a.im * b.im), fma(a.re, b.im, a.im * b.re)};
+}
+template <int M>
+__device__ __forceinline__ Cplx cpow(Cplx u) {
+    if constexpr (M == 0) {
+        return {1.0, 0.0};
+    } else if constexpr (M == 1) {
+        return u;
+    } else if constexpr (M % 2 == 0) {
+        const Cplx h = cpow<M / 2>(u);
+        return cmul(h, h);
+    } else {
+        return cmul(cpow<M - 1>(u), u);
+    }
+}
+
+// 1/sqrt(x) to full FP64 accuracy: FP32 hardware estimate, two Newton steps in FP64.
+__device__ __forceinline__ double rsqrt_fp64(double x) {
+    double y = (double)__builtin_amdgcn_rsqf((float)x);
+    double h = fma(-0.5 * x * y, y, 0.5);
+    y = fma(y, h, y);
+    h = fma(-0.5 * x * y, y, 0.5);
+    return fma(y, h, y);
+}
+// 1/sqrt(x) with one Newton step (relative error ~3e-14): the mixed-precision path.
+__device__ __forceinline__ double rsqrt_mixed(double x) {
+    const double y = (double)__builtin_amdgcn_rsqf((float)x);
+    const double h = fma(-0.5 * x * y, y, 0.5);
+    return fma(y, h, y);
+}
+// plain FP64 (library sqrt and division, as the reference's 1.0/sqrt(r2))
+__device__ __forceinline__ double rsqrt_plain(double x) { return 1.0 / sqrt(x); }
+
+template <bool MIXED>
+__device__ __forceinline__ double rsqrt_sel(double x) {
+    if constexpr (MIXED) return rsqrt_mixed(x);
+    else return rsqrt_plain(x);
+}
+
+struct Geom {
+    double E;        // box side
+    double w;        // leaf width
+    double c0;       // centre of leaf 0 along an axis: -E/2 + w/2
+    int ncps;        // leaves per side
+    int bc;
+};
+
+// leaf of a coordinate: truncate (p + E/2)/w and clamp (coulomb_kmc/kmc_fmm.py:229-242)
+__host__ __device__ inline int bin_axis(double p, double E, double w, int ncps) {
+    long long c = (long long)((0.5 * E + p) / w);
+    if (c < 0) c = 0;
+    if (c > ncps - 1) c = ncps - 1;
+    return (int)c;
+}
+
+// Compensated summation.  two_sum is Knuth's error-free transformation (s + e == a + b
+// exactly, s = fl(a + b)); a CSum carries a running sum and the sum of the rounding
+// errors of its additions, so a sum of k terms is good to ~1 ulp of the result plus
+// k eps^2 of the terms instead of k eps of the partial sums.  (No -ffast-math
+// anywhere in this build, so the compiler keeps these operations as written.)
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+struct CSum {
+    double s, c;
+    __device__ __forceinline__ CSum() : s(0.0), c(0.0) {}
+    __device__ __forceinline__ void add(double v) {
+        double e;
+        two_sum(s, v, s, e);
+        c += e;
+    }
+    // a * b added with the rounding error of the product (FMA two-product)
+    __device__ __forceinline__ void add_prod(double a, double b) {
+        const double p = a * b;
+        const double pe = fma(a, b, -p);
+        double e;
+        two_sum(s, p, s, e);
+        c += e + pe;
+    }
+    __device__ __forceinline__ void add(const CSum& o) {
+        double e;
+        two_sum(s, o.s, s, e);
+        c += e + o.c;
+    }
+    __device__ __forceinline__ double value() const { return s + c; }
+};
+// compensated sum over the lanes of groups of G consecutive lanes (G a power of two <= 64)
+template <int G>
+__device__ __forceinline__ CSum csum_group(CSum v) {
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+        CSum w;
+        w.s = __shfl_xor(v.s, o, G);
+        w.c = __shfl_xor(v.c, o, G);
+        v.add(w);
+    }
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// stencil offsets, x fastest (coulomb_kmc/common.py:51-81)
+__device__ __forceinline__ void stencil(int k, int& ox, int& oy, int& oz) {
+    ox = k % 3 - 1;
+    oy = (k / 3) % 3 - 1;
+    oz = k / 9 - 1;
+}
+
+// ============================================================================
+// accept sweep / initial P2L: every leaf receives q I_l^m conj(...) from sources
+// ============================================================================
+// Sources come either from the accepted move (old position with -q, new position
+// with +q) or from a range of particles (+q, initialisation).  For '27'/PBC each
+// source is applied with its 27 images (image shift o*E, cell shift o*ncps) and the
+// reference's well-separated test (coulomb_kmc/kmc_octal.py:966-975).
+struct MoveRec {
+    double old_pos[3];
+    double new_pos[3];
+    double q;
+    double dE;
+    int32_t pid;
+    int32_t old_cell;
+    int32_t new_cell;
+    int32_t valid;
+};
+
+struct SweepArgs {
+    Geom g;
+    double2* lexp;             // [ncells][nh]
+    int ncells;
+    int mode;                  // 0: move record, 1: particle range
+    const MoveRec* rec;
+    const double4* src_pos4;   // mode 1
+    const int32_t* src_cell;   // mode 1
+    int src_begin, src_end;
+};
+
+// Rescaled column recurrence of the sweep: e_l = c_l / Gam_l obeys
+//   e_l = a2_lm zr e_{l-1} - ir2 e_{l-2},  Gam_l = beta_lm Gam_{l-2},  a2_lm = alpha_lm Gam_{l-1} / Gam_l,
+// one multiply less per coefficient; accumulators are multiplied by Gam_l once per leaf.
+struct RecTab2 {
+    double a2[KMAX_L + 1][KMAX_L + 1];
+    double gam[KMAX_L + 1][KMAX_L + 1];
+};
+constexpr RecTab2 make_rectab2() {
+    RecTab2 t{};
+    for (int m = 0; m <= KMAX_L; m++) {
+        t.gam[m][m] = 1.0;
+        if (m + 1 <= KMAX_L) {
+            t.gam[m + 1][m] = 1.0;
+            t.a2[m + 1][m] = kRec.a[m + 1][m];
+        }
+        for (int l = m + 2; l <= KMAX_L; l++) {
+            t.gam[l][m] = kRec.b[l][m] * t.gam[l - 2][m];
+            t.a2[l][m] = kRec.a[l][m] * t.gam[l - 1][m] / t.gam[l][m];
+        }
+    }
+    return t;
+}
+static constexpr RecTab2 kRec2 = make_rectab2();
+
+// one column m of NI source-images: acc[k] += e_{m+k} * W_j, k = 0 .. L-m-1, j = 0 .. NI-1.
+// The NI recurrences are independent dependency chains (instruction-level
+// parallelism); each accumulator still receives the images in order j = 0, 1, ...
+template <int L, int M, int NI>
+__device__ __forceinline__ void sweep_column(const double* zr, const double* ir2, const Cplx* W, double* ar,
+                                             double* ai) {
+    double e2[NI], e1[NI];
+#pragma unroll
+    for (int j = 0; j < NI; j++) {
+        e2[j] = 0.0;
+        e1[j] = 1.0;
+        ar[0] += W[j].re;
+        ai[0] += W[j].im;
+    }
+#pragma unroll
+    for (int k = 1; k < L - M; k++) {
+        const int l = M + k;
+#pragma unroll
+        for (int j = 0; j < NI; j++) {
+            const double e = fma(kRec2.a2[l][M] * zr[j], e1[j], -ir2[j] * e2[j]);
+            e2[j] = e1[j];
+            e1[j] = e;
+        }
+#pragma unroll
+        for (int j = 0; j < NI; j++) {
+            ar[k] = fma(e1[j], W[j].re, ar[k]);
+            ai[k] = fma(e1[j], W[j].im, ai[k]);
+        }
+    }
+}
+
+template <int L, int M>
+__device__ __forceinline__ void sweep_store_column(double2* le, const double* ar, const double* ai) {
+#pragma unroll
+    for (int k = 0; k < L - M; k++) {
+        const double gm = kRec2.gam[M + k][M];
+        double2 v = le[hidx(M + k, M)];
+        v.x = fma(gm, ar[k], v.x);
+        v.y = fma(gm, ai[k], v.y);
+        le[hidx(M + k, M)] = v;
+    }
+}
+
+// columns (P, L-1-P) of pair P, NI images, into accumulator row ar/ai (first L-P entries: column P)
+// column M with weight base * s_M * U (U = conj((x+iy)/r^2)^M) into the accumulator row
+template <int L, int M, int NI>
+__device__ __forceinline__ void sweep_weighted_column(const double* zr, const double* ir2, const Cplx* U,
+                                                      const double* base, double* ar, double* ai) {
+    Cplx W[NI];
+#pragma unroll
+    for (int j = 0; j < NI; j++) {
+        const double f = base[j] * kRec.s[M];
+        W[j] = Cplx{f * U[j].re, f * U[j].im};
+    }
+    sweep_column<L, M, NI>(zr, ir2, W, ar, ai);
+}
+// low columns P0 + J, J = 0 .. NPE-1 (ascending powers: one multiply per column)
+template <int L, int P0, int NPE, int NI, int J = 0>
+__device__ __forceinline__ void sweep_low_cols(const double* zr, const double* ir2, const Cplx* u, Cplx* U,
+                                               const double* base, double (*ar)[L + 1], double (*ai)[L + 1]) {
+    if constexpr (J < NPE) {
+        if constexpr (J > 0) {
+#pragma unroll
+            for (int j = 0; j < NI; j++) U[j] = cmul(U[j], u[j]);
+        }
+        sweep_weighted_column<L, P0 + J, NI>(zr, ir2, U, base, ar[J], ai[J]);
+        sweep_low_cols<L, P0, NPE, NI, J + 1>(zr, ir2, u, U, base, ar, ai);
+    }
+}
+// high columns L-1-P0-J for J = NPE-1 down to 0 (ascending powers), skipping the
+// middle column that equals its low partner when L is odd
+template <int L, int P0, int NPE, int NI, int J>
+__device__ __forceinline__ void sweep_high_cols(const double* zr, const double* ir2, const Cplx* u, Cplx* U,
+                                                const double* base, double (*ar)[L + 1], double (*ai)[L + 1]) {
+    if constexpr (J >= 0) {
+        constexpr int M1 = P0 + J, M2 = L - 1 - P0 - J;
+        if constexpr (M2 != M1)
+            sweep_weighted_column<L, M2, NI>(zr, ir2, U, base, ar[J] + (L - M1), ai[J] + (L - M1));
+        if constexpr (J > 0) {
+#pragma unroll
+            for (int j = 0; j < NI; j++) U[j] = cmul(U[j], u[j]);
+        }
+        sweep_high_cols<L, P0, NPE, NI, J - 1>(zr, ir2, u, U, base, ar, ai);
+    }
+}
+// the column pairs (P, L-1-P), P = P0 .. P0+NP-1, of one lane: the powers of u run up
+// through the low columns and on (one cpow) through the high ones
+template <int L, int P0, int NP, int NI>
+__device__ __forceinline__ void sweep_group_cols(const double* zr, const double* ir2, const Cplx* u,
+                                                 const double* base, double (*ar)[L + 1], double (*ai)[L + 1]) {
+    constexpr int NPAIR = (L + 1) / 2;
+    constexpr int NPE = (P0 + NP <= NPAIR) ? NP : NPAIR - P0;   // pairs in this group
+    if constexpr (NPE > 0) {
+        Cplx U[NI];
+#pragma unroll
+        for (int j = 0; j < NI; j++) U[j] = cpow<P0>(u[j]);
+        sweep_low_cols<L, P0, NPE, NI>(zr, ir2, u, U, base, ar, ai);
+        constexpr int LOW_TOP = P0 + NPE - 1;          // power held in U now
+        constexpr int HIGH_BOTTOM = L - 1 - LOW_TOP;    // lowest high column
+        if constexpr (HIGH_BOTTOM > LOW_TOP) {
+#pragma unroll
+            for (int j = 0; j < NI; j++) U[j] = cmul(U[j], cpow<HIGH_BOTTOM - LOW_TOP>(u[j]));
+            sweep_high_cols<L, P0, NPE, NI, NPE - 1>(zr, ir2, u, U, base, ar, ai);
+        } else if constexpr (HIGH_BOTTOM == LOW_TOP && NPE > 1) {
+            // L odd: the last low column is the middle one; the high columns start one above
+#pragma unroll
+            for (int j = 0; j < NI; j++) U[j] = cmul(U[j], u[j]);
+            sweep_high_cols<L, P0, NPE, NI, NPE - 2>(zr, ir2, u, U, base, ar, ai);
+        }
+    }
+}
+template <int L, int P0, int NP, int J = 0>
+__device__ __forceinline__ void sweep_group_store(double2* le, double (*ar)[L + 1], double (*ai)[L + 1]) {
+    if constexpr (J < NP && P0 + J < (L + 1) / 2) {
+        constexpr int M1 = P0 + J, M2 = L - 1 - M1;
+        sweep_store_column<L, M1>(le, ar[J], ai[J]);
+        if constexpr (M2 != M1) sweep_store_column<L, M2>(le, ar[J] + (L - M1), ai[J] + (L - M1));
+        sweep_group_store<L, P0, NP, J + 1>(le, ar, ai);
+    }
+}
+
+// NI source images (displacement leaf centre -> image, charge or 0 when not well separated)
+template <int L, int P0, int NP, int NI>
+__device__ __forceinline__ void sweep_images(const double* dx, double dy, double dz, const double* qe,
+                                             double (*ar)[L + 1], double (*ai)[L + 1]) {
+    double zr[NI], ir2[NI], base[NI];
+    Cplx u[NI];
+#pragma unroll
+    for (int j = 0; j < NI; j++) {
+        double r2 = fma(dx[j], dx[j], fma(dy, dy, dz * dz));
+        r2 = qe[j] != 0.0 ? r2 : 1.0;
+        const double ir = rsqrt_fp64(r2);
+        ir2[j] = ir * ir;
+        zr[j] = dz * ir2[j];
+        u[j] = Cplx{dx[j] * ir2[j], -dy * ir2[j]};     // conj((x+iy)/r^2)
+        base[j] = qe[j] * ir;
+    }
+    sweep_group_cols<L, P0, NP, NI>(zr, ir2, u, base, ar, ai);
+}
+template <int L, int P0, int NP, int NI>
+__device__ __forceinline__ void sweep_images_xyz(const double* dx, const double* dy, const double* dz,
+                                                 const double* qe, double (*ar)[L + 1], double (*ai)[L + 1]) {
+    double zr[NI], ir2[NI], base[NI];
+    Cplx u[NI];
+#pragma unroll
+    for (int j = 0; j < NI; j++) {
+        double r2 = fma(dx[j], dx[j], fma(dy[j], dy[j], dz[j] * dz[j]));
+        r2 = qe[j] != 0.0 ? r2 : 1.0;
+        const double ir = rsqrt_fp64(r2);
+        ir2[j] = ir * ir;
+        zr[j] = dz[j] * ir2[j];
+        u[j] = Cplx{dx[j] * ir2[j], -dy[j] * ir2[j]};
+        base[j] = qe[j] * ir;
+    }
+    sweep_group_cols<L, P0, NP, NI>(zr, ir2, u, base, ar, ai);
+}
+
+__device__ __forceinline__ bool near1(int d) { return d < 2 && d > -2; }
+
+// pairs P = G*NP .. G*NP+NP-1 of columns (P, L-1-P) for one leaf (one lane).
+// PER ('27'/PBC): the 27 images of a source in stencil order (x fastest), of which
+// this lane takes the z-layers oz = z0 .. z1 (all three unless the images are split
+// over the waves of the workgroup); free space: the source itself.
+template <int L, int G, int NP, bool PER>
+__device__ __forceinline__ void sweep_group(const SweepArgs& a, int cell, int z0, int z1, double (*ar)[L + 1],
+                                            double (*ai)[L + 1]) {
+    constexpr int P0 = G * NP;
+    const int n = a.g.ncps;
+    const double E = a.g.E;
+    const int tx = cell % n, ty = (cell / n) % n, tz = cell / (n * n);
+    const double cx = a.g.c0 + tx * a.g.w;
+    const double cy = a.g.c0 + ty * a.g.w;
+    const double cz = a.g.c0 + tz * a.g.w;
+    int nsrc;
+    if (a.mode == 0) nsrc = a.rec->valid ? 2 : 0;
+    else nsrc = a.src_end - a.src_begin;
+
+    if constexpr (PER && NP == 2) {
+        if (a.mode == 0 && nsrc == 2) {
+            // the move: each image of the old (-q) and the new (+q) position together
+            // (two independent recurrences per column; with one column pair per lane the
+            // extra registers would spill)
+            const MoveRec* rc = a.rec;
+            double dx0[2], dy0[2], dz0[2], sq[2];
+            int dcx[2], dcy[2], dcz[2];
+#pragma unroll
+            for (int s = 0; s < 2; s++) {
+                const double* p = s == 0 ? rc->old_pos : rc->new_pos;
+                const int sc = s == 0 ? rc->old_cell : rc->new_cell;
+                sq[s] = s == 0 ? -1.0 * rc->q : rc->q;
+                dx0[s] = p[0] - cx; dy0[s] = p[1] - cy; dz0[s] = p[2] - cz;
+                dcx[s] = sc % n - tx; dcy[s] = (sc / n) % n - ty; dcz[s] = sc / (n * n) - tz;
+            }
+#pragma unroll 1
+            for (int oz = z0; oz <= z1; oz++) {
+#pragma unroll 1
+                for (int oy = -1; oy <= 1; oy++) {
+#pragma unroll 1
+                    for (int ox = -1; ox <= 1; ox++) {
+                        double dx[2], dy[2], dz[2], qe[2];
+#pragma unroll
+                        for (int s = 0; s < 2; s++) {
+                            dx[s] = fma((double)ox, E, dx0[s]);
+                            dy[s] = fma((double)oy, E, dy0[s]);
+                            dz[s] = fma((double)oz, E, dz0[s]);
+                            const bool nr = near1(dcx[s] + ox * n) && near1(dcy[s] + oy * n) && near1(dcz[s] + oz * n);
+                            qe[s] = nr ? 0.0 : sq[s];
+                        }
+                        sweep_images_xyz<L, P0, NP, 2>(dx, dy, dz, qe, ar, ai);
+                    }
+                }
+            }
+            return;
+        }
+    }
+    for (int s = 0; s < nsrc; s++) {
+        double sx, sy, sz, sq;
+        int sc;
+        if (a.mode == 0) {
+            const double* p = s == 0 ? a.rec->old_pos : a.rec->new_pos;
+            sx = p[0]; sy = p[1]; sz = p[2];
+            sq = s == 0 ? -1.0 * a.rec->q : a.rec->q;
+            sc = s == 0 ? a.rec->old_cell : a.rec->new_cell;
+        } else {
+            const double4 p = a.src_pos4[a.src_begin + s];
+            sx = p.x; sy = p.y; sz = p.z; sq = p.w;
+            sc = a.src_cell[a.src_begin + s];
+        }
+        const int dcx = sc % n - tx, dcy = (sc / n) % n - ty, dcz = sc / (n * n) - tz;
+        const double dx0 = sx - cx, dy0 = sy - cy, dz0 = sz - cz;
+        if constexpr (!PER) {
+            const double dx[1] = {dx0};
+            const double qe[1] = {(near1(dcx) && near1(dcy) && near1(dcz)) ? 0.0 : sq};
+            sweep_images<L, P0, NP, 1>(dx, dy0, dz0, qe, ar, ai);
+        } else {
+            const double dxm = dx0 - E, dxp = dx0 + E;
+            const bool nxm = near1(dcx - n), nx0 = near1(dcx), nxp = near1(dcx + n);
+#pragma unroll 1
+            for (int oz = z0; oz <= z1; oz++) {
+                const double dz = fma((double)oz, E, dz0);
+                const bool nz = near1(dcz + oz * n);
+#pragma unroll 1
+                for (int oy = -1; oy <= 1; oy++) {
+                    const double dy = fma((double)oy, E, dy0);
+                    const bool nyz = nz && near1(dcy + oy * n);
+#pragma unroll 1
+                    for (int ox = -1; ox <= 1; ox++) {
+                        const double dx[1] = {ox < 0 ? dxm : (ox == 0 ? dx0 : dxp)};
+                        const bool nxs = ox < 0 ? nxm : (ox == 0 ? nx0 : nxp);
+                        const double qe[1] = {(nyz && nxs) ? 0.0 : sq};
+                        sweep_images<L, P0, NP, 1>(dx, dy, dz, qe, ar, ai);
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <int L, int NP, bool PER, int G0>
+__device__ __forceinline__ void sweep_dispatch(const SweepArgs& a, int cell, int group, int z0, int z1,
+                                               double (*ar)[L + 1], double (*ai)[L + 1]) {
+    constexpr int NG = ((L + 1) / 2 + NP - 1) / NP;
+    if constexpr (G0 < NG) {
+        if (group == G0) sweep_group<L, G0, NP, PER>(a, cell, z0, z1, ar, ai);
+        else sweep_dispatch<L, NP, PER, G0 + 1>(a, cell, group, z0, z1, ar, ai);
+    }
+}
+template <int L, int NP, int G0>
+__device__ __forceinline__ void sweep_store_dispatch(double2* le, int group, double (*ar)[L + 1], double (*ai)[L + 1]) {
+    constexpr int NG = ((L + 1) / 2 + NP - 1) / NP;
+    if constexpr (G0 < NG) {
+        if (group == G0) sweep_group_store<L, G0 * NP, NP>(le, ar, ai);
+        else sweep_store_dispatch<L, NP, G0 + 1>(le, group, ar, ai);
+    }
+}
+
+// scale a group's sums in place by Gam (rescaled recurrence), then copy them to a leaf row
+template <int L, int M>
+__device__ __forceinline__ void sweep_scale_column(double* ar, double* ai) {
+#pragma unroll
+    for (int k = 0; k < L - M; k++) {
+        ar[k] *= kRec2.gam[M + k][M];
+        ai[k] *= kRec2.gam[M + k][M];
+    }
+}
+template <int L, int M>
+__device__ __forceinline__ void sweep_stage_column(double2* row, const double* ar, const double* ai) {
+#pragma unroll
+    for (int k = 0; k < L - M; k++) row[hidx(M + k, M)] = make_double2(ar[k], ai[k]);
+}
+template <int L, int P0, int NP, bool SCALE, int J = 0>
+__device__ __forceinline__ void sweep_group_stage(double2* row, double (*ar)[L + 1], double (*ai)[L + 1]) {
+    if constexpr (J < NP && P0 + J < (L + 1) / 2) {
+        constexpr int M1 = P0 + J, M2 = L - 1 - M1;
+        if constexpr (SCALE) sweep_scale_column<L, M1>(ar[J], ai[J]);
+        else sweep_stage_column<L, M1>(row, ar[J], ai[J]);
+        if constexpr (M2 != M1) {
+            if constexpr (SCALE) sweep_scale_column<L, M2>(ar[J] + (L - M1), ai[J] + (L - M1));
+            else sweep_stage_column<L, M2>(row, ar[J] + (L - M1), ai[J] + (L - M1));
+        }
+        sweep_group_stage<L, P0, NP, SCALE, J + 1>(row, ar, ai);
+    }
+}
+template <int L, int NP, bool SCALE, int G0>
+__device__ __forceinline__ void sweep_stage_dispatch(double2* row, int group, double (*ar)[L + 1], double (*ai)[L + 1]) {
+    constexpr int NG = ((L + 1) / 2 + NP - 1) / NP;
+    if constexpr (G0 < NG) {
+        if (group == G0) sweep_group_stage<L, G0 * NP, NP, SCALE>(row, ar, ai);
+        else sweep_stage_dispatch<L, NP, SCALE, G0 + 1>(row, group, ar, ai);
+    }
+}
+
+// Lane = leaf (threadIdx.x), column-pair group = blockIdx.y.  With SPLIT = 3 the
+// workgroup is 64 leaves x 3 waves and wave threadIdx.y takes the image layer
+// oz = threadIdx.y - 1; the two upper waves hand their sums to wave 0 through LDS
+// (fixed order), which alone updates the expansions.
+template <int L, int NP, bool PER, int SPLIT>
+__global__ void __launch_bounds__(256) k_sweep(SweepArgs a) {
+    constexpr int W = L + 1;                 // accumulator entries per pair
+    const int cell0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = cell0 < a.ncells;
+    const int cell = live ? cell0 : a.ncells - 1;
+    const int group = (int)blockIdx.y;
+    double ar[NP][W], ai[NP][W];
+#pragma unroll
+    for (int j = 0; j < NP; j++)
+#pragma unroll
+        for (int k = 0; k < W; k++) { ar[j][k] = 0.0; ai[j][k] = 0.0; }
+    if constexpr (SPLIT == 1) {
+        sweep_dispatch<L, NP, PER, 0>(a, cell, group, -1, 1, ar, ai);
+    } else {
+        static_assert(SPLIT == 3 && PER, "image split needs the 27-image stencil");
+        __shared__ double red[2][NP][2 * W][64];
+        const int sl = (int)threadIdx.y;
+        sweep_dispatch<L, NP, PER, 0>(a, cell, group, sl - 1, sl - 1, ar, ai);
+        if (sl > 0) {
+#pragma unroll
+            for (int j = 0; j < NP; j++)
+#pragma unroll
+                for (int k = 0; k < W; k++) {
+                    red[sl - 1][j][2 * k][threadIdx.x] = ar[j][k];
+                    red[sl - 1][j][2 * k + 1][threadIdx.x] = ai[j][k];
+                }
+        }
+        __syncthreads();
+        if (sl > 0) return;
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+#pragma unroll
+            for (int j = 0; j < NP; j++)
+#pragma unroll
+                for (int k = 0; k < W; k++) {
+                    ar[j][k] += red[u][j][2 * k][threadIdx.x];
+                    ai[j][k] += red[u][j][2 * k + 1][threadIdx.x];
+                }
+    }
+    if (!live) return;
+    sweep_store_dispatch<L, NP, 0>(a.lexp + (size_t)cell * nhalf(L), group, ar, ai);
+}
+
+// Fused layout: a workgroup owns 64 consecutive leaves and all their column groups
+// (wave threadIdx.y = group), so the update of the leaves' expansions is one
+// contiguous block of 64 * nh coefficients.  The sums are staged through LDS in two
+// halves of 32 leaves and added to the expansions with coalesced loads and stores
+// (a lane's own columns are scattered over its leaf: written directly they touch a
+// cache line per lane and the groups of a leaf would re-fetch the same lines).
+constexpr int stage_stride(int nh) { return nh + ((9 - nh % 8) % 8); }   // = 1 (mod 8): conflict-free b128 rows
+template <int L, int NP, bool PER>
+__device__ __forceinline__ void sweep_fused_body(const SweepArgs& a) {
+    constexpr int W = L + 1;
+    constexpr int NH = nhalf(L);
+    constexpr int ST = stage_stride(NH);
+    __shared__ double2 stage[32 * ST];
+    const int lane = threadIdx.x;
+    const int group = __builtin_amdgcn_readfirstlane((int)threadIdx.y);   // one group per wave
+    const int cell0 = blockIdx.x * 64;
+    const int cell = cell0 + lane;
+    double ar[NP][W], ai[NP][W];
+#pragma unroll
+    for (int j = 0; j < NP; j++)
+#pragma unroll
+        for (int k = 0; k < W; k++) { ar[j][k] = 0.0; ai[j][k] = 0.0; }
+    sweep_dispatch<L, NP, PER, 0>(a, cell, group, -1, 1, ar, ai);
+    sweep_stage_dispatch<L, NP, true, 0>(nullptr, group, ar, ai);
+    const int tid = threadIdx.x + 64 * threadIdx.y;
+    const int nthr = 64 * blockDim.y;
+#pragma unroll 1
+    for (int half = 0; half < 2; half++) {
+        __syncthreads();
+        if ((lane >> 5) == half) sweep_stage_dispatch<L, NP, false, 0>(stage + (lane & 31) * ST, group, ar, ai);
+        __syncthreads();
+        double2* base = a.lexp + (size_t)(cell0 + 32 * half) * NH;
+        // eight loads in flight per thread before their adds and stores
+        constexpr int U = 8;
+#pragma unroll 1
+        for (int e0 = tid; e0 < 32 * NH; e0 += nthr * U) {
+            double2 v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int e = e0 + u * nthr;
+                if (e < 32 * NH) v[u] = base[e];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int e = e0 + u * nthr;
+                if (e < 32 * NH) {
+                    const int j = e / NH, h = e - j * NH;
+                    const double2 sv = stage[j * ST + h];
+                    v[u].x += sv.x;
+                    v[u].y += sv.y;
+                    base[e] = v[u];
+                }
+            }
+        }
+    }
+}
+template <int L, int NP, bool PER>
+__global__ void __launch_bounds__(512, ((NP == 1 && L <= 16) ? 4 : 2)) k_sweep_fused(SweepArgs a) {
+    sweep_fused_body<L, NP, PER>(a);
+}
+template <int L, int NP, bool PER>
+__global__ void __launch_bounds__(512, ((NP == 1 && L <= 16) ? 4 : 2)) kb_sweep_fused(const SweepArgs* __restrict__ args) {
+    const SweepArgs& a = args[blockIdx.z];
+    if ((int)blockIdx.x * 64 >= a.ncells) return;   // grid sized for the largest system of the batch
+    sweep_fused_body<L, NP, PER>(a);
+}
+
+
+// ============================================================================
+// far field: phi = Re sum L_lm R_l^m(x - centre), one lane per evaluation point
+// ============================================================================
+struct EvalArgs {
+    Geom g;
+    int n;                       // evaluation points
+    const double4* pos4;         // (x, y, z, q)
+    const int32_t* cell;
+    const int32_t* excl;         // id excluded from the near field (-1: none)
+    double* out;
+    const double2* lexp;
+    const int* occ;
+    const double4* slot_pos;
+    const int32_t* slot_id;
+    int occmax;
+};
+
+// columns M1 and M2 (M2 == M1: one column) of  Re sum_l L_lm R_l^m(d)  with weight
+// 2 for m > 0 (the m < 0 terms are the complex conjugates)
+template <int L, int M1, int M2>
+__device__ __forceinline__ double far_columns(double dx, double dy, double dz, double r2, const double2* le) {
+    CSum phi;
+    const Cplx w = {dx, dy};
+    const Cplx C1 = cpow<M1>(w);
+#pragma unroll
+    for (int col = 0; col < 2; col++) {
+        if (col == 1 && M2 == M1) break;
+        const int m = col == 0 ? M1 : M2;
+        const Cplx C = col == 0 ? C1 : cmul(C1, cpow<M2 - M1>(w));
+        double b2 = 0.0, b1 = kRec.s[m];
+        double2 v = le[hidx(m, m)];
+        CSum sr, si;
+        sr.add_prod(b1, v.x);
+        si.add_prod(b1, v.y);
+#pragma unroll
+        for (int l = m + 1; l < L; l++) {
+            const double bb = fma(kRec.a[l][m] * dz, b1, -(kRec.b[l][m] * r2) * b2);
+            b2 = b1;
+            b1 = bb;
+            v = le[hidx(l, m)];
+            sr.add_prod(bb, v.x);
+            si.add_prod(bb, v.y);
+        }
+        // Re(C (sr + i si)) with weight 2 for m > 0 (exact scaling)
+        const double wgt = m == 0 ? 1.0 : 2.0;
+        phi.add_prod(wgt * C.re, sr.s);
+        phi.add_prod(-wgt * C.im, si.s);
+        phi.add(wgt * fma(C.re, sr.c, -C.im * si.c));
+    }
+    return phi.value();
+}
+
+template <int L, int P0>
+__device__ __forceinline__ double far_dispatch(int pair, double dx, double dy, double dz, double r2,
+                                               const double2* le) {
+    if constexpr (P0 < (L + 1) / 2) {
+        if (pair == P0) return far_columns<L, P0, L - 1 - P0>(dx, dy, dz, r2, le);
+        return far_dispatch<L, P0 + 1>(pair, dx, dy, dz, r2, le);
+    } else {
+        return 0.0;
+    }
+}
+
+// one lane per evaluation point; the column pairs (m, L-1-m) of 64 points are the
+// waves of one workgroup (pair = blockIdx.y * blockDim.y + threadIdx.y), so the
+// waves reading one leaf expansion run together and share its cache lines;
+// out[pair * n + i]
+template <int L>
+__device__ __forceinline__ void far_body(const EvalArgs& a) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int pair = __builtin_amdgcn_readfirstlane((int)(blockIdx.y * blockDim.y + threadIdx.y));
+    if (i >= a.n || pair >= (L + 1) / 2) return;
+    const double4 p = a.pos4[i];
+    const int c = a.cell[i];
+    const int n = a.g.ncps;
+    const int tx = c % n, ty = (c / n) % n, tz = c / (n * n);
+    const double dx = p.x - (a.g.c0 + tx * a.g.w);
+    const double dy = p.y - (a.g.c0 + ty * a.g.w);
+    const double dz = p.z - (a.g.c0 + tz * a.g.w);
+    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    const double2* le = a.lexp + (size_t)c * nhalf(L);
+    a.out[(size_t)pair * a.n + i] = p.w * far_dispatch<L, 0>(pair, dx, dy, dz, r2, le);
+}
+template <int L>
+__global__ void __launch_bounds__(512) k_far(EvalArgs a) { far_body<L>(a); }
+template <int L>
+__global__ void __launch_bounds__(512) kb_far(const EvalArgs* __restrict__ args) { far_body<L>(args[blockIdx.z]); }
+
+
+// ============================================================================
+// near field: q_t * sum over the 27 neighbour leaves of q_j / |x_t - x_j|
+// G lanes per evaluation point, neighbour slots strided over the G lanes,
+// shuffle reduction inside the lane group.
+// ============================================================================
+template <int G, bool MIXED>
+__device__ __forceinline__ void near_body(const EvalArgs& a) {
+    const int gid = (blockIdx.x * blockDim.x + threadIdx.x);
+    const int i = gid / G;
+    const int lane = gid % G;
+    const bool active = i < a.n;
+    CSum acc;
+    if (active) {
+        const double4 p = a.pos4[i];
+        const int c = a.cell[i];
+        const int ex = a.excl[i];
+        const int n = a.g.ncps;
+        const int tx = c % n, ty = (c / n) % n, tz = c / (n * n);
+        const bool periodic = a.g.bc != KMC_BC_FREE_SPACE;
+        for (int k = 0; k < 27; k++) {
+            int ox, oy, oz;
+            stencil(k, ox, oy, oz);
+            int nx = tx + ox, ny = ty + oy, nz = tz + oz;
+            double shx = 0.0, shy = 0.0, shz = 0.0;
+            if (periodic) {
+                if (nx < 0) { nx += n; shx = -a.g.E; } else if (nx >= n) { nx -= n; shx = a.g.E; }
+                if (ny < 0) { ny += n; shy = -a.g.E; } else if (ny >= n) { ny -= n; shy = a.g.E; }
+                if (nz < 0) { nz += n; shz = -a.g.E; } else if (nz >= n) { nz -= n; shz = a.g.E; }
+            } else if (nx < 0 || nx >= n || ny < 0 || ny >= n || nz < 0 || nz >= n) {
+                continue;
+            }
+            const int nc = nx + n * (ny + n * nz);
+            const int occ = min(a.occ[nc], a.occmax);
+            const double px = p.x - shx, py = p.y - shy, pz = p.z - shz;
+            const size_t base = (size_t)nc * a.occmax;
+            for (int s = lane; s < occ; s += G) {
+                const double4 q = a.slot_pos[base + s];
+                const double dx = px - q.x, dy = py - q.y, dz = pz - q.z;
+                const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                const double contrib = q.w * rsqrt_sel<MIXED>(r2);
+                if (ex < 0 || a.slot_id[base + s] != ex) acc.add(contrib);
+            }
+        }
+    }
+    if constexpr (G > 1) acc = csum_group<G>(acc);
+    if (active && lane == 0) a.out[i] = acc.value() * a.pos4[i].w;
+}
+
+// near field, one lane per neighbour leaf: 32 lanes per evaluation point, lanes
+// 0..26 each walk one leaf of the stencil, shuffle reduction over the 32 lanes.
+template <bool MIXED>
+__device__ __forceinline__ void near_cells_body(const EvalArgs& a) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = gid >> 5;
+    const int k = gid & 31;
+    CSum acc;
+    if (i < a.n && k < 27) {
+        const double4 p = a.pos4[i];
+        const int c = a.cell[i];
+        const int ex = a.excl[i];
+        const int n = a.g.ncps;
+        const int tx = c % n, ty = (c / n) % n, tz = c / (n * n);
+        int ox, oy, oz;
+        stencil(k, ox, oy, oz);
+        int nx = tx + ox, ny = ty + oy, nz = tz + oz;
+        double shx = 0.0, shy = 0.0, shz = 0.0;
+        bool ok = true;
+        if (a.g.bc != KMC_BC_FREE_SPACE) {
+            if (nx < 0) { nx += n; shx = -a.g.E; } else if (nx >= n) { nx -= n; shx = a.g.E; }
+            if (ny < 0) { ny += n; shy = -a.g.E; } else if (ny >= n) { ny -= n; shy = a.g.E; }
+            if (nz < 0) { nz += n; shz = -a.g.E; } else if (nz >= n) { nz -= n; shz = a.g.E; }
+        } else if (nx < 0 || nx >= n || ny < 0 || ny >= n || nz < 0 || nz >= n) {
+            ok = false;
+        }
+        if (ok) {
+            const int nc = nx + n * (ny + n * nz);
+            const int occ = min(a.occ[nc], a.occmax);
+            const double px = p.x - shx, py = p.y - shy, pz = p.z - shz;
+            const size_t base = (size_t)nc * a.occmax;
+            // slots in groups of 4 with all loads issued up front (indices clamped to
+            // the cell's slot range; slots past the occupancy contribute 0)
+            for (int s0 = 0; s0 < occ; s0 += 4) {
+                double4 q[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) q[j] = a.slot_pos[base + min(s0 + j, a.occmax - 1)];
+                int id[4] = {-1, -1, -1, -1};
+                if (ex >= 0) {
+#pragma unroll
+                    for (int j = 0; j < 4; j++) id[j] = a.slot_id[base + min(s0 + j, a.occmax - 1)];
+                }
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const double dx = px - q[j].x, dy = py - q[j].y, dz = pz - q[j].z;
+                    double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                    const bool use = (s0 + j < occ) && (ex < 0 || id[j] != ex);
+                    r2 = use ? r2 : 1.0;
+                    const double contrib = q[j].w * rsqrt_sel<MIXED>(r2);
+                    acc.add(use ? contrib : 0.0);
+                }
+            }
+        }
+    }
+    acc = csum_group<32>(acc);
+    if (i < a.n && k == 0) a.out[i] = acc.value() * a.pos4[i].w;
+}
+
+// near field of the propose paths, grouped by moving charge: one wave per moving
+// charge p takes its old position (entry T + p, its own id excluded) and its proposals
+// (entries es[p] .. es[p+1]-1) together, up to 16 points per pass.  The leaves within
+// one of the points' leaves form a box around the old position's leaf (at most 5 per
+// side); the wave lists the box's slots leaf by leaf (prefix sums in LDS) and each lane
+// reads its share of them once and adds each to every point whose 27-leaf stencil holds
+// its leaf.  Per pair the image shift and the arithmetic are those of the per-point
+// kernels (only the order of the sums differs).  A pass whose points span more than
+// 3 leaves along an axis is evaluated point by point over each point's own 27 leaves.
+__device__ __forceinline__ double readlane_d(double v, int k) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffll), k);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+__device__ __forceinline__ int floordiv(int a, int n) { return a >= 0 ? a / n : -((-a + n - 1) / n); }
+
+constexpr int NEAR_GROUP_POINTS = 16;
+constexpr int NEAR_BOX = 5;
+
+
+
+// ============================================================================
+// proposal setup (tuple form) and validation
+// ============================================================================
+struct SetupArgs {
+    Geom g;
+    double max_move;
+    int P, T, n;
+    const int64_t* pid;
+    const int64_t* es;
+    const double* new_pos;
+    const double* new_q;
+    const double4* ppos4;
+    const int32_t* pcell;
+    double4* epos4;
+    int32_t* ecell;
+    int32_t* eexcl;
+    int32_t* owner;
+    int* err;
+    int64_t* es_out;     // copy of es[0..P] (read by the grouped near-field kernel)
+};
+
+// owner p of proposal t: es[p] <= t < es[p+1]; always returns a value in [0, P)
+__device__ __forceinline__ int owner_of(int t, const int64_t* es, int P) {
+    int lo = 0, hi = P - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (es[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// One thread per proposal t (< T) and per moving charge p (< P).
+// kmc_mpi_decomp.py:312-345 (domain and max_move checks, minimum image for '27'/PBC)
+__device__ __forceinline__ void setup_body(const SetupArgs& a) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int bad = 0;
+    if (x < a.P) {
+        int i = (int)a.pid[x];
+        if (i < 0 || i >= a.n) { bad++; i = 0; }
+        a.epos4[a.T + x] = a.ppos4[i];
+        a.ecell[a.T + x] = a.pcell[i];
+        a.eexcl[a.T + x] = i;
+        if (a.es[x] > a.es[x + 1]) bad++;
+        if (x == 0 && a.es[0] != 0) bad++;
+        if (x == a.P - 1 && a.es[a.P] != a.T) bad++;
+        a.es_out[x] = a.es[x];
+        if (x == a.P - 1) a.es_out[a.P] = a.es[a.P];
+    }
+    if (x < a.T) {
+        const int p = owner_of(x, a.es, a.P);
+        a.owner[x] = p;
+        int i = (int)a.pid[p];
+        if (i < 0 || i >= a.n) i = 0;
+        const double4 o = a.ppos4[i];
+        const int n = a.g.ncps;
+        const double px = a.new_pos[3 * x], py = a.new_pos[3 * x + 1], pz = a.new_pos[3 * x + 2];
+        const double nq = a.new_q ? a.new_q[x] : o.w;
+        a.epos4[x] = make_double4(px, py, pz, nq);
+        const int cx = bin_axis(px, a.g.E, a.g.w, n), cy = bin_axis(py, a.g.E, a.g.w, n), cz = bin_axis(pz, a.g.E, a.g.w, n);
+        a.ecell[x] = cx + n * (cy + n * cz);
+        a.eexcl[x] = -1;
+        const double np3[3] = {px, py, pz};
+        const double op3[3] = {o.x, o.y, o.z};
+        for (int d = 0; d < 3; d++) {
+            if (!(np3[d] >= -0.5 * a.g.E && np3[d] <= 0.5 * a.g.E)) bad++;
+            double dd = np3[d] - op3[d];
+            if (a.g.bc != KMC_BC_FREE_SPACE) {
+                if (dd < -0.5 * a.g.E) dd += a.g.E;
+                if (dd > 0.5 * a.g.E) dd -= a.g.E;
+            }
+            if (fabs(dd) > a.max_move) bad++;
+        }
+    }
+    if (bad) atomicAdd(a.err, bad);
+}
+
+
+// ============================================================================
+// self term + long-range difference + combine (diff_lib)
+// ============================================================================
+struct CombineArgs {
+    Geom g;
+    int T;
+    double eu;
+    double self27;               // -sum_{o != 0} 1/|o E|
+    const double4* epos4;
+    const int32_t* owner;
+    const double* UD;
+    const double* UI;            // far-field partials [nfar][T+P]
+    int nfar;
+    const double* lpart;         // long range per proposal partials [12][T], or null
+    const double* cpart;         // long range per moving charge partials [12][P]
+    int P;
+    const int64_t* rate_loc;     // output index, or null (identity)
+    double* out;
+    double kT;
+    double* rates;               // rates per proposal (rate_rule), or null
+    double* bsum;                // their sums over blocks of 64 proposals
+    const int64_t* mask;         // proposals with mask <= 0 get rate 0, or null
+    int rate_rule;               // 0: min(1, exp(-dE/kT)) (Metropolis), 1: exp(-dE/kT)
+};
+
+__device__ __forceinline__ void add_parts(CSum& acc, double sign, const double* v, int nparts, size_t stride,
+                                          size_t i) {
+    for (int j = 0; j < nparts; j++) acc.add(sign * v[(size_t)j * stride + i]);
+}
+
+__device__ __forceinline__ double rate_of(double de, double kT, int rule) {
+    if (!(de == de) || isinf(de)) return 0.0;
+    const double w = exp(-de / kT);
+    if (rule == 1) return isinf(w) ? 0.0 : w;
+    return w < 1.0 ? w : 1.0;
+}
+
+// energy difference of proposal t (writes it at its output location):
+//   dE = eu [ (U1D + U1I) - (U0D + U0I) - USI ],  USI = q_o^2 e27 - (long-range parts)
+// (diff_lib, coulomb_kmc/kmc_fmm.py:583-663), every term added into one compensated
+// sum, so the old - new cancellation costs no more than one rounding of dE itself.
+template <bool MIXED>
+__device__ __forceinline__ double combine_one(const CombineArgs& a, int t) {
+    const int ow = a.owner[t];
+    const int po = a.T + ow;
+    const double4 xo = a.epos4[po];
+    const double4 xn = a.epos4[t];
+    // self term (kmc_fmm_self_interaction.py:48-54, 163-182, 227-236)
+    CSum e27;
+    if (a.g.bc == KMC_BC_FREE_SPACE) {
+        const double dx = xo.x - xn.x, dy = xo.y - xn.y, dz = xo.z - xn.z;
+        e27.add(rsqrt_sel<MIXED>(fma(dx, dx, fma(dy, dy, dz * dz))));
+    } else if (a.g.bc != KMC_BC_FF_ONLY) {
+        e27.add(a.self27);
+        for (int k = 0; k < 27; k++) {
+            int ox, oy, oz;
+            stencil(k, ox, oy, oz);
+            const double dx = ox * a.g.E + xo.x - xn.x;
+            const double dy = oy * a.g.E + xo.y - xn.y;
+            const double dz = oz * a.g.E + xo.z - xn.z;
+            e27.add(rsqrt_sel<MIXED>(fma(dx, dx, fma(dy, dy, dz * dz))));
+        }
+    }
+    CSum tot;
+    if (a.g.bc != KMC_BC_FF_ONLY) {
+        const size_t ne = (size_t)a.T + a.P;
+        tot.add(a.UD[t]);
+        add_parts(tot, 1.0, a.UI, a.nfar, ne, t);
+        tot.add(-a.UD[po]);
+        add_parts(tot, -1.0, a.UI, a.nfar, ne, po);
+    }
+    const double q2 = xo.w * xo.w;
+    tot.add_prod(-q2, e27.s);
+    tot.add(-q2 * e27.c);
+    if (a.lpart) {
+        add_parts(tot, 1.0, a.lpart, 12, a.T, t);
+        add_parts(tot, 1.0, a.cpart, 12, a.P, ow);
+    }
+    const double de = a.eu * tot.value();
+    const int64_t o = a.rate_loc ? a.rate_loc[t] : (int64_t)t;
+    a.out[o] = de;
+    return de;
+}
+
+// one lane per proposal, blocks of 64; with rates: the Metropolis rate per proposal
+// and the rate sum of each block of 64 (for the selection)
+template <bool MIXED>
+__device__ __forceinline__ void combine_body(const CombineArgs& a) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double rate = 0.0;
+    if (t < a.T) {
+        const double de = combine_one<MIXED>(a, t);
+        if (a.rates) {
+            rate = (a.mask && a.mask[t] <= 0) ? 0.0 : rate_of(de, a.kT, a.rate_rule);
+            a.rates[t] = rate;
+        }
+    }
+    if (a.rates && (int)blockIdx.x * 64 < a.T) {   // (a batched grid can run past this system's T)
+        const double bs = wave_sum(rate);
+        if (threadIdx.x == 0) a.bsum[blockIdx.x] = bs;
+    }
+}
+
+
+// ============================================================================
+// long range
+// ============================================================================
+// packed multipole of charge q at x about the origin: M_lm = q conj(R_l^m(x)), m >= 0.
+// Lane m (< L) writes column m into dst (stride 2 doubles per (l,m)).
+template <int L>
+__device__ __forceinline__ void multipole_column(int m, double x, double y, double z, double q, double* dst,
+                                                 double sign, bool accumulate) {
+    const double r2 = fma(x, x, fma(y, y, z * z));
+    Cplx C = {1.0, 0.0};
+    const Cplx w = {x, -y};
+    for (int k = 0; k < m; k++) C = cmul(C, w);
+    double b2 = 0.0, b1 = 0.0;
+#pragma unroll
+    for (int l = 0; l < L; l++) {
+        if (l < m) continue;
+        double b;
+        if (l == m) b = kRec.s[m];
+        else b = fma(kRec.a[l][m] * z, b1, -(kRec.b[l][m] * r2) * b2);
+        b2 = b1;
+        b1 = b;
+        const double vr = sign * q * b * C.re, vi = sign * q * b * C.im;
+        const int h = hidx(l, m);
+        if (accumulate) {
+            dst[2 * h] += vr;
+            dst[2 * h + 1] += vi;
+        } else {
+            dst[2 * h] = vr;
+            dst[2 * h + 1] = vi;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Long range, blocked form.  Under the cubic lattice's symmetry G is exactly
+// block diagonal over 12 blocks: (real | imaginary part) x (m mod 4 class:
+// {0}, {2}, odd) x (parity of l) - only lattice sums S_p^q with p even and
+// q = 0 mod 4 survive.  Entries of a block are ordered by m, then l.  The host
+// gathers the dense blocks (kmc_set_lr_operator); the kernels below keep a
+// block's multipole entries in registers and read the block coefficients as
+// wave-uniform (scalar) operands.
+//
+//   dLR(old - new) = -( v_n . a_o + 0.5 v_n G v_n + c_o ),
+//   a_o = g - G v_o,  c_o = 0.5 v_o G v_o - g . v_o,  g = G M,
+// with v = q conj(R(x)) the packed multipole of the moved charge.
+// ---------------------------------------------------------------------------
+constexpr int LRB_MAX = 64;
+struct LRBlocks {
+    int n[6];
+    int off[6];            // offset of the block pair (real block, then imaginary block) in gblk
+    int l[6][LRB_MAX];
+    int m[6][LRB_MAX];
+};
+__host__ __device__ constexpr int m_class(int m) { return (m % 4 == 0) ? 0 : ((m % 4 == 2) ? 1 : 2); }
+__host__ __device__ constexpr LRBlocks make_lr_blocks(int L) {
+    LRBlocks t{};
+    int off = 0;
+    for (int b = 0; b < 6; b++) {
+        const int c = b / 2, par = b % 2;
+        int k = 0;
+        for (int m = 0; m < L; m++) {
+            if (m_class(m) != c) continue;
+            for (int l = m; l < L; l++)
+                if ((l & 1) == par) {
+                    t.l[b][k] = l;
+                    t.m[b][k] = m;
+                    k++;
+                }
+        }
+        t.n[b] = k;
+        t.off[b] = off;
+        off += 2 * k * k;
+    }
+    return t;
+}
+template <int L>
+struct LRTab {
+    static constexpr LRBlocks T = make_lr_blocks(L);
+};
+
+// entries of block B of v = q conj(R(x)): real parts (COMP 0) or imaginary parts (COMP 1)
+template <int L, int B, int COMP>
+__device__ __forceinline__ void lr_block_values(double x, double y, double z, double q, double* v) {
+    constexpr int C = B / 2, PAR = B % 2;
+    const double r2 = fma(x, x, fma(y, y, z * z));
+    const Cplx w = {x, -y};
+    const Cplx w2 = cmul(w, w);
+    const Cplx w4 = cmul(w2, w2);
+    Cplx Cm = (C == 0) ? Cplx{1.0, 0.0} : ((C == 1) ? w2 : w);
+    const Cplx step = (C == 2) ? w2 : w4;
+    int k = 0;
+#pragma unroll
+    for (int m = 0; m < L; m++) {
+        if (m_class(m) != C) continue;
+        const double qc = q * (COMP == 0 ? Cm.re : Cm.im);
+        double b2 = 0.0, b1 = 0.0;
+#pragma unroll
+        for (int l = m; l < L; l++) {
+            const double bb = (l == m) ? kRec.s[m] : fma(kRec.a[l][m] * z, b1, -(kRec.b[l][m] * r2) * b2);
+            b2 = b1;
+            b1 = bb;
+            if ((l & 1) == PAR) v[k++] = bb * qc;
+        }
+        Cm = cmul(Cm, step);
+    }
+}
+
+__host__ __device__ constexpr int lr_nbmax(int L) {
+    const LRBlocks t = make_lr_blocks(L);
+    int m = 0;
+    for (int b = 0; b < 6; b++) m = t.n[b] > m ? t.n[b] : m;
+    return m + (m & 1);   // even: rows of a_o stay 16-byte aligned
+}
+
+struct LR2Args {
+    int P, T;
+    const double* gblk;
+    const double* gvec;
+    const double4* epos4;
+    const int32_t* owner;
+    double* aold;              // [12][P][NBMAX]  a_o per (block, component), row per moving charge
+    double* cpart;             // [12][P]   c_o per (block, component)
+    double* lpart;             // [12][T]   v.a_o + 0.5 v G v per (block, component)
+};
+
+// the G tile of (block B, component COMP) into LDS
+template <int L, int B, int COMP>
+__device__ __forceinline__ void lr_stage_g(const double* __restrict__ gblk, double* Gs) {
+    constexpr LRBlocks T = LRTab<L>::T;
+    constexpr int NB = T.n[B];
+    const double* G = gblk + T.off[B] + COMP * NB * NB;
+    for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) Gs[e] = G[e];
+    __syncthreads();
+}
+
+// per moving charge p: a_o = g - G v_o over one component of block B, its share of c_o
+template <int L, int B, int COMP>
+__device__ __forceinline__ void lr_block_part(const LR2Args& a, const double* __restrict__ gblk,
+                                              const double* __restrict__ g, double* Gs) {
+    constexpr LRBlocks T = LRTab<L>::T;
+    constexpr int NB = T.n[B];
+    constexpr int NBMAX = lr_nbmax(L);
+    constexpr int Y = 2 * B + COMP;
+    if constexpr (NB > 0) {
+        // the block's coefficients are wave-uniform: read straight from memory they
+        // become scalar loads (s_load into SGPRs), one per several FMAs
+        const double* __restrict__ G = gblk + T.off[B] + COMP * NB * NB;
+        (void)Gs;
+        const int p = blockIdx.x * blockDim.x + threadIdx.x;
+        if (p >= a.P) return;
+        const double4 xo = a.epos4[a.T + p];
+        double v[NB];
+        lr_block_values<L, B, COMP>(xo.x, xo.y, xo.z, xo.w, v);
+        double* ao = a.aold + ((size_t)Y * a.P + p) * NBMAX;
+        double c = 0.0;
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            const int h = 2 * hidx(T.l[B][i], T.m[B][i]) + COMP;
+            double yy = 0.0;
+#pragma unroll
+            for (int j = 0; j < NB; j++) yy = fma(G[i * NB + j], v[j], yy);
+            ao[i] = g[h] - yy;
+            c += v[i] * fma(0.5, yy, -g[h]);
+        }
+        a.cpart[(size_t)Y * a.P + p] = c;
+    } else {
+        const int p = blockIdx.x * blockDim.x + threadIdx.x;
+        if (p < a.P) a.cpart[(size_t)Y * a.P + p] = 0.0;
+    }
+}
+
+// per proposal t: v_n . a_o + 0.5 v_n G v_n over one component of block B
+template <int L, int B, int COMP>
+__device__ __forceinline__ void lr_block_prop(const LR2Args& a, const double* __restrict__ gblk, double* Gs) {
+    constexpr LRBlocks T = LRTab<L>::T;
+    constexpr int NB = T.n[B];
+    constexpr int NBMAX = lr_nbmax(L);
+    constexpr int Y = 2 * B + COMP;
+    if constexpr (NB > 0) {
+        const double* __restrict__ G = gblk + T.off[B] + COMP * NB * NB;   // wave-uniform: scalar loads
+        (void)Gs;
+        const int t = blockIdx.x * blockDim.x + threadIdx.x;
+        if (t >= a.T) return;
+        const int p = a.owner[t];
+        const double4 xn = a.epos4[t];
+        double v[NB];
+        lr_block_values<L, B, COMP>(xn.x, xn.y, xn.z, xn.w, v);
+        const double* ao = a.aold + ((size_t)Y * a.P + p) * NBMAX;
+        double acc = 0.0;
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            double s = fma(0.5 * G[i * NB + i], v[i], ao[i]);
+#pragma unroll
+            for (int j = 0; j < i; j++) s = fma(G[i * NB + j], v[j], s);
+            acc = fma(v[i], s, acc);
+        }
+        a.lpart[(size_t)Y * a.T + t] = acc;
+    } else {
+        const int t = blockIdx.x * blockDim.x + threadIdx.x;
+        if (t < a.T) a.lpart[(size_t)Y * a.T + t] = 0.0;
+    }
+}
+
+template <int L, int Y>
+__device__ __forceinline__ void lr_part_dispatch(int y, const LR2Args& a, const double* gblk, const double* g,
+                                                 double* Gs) {
+    if constexpr (Y < 12) {
+        if (y == Y) lr_block_part<L, Y / 2, Y % 2>(a, gblk, g, Gs);
+        else lr_part_dispatch<L, Y + 1>(y, a, gblk, g, Gs);
+    }
+}
+template <int L, int Y>
+__device__ __forceinline__ void lr_prop_dispatch(int y, const LR2Args& a, const double* gblk, double* Gs) {
+    if constexpr (Y < 12) {
+        if (y == Y) lr_block_prop<L, Y / 2, Y % 2>(a, gblk, Gs);
+        else lr_prop_dispatch<L, Y + 1>(y, a, gblk, Gs);
+    }
+}
+
+// grid.y = (block, component) pair Y; the workgroup stages that G tile in LDS once and
+// its lanes (one per moving charge / proposal) read it as broadcasts
+template <int L>
+__global__ void __launch_bounds__(256) k_lr_part(LR2Args a, const double* __restrict__ gblk,
+                                                 const double* __restrict__ gvec) {
+    constexpr int NBMAX = lr_nbmax(L);
+    __shared__ double Gs[NBMAX * NBMAX];
+    lr_part_dispatch<L, 0>((int)blockIdx.y, a, gblk, gvec, Gs);
+}
+
+template <int L>
+__global__ void __launch_bounds__(256) k_lr_prop2(LR2Args a, const double* __restrict__ gblk) {
+    constexpr int NBMAX = lr_nbmax(L);
+    __shared__ double Gs[NBMAX * NBMAX];
+    lr_prop_dispatch<L, 0>((int)blockIdx.y, a, gblk, Gs);
+}
+template <int L>
+__global__ void __launch_bounds__(256) kb_lr_part(const LR2Args* __restrict__ args) {
+    constexpr int NBMAX = lr_nbmax(L);
+    __shared__ double Gs[NBMAX * NBMAX];
+    const LR2Args& a = args[blockIdx.z];
+    lr_part_dispatch<L, 0>((int)blockIdx.y, a, a.gblk, a.gvec, Gs);
+}
+template <int L>
+__global__ void __launch_bounds__(256) kb_lr_prop2(const LR2Args* __restrict__ args) {
+    constexpr int NBMAX = lr_nbmax(L);
+    __shared__ double Gs[NBMAX * NBMAX];
+    const LR2Args& a = args[blockIdx.z];
+    lr_prop_dispatch<L, 0>((int)blockIdx.y, a, a.gblk, Gs);
+}
+
+
+// level-0 multipole of a particle range: per block partial sums (deterministic),
+// lane m of each wave owns column m.
+template <int L>
+__global__ void __launch_bounds__(64) k_multipole_partial(int n, int chunk, const double4* pos4, double* partial) {
+    constexpr int NR = 2 * nhalf(L);
+    const int b = blockIdx.x;
+    const int m = threadIdx.x;
+    double* dst = partial + (size_t)b * NR;
+    if (m >= L) return;
+    for (int l = m; l < L; l++) {
+        dst[2 * hidx(l, m)] = 0.0;
+        dst[2 * hidx(l, m) + 1] = 0.0;
+    }
+    const int i0 = b * chunk, i1 = min(n, i0 + chunk);
+    for (int i = i0; i < i1; i++) {
+        const double4 p = pos4[i];
+        multipole_column<L>(m, p.x, p.y, p.z, p.w, dst, 1.0, true);
+    }
+}
+
+// ============================================================================
+// accept: move record, cell tables
+// ============================================================================
+
+// kmc_local.py:180-271 (extract: swap the last occupant into the freed slot) then
+// kmc_local.py:274-359 (inject: append), then the particle arrays.
+__device__ void store_update(const MoveRec* rec, int* occ, double4* slot_pos, int32_t* slot_id, int occmax,
+                             double4* ppos4, int32_t* pcell, int* flags) {
+    const int pid = rec->pid;
+    const int oc = rec->old_cell, nc = rec->new_cell;
+    const size_t ob = (size_t)oc * occmax, nb = (size_t)nc * occmax;
+    const int oo = occ[oc];
+    int index = -1, found = 0;
+    for (int s = 0; s < oo; s++)
+        if (slot_id[ob + s] == pid) { index = s; found++; }
+    if (found != 1) { atomicOr(flags, 1); return; }
+    slot_id[ob + index] = slot_id[ob + oo - 1];
+    slot_pos[ob + index] = slot_pos[ob + oo - 1];
+    occ[oc] = oo - 1;
+    const int no = occ[nc];
+    if (no >= occmax) { atomicOr(flags, 2); return; }
+    slot_id[nb + no] = pid;
+    slot_pos[nb + no] = make_double4(rec->new_pos[0], rec->new_pos[1], rec->new_pos[2], rec->q);
+    occ[nc] = no + 1;
+    ppos4[pid] = make_double4(rec->new_pos[0], rec->new_pos[1], rec->new_pos[2], rec->q);
+    pcell[pid] = nc;
+}
+
+struct TailArgs {
+    const MoveRec* rec;
+    int* occ;
+    double4* slot_pos;
+    int32_t* slot_id;
+    int occmax;
+    double4* ppos4;
+    int32_t* pcell;
+    int* flags;
+    int lr;                      // update M, g = G M and E_lr
+    double* M;
+    const double* gdat;
+    const int64_t* gptr;
+    const int32_t* gidx;
+    double* g;
+    double* elr;
+    int64_t* counter;            // step counter to advance, or null
+};
+
+// everything of an accept except the leaf sweep, in one block: cell tables and
+// particle arrays (thread 0), M += m(old, -q) then += m(new, +q)
+// (kmc_full_long_range.py:270-316), g = G M, E_lr = 0.5 M.g, step counter.
+template <int L>
+__device__ __forceinline__ void accept_tail_body(const TailArgs& a) {
+    constexpr int NR = 2 * nhalf(L);
+    __shared__ double dold[NR], dnew[NR];
+    __shared__ double part[256];
+    const int tid = threadIdx.x;
+    const bool valid = a.rec->valid != 0;
+    if (valid && tid == 0) store_update(a.rec, a.occ, a.slot_pos, a.slot_id, a.occmax, a.ppos4, a.pcell, a.flags);
+    if (valid && a.lr) {
+        const MoveRec* r = a.rec;
+        if (tid < L) {
+            multipole_column<L>(tid, r->old_pos[0], r->old_pos[1], r->old_pos[2], r->q, dold, -1.0, false);
+            multipole_column<L>(tid, r->new_pos[0], r->new_pos[1], r->new_pos[2], r->q, dnew, 1.0, false);
+        }
+        __syncthreads();
+        for (int k = tid; k < NR; k += blockDim.x) a.M[k] = (a.M[k] + dold[k]) + dnew[k];
+        __syncthreads();
+        double acc = 0.0;
+        for (int row = tid; row < NR; row += blockDim.x) {
+            double y = 0.0;
+            for (int64_t k = a.gptr[row]; k < a.gptr[row + 1]; k++) y = fma(a.gdat[k], a.M[a.gidx[k]], y);
+            a.g[row] = y;
+            acc = fma(a.M[row], y, acc);
+        }
+        part[tid] = acc;
+        __syncthreads();
+        for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+            if (tid < o) part[tid] += part[tid + o];
+            __syncthreads();
+        }
+        if (tid == 0) *a.elr = 0.5 * part[0];
+    }
+    if (tid == 0 && a.counter) *a.counter += 1;
+}
+template <int L>
+__global__ void __launch_bounds__(256) k_accept_tail(TailArgs a) { accept_tail_body<L>(a); }
+template <int L>
+__global__ void __launch_bounds__(256) kb_accept_tail(const TailArgs* __restrict__ args) {
+    accept_tail_body<L>(args[blockIdx.z]);
+}
+
+
+// ============================================================================
+// Metropolis selection on the device (one block)
+// ============================================================================
+// row of a cyclic per-step table for a step counter: counter mod nsteps in [0, nsteps)
+// for every counter value, negative ones included
+__device__ __forceinline__ int64_t step_of(int64_t counter, int64_t nsteps) {
+    const int64_t r = counter % nsteps;
+    return r < 0 ? r + nsteps : r;
+}
+
+struct SelectArgs {
+    Geom g;
+    int T;
+    int n;
+    int64_t nsteps;
+    double kT;
+    const double* dE;
+    const double* rates;
+    const double* bsum;
+    const double* uniform;
+    const int64_t* counter;
+    int64_t* selected;
+    const double4* epos4;
+    const int32_t* ecell;
+    const int32_t* owner;
+    const int64_t* pid;
+    const double4* ppos4;
+    const int32_t* pcell;
+    const int* occ;
+    int occmax;
+    double* energy;
+    MoveRec* rec;
+    int* flags;
+    int* err;                    // bad-move count of this step's set-up (k_setup); reset here
+};
+
+
+// Rates and their sums over blocks of 64 proposals come from k_combine.  Each thread
+// sums a contiguous chunk of block sums; chunk sums are scanned within each wave
+// (shuffles) and across the 16 waves (LDS); the thread whose chunk holds u * total
+// walks its block sums, then that block's rates, to find the proposal.
+__device__ __forceinline__ void select_body(const SelectArgs& a) {
+    __shared__ double wsum[16];
+    __shared__ int pick;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wid = tid >> 6;
+    const int nt = blockDim.x;
+    const int nw = nt >> 6;
+    const int nb = (a.T + 63) / 64;
+    const int chunk = (nb + nt - 1) / nt;
+    const int b0 = min(nb, tid * chunk), b1 = min(nb, b0 + chunk);
+    double s = 0.0;
+    for (int b = b0; b < b1; b++) s += a.bsum[b];
+    if (tid == 0) pick = -1;
+    double v = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const double y = __shfl_up(v, o, 64);
+        if (lane >= o) v += y;
+    }
+    if (lane == 63) wsum[wid] = v;
+    __syncthreads();
+    if (wid == 0) {
+        double w = lane < nw ? wsum[lane] : 0.0;
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+            const double y = __shfl_up(w, o, 64);
+            if (lane >= o) w += y;
+        }
+        if (lane < nw) wsum[lane] = w;
+    }
+    __syncthreads();
+    // chunk intervals [lo, incl): each lower bound is the very value its predecessor
+    // computed as upper bound, so the intervals tile [0, total) without gaps
+    const double incl = v + (wid > 0 ? wsum[wid - 1] : 0.0);
+    __shared__ double wlast[16];
+    if (lane == 63) wlast[wid] = incl;
+    const double prev = __shfl_up(incl, 1, 64);
+    __syncthreads();
+    const double lo = lane > 0 ? prev : (wid > 0 ? wlast[wid - 1] : 0.0);
+    const int64_t step = step_of(*a.counter, a.nsteps);
+    const double total = wlast[nw - 1];
+    const double target = a.uniform[step] * total;
+    if (target >= lo && target < incl && b1 > b0) {
+        double c = lo;
+        int k = -1;
+        for (int b = b0; b < b1 && k < 0; b++) {
+            const double cb = c + a.bsum[b];
+            if (target < cb || b == b1 - 1) {
+                const int t0 = b * 64, t1 = min(a.T, t0 + 64);
+                for (int t = t0; t < t1; t++) {
+                    c += a.rates[t];
+                    if (target < c) { k = t; break; }
+                }
+                if (k < 0) k = t1 - 1;
+            } else {
+                c = cb;
+            }
+        }
+        pick = k;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        // a proposal outside the domain or beyond max_move (or a malformed es / pid):
+        // nothing is accepted, flag 32 (the host paths raise KMC_E_BAD_MOVE instead)
+        const int bad = *a.err;
+        *a.err = 0;
+        if (bad) {
+            a.selected[step] = -1;
+            a.rec->valid = 0;
+            atomicOr(a.flags, 32);
+            return;
+        }
+        int k = pick;
+        if (k < 0) {   // target == total (u == 1) or no positive rate: last positive rate
+            for (int b = nb - 1; b >= 0 && k < 0; b--) {
+                if (!(a.bsum[b] > 0.0)) continue;
+                for (int t = min(a.T, b * 64 + 64) - 1; t >= b * 64; t--)
+                    if (a.rates[t] > 0.0) { k = t; break; }
+            }
+        }
+        MoveRec* r = a.rec;
+        a.selected[step] = k;
+        if (k < 0) { r->valid = 0; atomicOr(a.flags, 4); return; }
+        const int p = a.owner[k];
+        const int pid = (int)a.pid[p];
+        if (pid < 0 || pid >= a.n) { r->valid = 0; atomicOr(a.flags, 8); return; }
+        const double4 np = a.epos4[k];
+        const double4 op = a.ppos4[pid];
+        r->old_pos[0] = op.x; r->old_pos[1] = op.y; r->old_pos[2] = op.z;
+        r->new_pos[0] = np.x; r->new_pos[1] = np.y; r->new_pos[2] = np.z;
+        r->q = op.w;
+        r->pid = pid;
+        r->old_cell = a.pcell[pid];
+        r->new_cell = a.ecell[k];
+        r->dE = a.dE[k];
+        const bool full = (r->new_cell != r->old_cell) && (a.occ[r->new_cell] >= a.occmax);
+        if (full) { r->valid = 0; atomicOr(a.flags, 2); return; }
+        r->valid = 1;
+        *a.energy += a.dE[k];
+    }
+}
+
+
+
+// ============================================================================
+// propose_with_dats set-up (mask compaction, kmc_mpi_decomp.py:548-614)
+// ============================================================================
+
+
+
+// synthetic move stream (benchmarks/tests)
+struct MatArgs {
+    Geom g;
+    int n;
+    int64_t nsteps;
+    int P, k;
+    const int64_t* pids_all;
+    const double* disp_all;
+    const int64_t* counter;
+    const double4* ppos4;
+    int64_t* pid;
+    int64_t* es;
+    double* new_pos;
+    int* flags;
+};
+__device__ __forceinline__ void materialize_body(const MatArgs& a) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int P = a.P, k = a.k;
+    const int T = P * k;
+    const int64_t step = step_of(*a.counter, a.nsteps);
+    if (t <= P && t >= 0 && t < P + 1) a.es[t] = (int64_t)t * k;
+    if (t >= T) return;
+    const int p = t / k;
+    int64_t i = a.pids_all[step * P + p];
+    if (i < 0 || i >= a.n) { atomicOr(a.flags, 16); i = 0; }
+    if (t % k == 0) a.pid[p] = i;
+    const double4 o = a.ppos4[i];
+    const double* d = a.disp_all + (step * T + t) * 3;
+    double v[3] = {o.x + d[0], o.y + d[1], o.z + d[2]};
+    const double h = 0.5 * a.g.E;
+    for (int c = 0; c < 3; c++) {
+        if (a.g.bc == KMC_BC_FREE_SPACE) {
+            v[c] = v[c] < -h ? -h : (v[c] > h ? h : v[c]);
+        } else {
+            if (v[c] < -h) v[c] += a.g.E;
+            if (v[c] >= h) v[c] -= a.g.E;
+        }
+        a.new_pos[3 * t + c] = v[c];
+    }
+}
+
+
+
+// ============================================================================
+// initial solve, hierarchical (large systems; coulomb_kmc/kmc_fmm.py:463-476,
+// 956-959 take the initial leaf expansions from ppmd's FMM)
+// ============================================================================
+// Working variables are scaled Racah harmonics (m >= 0 stored, packed index
+// l(l+1)/2 + m; the m < 0 entries are (-1)^m conj of the m > 0 ones):
+//   R~_l^m(x) = R_l^m(x) / N_lm,  I~_l^m(x) = I_l^m(x) N_lm,  N_lm = sqrt((l-m)!(l+m)!)
+// (R_l^m = r^l Y_l^m, I_l^m = r^{-l-1} Y_l^m as in the rest of this file).  At a level
+// with cell width w:
+//   multipole  Qh_lm = sum_i q_i conj(R~_l^m((x_i - c) / w))
+//   local      Lh_jk = w^{j+1} N_jk L_jk   (L_jk the leaf-expansion layout)
+// so the translations do not depend on the level:
+//   M2M  Qh_p[j,k] = 2^-j sum Qh_c[l,m] conj(R~_{j-l}^{k-m}(b)),  b = (child - parent) / w_child
+//   M2L  Lh_t[j,k] = (-1)^k sum (-1)^l I~_{j+l}^{m-k}(o) Qh_s[l,m],  o = (c_s - c_t) / w
+//   L2L  Lh_c[l,m] = sum_{j>=l} 2^-(j+1) Lh_p[j,k] R~_{j-l}^{k-m}(b)
+// The two finest levels' interaction lists (the 12^3 leaves around the leaf's
+// grandparent block, minus the 27 near leaves) are added by direct P2L of their
+// charges; M2L runs on levels 1 (2 for free space) .. R-2 only.
+
+__host__ __device__ inline double rec_alpha(int l, int m) { return (2.0 * l - 1.0) / sqrt((double)(l * l - m * m)); }
+__host__ __device__ inline double rec_beta(int l, int m) {
+    return sqrt((double)((l - 1) * (l - 1) - m * m) / (double)(l * l - m * m));
+}
+__host__ __device__ inline void half_lm(int h, int& l, int& m) {
+    l = 0;
+    while ((l + 1) * (l + 2) / 2 <= h) l++;
+    m = h - l * (l + 1) / 2;
+}
+
+// regular (IRR = false) or irregular (IRR = true) Schmidt solid harmonic of one (l, m), m >= 0
+template <bool IRR>
+__host__ __device__ inline Cplx solid_lm(int l, int m, double x, double y, double z) {
+    const double r2 = x * x + y * y + z * z;
+    double s = 1.0;
+    for (int k = 1; k <= m; k++) s *= -sqrt((2.0 * k - 1.0) / (2.0 * k));
+    Cplx w = {1.0, 0.0};
+    for (int k = 0; k < m; k++) w = Cplx{w.re * x - w.im * y, w.re * y + w.im * x};
+    double b1 = s, b2 = 0.0;
+    if (IRR) {
+        const double ir2 = 1.0 / r2;
+        double p = 1.0 / sqrt(r2);
+        for (int k = 0; k < m; k++) p *= ir2;
+        b1 = s * p;
+        for (int k = m + 1; k <= l; k++) {
+            const double b = (rec_alpha(k, m) * z * b1 - rec_beta(k, m) * b2) * ir2;
+            b2 = b1;
+            b1 = b;
+        }
+    } else {
+        for (int k = m + 1; k <= l; k++) {
+            const double b = rec_alpha(k, m) * z * b1 - rec_beta(k, m) * r2 * b2;
+            b2 = b1;
+            b1 = b;
+        }
+    }
+    return Cplx{b1 * w.re, b1 * w.im};
+}
+
+__device__ __forceinline__ double2 conj_sgn(double2 v, int m) {   // (-1)^m conj(v)
+    return (m & 1) ? make_double2(-v.x, v.y) : make_double2(v.x, -v.y);
+}
+// entry (l, m) of a packed half expansion for any -l <= m <= l
+__device__ __forceinline__ double2 half_get(const double2* a, int l, int m) {
+    const double2 v = a[hidx(l, m < 0 ? -m : m)];
+    return m < 0 ? conj_sgn(v, -m) : v;
+}
+
+struct FmmLevel {
+    int n;                 // cells per side
+    double w;              // cell width
+};
+
+
+
+
+// M2L at one level: block per target cell, lane per packed output (j, k).  For each
+// source of the target's interaction list (children of the parent's neighbours not
+// adjacent to the target; periodic: indices wrap, the displacement does not) the
+// block builds the I~(o) table in LDS and stages the source multipole.
+constexpr int FMM_M2L_THREADS = 256;
+
+// Leaf P2L of the charges in the 12^3 leaves around the leaf's grandparent block
+// (offsets o = -4-b .. 7-b per axis, b = leaf index mod 4) except its 27 near leaves.
+// One wave per 64 leaves sharing b; grid.y = column-pair group (as the accept sweep).
+template <int L, int G, int NP>
+__device__ __forceinline__ void p2l_near_group(const SweepArgs& a, const int* occ, const double4* slot_pos, int occmax,
+                                               int cell, int bx, int by, int bz) {
+    constexpr int P0 = G * NP;
+    constexpr int W = L + 1;
+    double ar[NP][W], ai[NP][W];
+#pragma unroll
+    for (int jj = 0; jj < NP; jj++)
+#pragma unroll
+        for (int kk = 0; kk < W; kk++) { ar[jj][kk] = 0.0; ai[jj][kk] = 0.0; }
+    const int n = a.g.ncps;
+    const double E = a.g.E;
+    const bool periodic = a.g.bc != KMC_BC_FREE_SPACE;
+    const int tx = cell % n, ty = (cell / n) % n, tz = cell / (n * n);
+    const double cx = a.g.c0 + tx * a.g.w, cy = a.g.c0 + ty * a.g.w, cz = a.g.c0 + tz * a.g.w;
+#pragma unroll 1
+    for (int oz = -4 - bz; oz <= 7 - bz; oz++) {
+        int sz = tz + oz;
+        double shz = 0.0;
+        if (sz < 0) { sz += n; shz = -E; } else if (sz >= n) { sz -= n; shz = E; }
+        if (!periodic && shz != 0.0) continue;
+#pragma unroll 1
+        for (int oy = -4 - by; oy <= 7 - by; oy++) {
+            int sy = ty + oy;
+            double shy = 0.0;
+            if (sy < 0) { sy += n; shy = -E; } else if (sy >= n) { sy -= n; shy = E; }
+            if (!periodic && shy != 0.0) continue;
+#pragma unroll 1
+            for (int ox = -4 - bx; ox <= 7 - bx; ox++) {
+                if (ox >= -1 && ox <= 1 && oy >= -1 && oy <= 1 && oz >= -1 && oz <= 1) continue;
+                int sx = tx + ox;
+                double shx = 0.0;
+                if (sx < 0) { sx += n; shx = -E; } else if (sx >= n) { sx -= n; shx = E; }
+                if (!periodic && shx != 0.0) continue;
+                const int s = sx + n * (sy + n * sz);
+                const int o = occ[s];
+#pragma unroll 1
+                for (int q = 0; q < o; q++) {
+                    const double4 p = slot_pos[(size_t)s * occmax + q];
+                    const double dx[1] = {(p.x + shx) - cx};
+                    const double qe[1] = {p.w};
+                    sweep_images<L, P0, NP, 1>(dx, (p.y + shy) - cy, (p.z + shz) - cz, qe, ar, ai);
+                }
+            }
+        }
+    }
+    sweep_group_store<L, P0, NP>(a.lexp + (size_t)cell * nhalf(L), ar, ai);
+}
+template <int L, int NP, int G0>
+__device__ __forceinline__ void p2l_near_dispatch(const SweepArgs& a, const int* occ, const double4* slot_pos, int occmax,
+                                                  int cell, int group, int bx, int by, int bz) {
+    constexpr int NG = ((L + 1) / 2 + NP - 1) / NP;
+    if constexpr (G0 < NG) {
+        if (group == G0) p2l_near_group<L, G0, NP>(a, occ, slot_pos, occmax, cell, bx, by, bz);
+        else p2l_near_dispatch<L, NP, G0 + 1>(a, occ, slot_pos, occmax, cell, group, bx, by, bz);
+    }
+}
+// grid.x = 64 (b classes) x ceil(ng^3 / 64) chunks, ng = ncps / 4; block = 64
+template <int L>
+__global__ void __launch_bounds__(64) k_fmm_p2l_near(SweepArgs a, const int* occ, const double4* slot_pos, int occmax) {
+    const int b = blockIdx.x % 64;
+    const int chunk = blockIdx.x / 64;
+    const int bx = b & 3, by = (b >> 2) & 3, bz = b >> 4;
+    const int ng = a.g.ncps / 4;
+    const int gi = chunk * 64 + threadIdx.x;
+    if (gi >= ng * ng * ng) return;
+    const int gx = gi % ng, gy = (gi / ng) % ng, gz = gi / (ng * ng);
+    const int cell = (4 * gx + bx) + a.g.ncps * ((4 * gy + by) + a.g.ncps * (4 * gz + bz));
+    p2l_near_dispatch<L, 1, 0>(a, occ, slot_pos, occmax, cell, (int)blockIdx.y, bx, by, bz);
+}
+
+// ============================================================================
+// potential at points (KMCFMM.eval_field, coulomb_kmc/kmc_fmm.py:544-567)
+// ============================================================================
+
+
+
+// ============================================================================
+// charge energies, energies of small groups, injection and extraction of charges
+// (coulomb_kmc/kmc_inject_extract.py:82-424)
+// ============================================================================
+
+
+
+// column m of the packed multipole of charge p.w at p about the origin, added to Mb
+// (the vector v(x) of k_eval_lr scaled by the charge: M = sum_i q_i v(x_i))
+__device__ void multipole_column_add(int L, int m, double4 p, double* Mb) {
+    const double r2 = p.x * p.x + p.y * p.y + p.z * p.z;
+    double s = 1.0;
+    Cplx C = {1.0, 0.0};
+    for (int k = 1; k <= m; k++) {
+        s *= -sqrt((2.0 * k - 1.0) / (2.0 * k));
+        C = Cplx{C.re * p.x + C.im * p.y, C.im * p.x - C.re * p.y};
+    }
+    double b2 = 0.0, b1 = s;
+    for (int l = m; l < L; l++) {
+        if (l > m) {
+            const double b = rec_alpha(l, m) * p.z * b1 - rec_beta(l, m) * r2 * b2;
+            b2 = b1;
+            b1 = b;
+        }
+        const int h = hidx(l, m);
+        Mb[2 * h] = fma(p.w * b1, C.re, Mb[2 * h]);
+        Mb[2 * h + 1] = fma(p.w * b1, C.im, Mb[2 * h + 1]);
+    }
+}
+
+
+
+
+
+
+
+
+
+
+// ============================================================================
+// lattice bookkeeping on the device (Algorithm 2 of the reference's examples:
+// docs/source/example_bookkeeping.rst, examples/propose_with_dats_example.py:217-312)
+// ============================================================================
+
+
+
+// ============================================================================
+// order dispatch tables
+// ============================================================================
+typedef void (*SweepLaunch)(SweepArgs, int, hipStream_t, int, int, int);
+typedef void (*EvalLaunch)(EvalArgs, hipStream_t);
+typedef void (*LRLaunch)(LR2Args, hipStream_t);
+typedef void (*MpLaunch)(int, int, const double4*, double*, int, hipStream_t);
+typedef void (*MdLaunch)(TailArgs, hipStream_t);
+typedef void (*P2LNearLaunch)(SweepArgs, const int*, const double4*, int, hipStream_t);
+
+template <int L>
+static void launch_p2l_near(SweepArgs a, const int* occ, const double4* slot_pos, int occmax, hipStream_t s) {
+    const int ng = a.g.ncps / 4;
+    const int chunks = (ng * ng * ng + 63) / 64;
+    hipLaunchKernelGGL((k_fmm_p2l_near<L>), dim3(64 * chunks, (L + 1) / 2), dim3(64), 0, s, a, occ, slot_pos, occmax);
+}
+
+template <int L, int NP>
+static void launch_sweep_np(SweepArgs a, int ncells, hipStream_t s, int block, int split) {
+    const int ng = ((L + 1) / 2 + NP - 1) / NP;
+    if (block == 0 && ncells % 64 == 0 && 64 * ng <= 512) {
+        if (a.g.bc == KMC_BC_FREE_SPACE)
+            hipLaunchKernelGGL((k_sweep_fused<L, NP, false>), dim3(ncells / 64), dim3(64, ng), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_sweep_fused<L, NP, true>), dim3(ncells / 64), dim3(64, ng), 0, s, a);
+        return;
+    }
+    if (block == 0) block = 64;
+    if (a.g.bc == KMC_BC_FREE_SPACE) {
+        hipLaunchKernelGGL((k_sweep<L, NP, false, 1>), dim3((ncells + block - 1) / block, ng), dim3(block), 0, s, a);
+    } else if (split == 3) {
+        hipLaunchKernelGGL((k_sweep<L, NP, true, 3>), dim3((ncells + 63) / 64, ng), dim3(64, 3), 0, s, a);
+    } else {
+        hipLaunchKernelGGL((k_sweep<L, NP, true, 1>), dim3((ncells + block - 1) / block, ng), dim3(block), 0, s, a);
+    }
+}
+template <int L>
+static void launch_sweep(SweepArgs a, int ncells, hipStream_t s, int block, int np, int split) {
+    if (np == 1) launch_sweep_np<L, 1>(a, ncells, s, block, split);
+    else launch_sweep_np<L, 2>(a, ncells, s, block, split);
+}
+
+template <int L>
+static void launch_far(EvalArgs a, hipStream_t s) {
+    if (a.n <= 0) return;
+    constexpr int npair = (L + 1) / 2;
+    constexpr int py = npair <= 8 ? npair : (npair + 1) / 2;   // pairs per workgroup, <= 8 waves
+    hipLaunchKernelGGL(k_far<L>, dim3((a.n + 63) / 64, (npair + py - 1) / py), dim3(64, py), 0, s, a);
+}
+template <int L>
+static void launch_lr(LR2Args a, hipStream_t s) {
+    if (a.T <= 0 || a.P <= 0) return;
+    hipLaunchKernelGGL(k_lr_part<L>, dim3((a.P + 255) / 256, 12), dim3(256), 0, s, a, a.gblk, a.gvec);
+    hipLaunchKernelGGL(k_lr_prop2<L>, dim3((a.T + 255) / 256, 12), dim3(256), 0, s, a, a.gblk);
+}
+template <int L>
+static void launch_mp(int n, int chunk, const double4* p, double* partial, int nblk, hipStream_t s) {
+    hipLaunchKernelGGL(k_multipole_partial<L>, dim3(nblk), dim3(64), 0, s, n, chunk, p, partial);
+}
+template <int L>
+static void launch_md(TailArgs a, hipStream_t s) {
+    hipLaunchKernelGGL(k_accept_tail<L>, dim3(1), dim3(256), 0, s, a);
+}
+
+// batched launches: argument tables in device memory, system = blockIdx.z
+struct BatchShape {
+    int n;           // systems in the table
+    int max_ne;      // largest T + P
+    int max_P, max_T;
+    int max_cells;   // largest leaf count (a multiple of 64)
+};
+typedef void (*BFarLaunch)(const EvalArgs*, BatchShape, hipStream_t);
+typedef void (*BLRLaunch)(const LR2Args*, BatchShape, hipStream_t);
+typedef void (*BSweepLaunch)(const SweepArgs*, BatchShape, bool, hipStream_t);
+typedef void (*BTailLaunch)(const TailArgs*, BatchShape, hipStream_t);
+
+template <int L>
+static void blaunch_far(const EvalArgs* t, BatchShape b, hipStream_t s) {
+    constexpr int npair = (L + 1) / 2;
+    constexpr int py = npair <= 8 ? npair : (npair + 1) / 2;
+    hipLaunchKernelGGL(kb_far<L>, dim3((b.max_ne + 63) / 64, (npair + py - 1) / py, b.n), dim3(64, py), 0, s, t);
+}
+template <int L>
+static void blaunch_lr(const LR2Args* t, BatchShape b, hipStream_t s) {
+    hipLaunchKernelGGL(kb_lr_part<L>, dim3((b.max_P + 255) / 256, 12, b.n), dim3(256), 0, s, t);
+    hipLaunchKernelGGL(kb_lr_prop2<L>, dim3((b.max_T + 255) / 256, 12, b.n), dim3(256), 0, s, t);
+}
+// fused accept sweep, two column pairs per lane
+template <int L>
+static void blaunch_sweep(const SweepArgs* t, BatchShape b, bool per, hipStream_t s) {
+    constexpr int ng = ((L + 1) / 2 + 1) / 2;
+    if (per) hipLaunchKernelGGL((kb_sweep_fused<L, 2, true>), dim3(b.max_cells / 64, 1, b.n), dim3(64, ng), 0, s, t);
+    else hipLaunchKernelGGL((kb_sweep_fused<L, 2, false>), dim3(b.max_cells / 64, 1, b.n), dim3(64, ng), 0, s, t);
+}
+template <int L>
+static void blaunch_tail(const TailArgs* t, BatchShape b, hipStream_t s) {
+    hipLaunchKernelGGL(kb_accept_tail<L>, dim3(1, 1, b.n), dim3(256), 0, s, t);
+}
+
+struct OrderFns {
+    SweepLaunch sweep;
+    EvalLaunch far;
+    LRLaunch lr;
+    MpLaunch mp;
+    MdLaunch md;
+    P2LNearLaunch p2ln;
+    BFarLaunch bfar;
+    BLRLaunch blr;
+    BSweepLaunch bsweep;
+    BTailLaunch btail;
+};
+template <int L>
+static OrderFns fns_for() {
+    return {launch_sweep<L>, launch_far<L>,  launch_lr<L>,    launch_mp<L>,     launch_md<L>,
+            launch_p2l_near<L>, blaunch_far<L>, blaunch_lr<L>, blaunch_sweep<L>, blaunch_tail<L>};
+}

@@ -297,7 +297,15 @@ int kmc_lattice_step(kmc_ctx* ctx, void* stream, int64_t M, const double* h_offs
                      double* d_new_pos, int64_t* d_masks, double* d_dE, int64_t nsteps, const double* d_uniform,
                      double kT, int32_t rate_rule, int64_t* d_counter, int64_t* d_selected, int32_t precision);
 
-/* Flags raised by device-side steps (store overflow, bad moves); cleared by the call. */
+/*
+ * Flags raised by device-side steps (kmc_step, kmc_run_steps, kmc_lattice_step, batches),
+ * or-ed into *h_flags and cleared by the call: 1 the moved charge was not found exactly
+ * once in its leaf's table, 2 the selected move's target leaf was full
+ * (not accepted), 4 no proposal had a positive rate, 8 a moving-charge id out of range,
+ * 16 a synthetic move-stream row out of range, 32 a proposal outside the domain or
+ * beyond max_move, or malformed es (nothing accepted in that step; the host propose
+ * paths return KMC_E_BAD_MOVE for the same inputs).
+ */
 int kmc_check_flags(kmc_ctx* ctx, void* stream, int32_t* h_flags);
 
 /*

@@ -161,6 +161,84 @@ static void dot_vec(const Basis& B, double q, double r, double th, double ph, do
     }
 }
 
+// ---------------------------------------------------------------------------
+// The same basis routines on a real type R (double or long double): used by the
+// checker below, which re-evaluates proposal energies of the stored state in
+// extended precision.  Same formulas as Basis / local_eval / p2m / dot_vec above.
+// ---------------------------------------------------------------------------
+template <typename R>
+struct BasisT {
+    int L;
+    std::vector<R> H;
+    explicit BasisT(int L_) : L(L_), H(L_ * L_, R(0)) {
+        for (int l = 0; l < L; l++)
+            for (int m = 0; m <= l; m++)
+                H[l * L + m] = std::exp(R(0.5) * (std::lgamma(R(l - m + 1)) - std::lgamma(R(l + m + 1))));
+    }
+    void Y(R theta, R phi, R* yre, R* yim) const {
+        const R ct = std::cos(theta), st = std::sin(theta);
+        std::vector<R> P(L * L, R(0));
+        R pmm = 1;
+        for (int m = 0; m < L; m++) {
+            if (m > 0) pmm *= -(R(2) * m - 1) * st;
+            P[m * L + m] = pmm;
+            if (m + 1 < L) P[(m + 1) * L + m] = (R(2) * m + 1) * ct * pmm;
+            for (int l = m + 2; l < L; l++)
+                P[l * L + m] = ((R(2) * l - 1) * ct * P[(l - 1) * L + m] - (R(l + m) - 1) * P[(l - 2) * L + m]) / R(l - m);
+        }
+        for (int l = 0; l < L; l++)
+            for (int m = 0; m <= l; m++) {
+                const R v = H[l * L + m] * P[l * L + m];
+                yre[l * L + m] = v * std::cos(R(m) * phi);
+                yim[l * L + m] = v * std::sin(R(m) * phi);
+            }
+    }
+};
+template <typename R>
+static inline void spherical_t(R dx, R dy, R dz, R* r, R* th, R* ph) {
+    const R xy = dx * dx + dy * dy;
+    *r = std::sqrt(xy + dz * dz);
+    *th = std::atan2(std::sqrt(xy), dz);
+    *ph = std::atan2(dy, dx);
+}
+template <typename R>
+static R local_eval_t(const BasisT<R>& B, const double* lexp, R r, R th, R ph) {
+    const int L = B.L, L2 = L * L;
+    std::vector<R> yre(L2), yim(L2);
+    B.Y(th, ph, yre.data(), yim.data());
+    R e = 0, rl = 1;
+    for (int l = 0; l < L; l++) {
+        for (int m = -l; m <= l; m++) {
+            const int am = m < 0 ? -m : m;
+            const R yr = yre[l * L + am];
+            const R yi = m < 0 ? -yim[l * L + am] : yim[l * L + am];
+            const int idx = l * l + l + m;
+            e += rl * (R(lexp[idx]) * yr - R(lexp[L2 + idx]) * yi);
+        }
+        rl *= r;
+    }
+    return e;
+}
+// P2M (sign of the imaginary part +) or dot vector (-) of charge q at (r, th, ph) into out
+template <typename R>
+static void p2m_t(const BasisT<R>& B, R q, R r, R th, R ph, R* out, bool dotvec) {
+    const int L = B.L, L2 = L * L;
+    std::vector<R> yre(L2), yim(L2);
+    B.Y(th, ph, yre.data(), yim.data());
+    R rl = 1;
+    for (int l = 0; l < L; l++) {
+        for (int m = -l; m <= l; m++) {
+            const int am = m < 0 ? -m : m;
+            const R yr = yre[l * L + am];
+            const R yi = m < 0 ? -yim[l * L + am] : yim[l * L + am];
+            const int idx = l * l + l + m;
+            out[idx] += yr * rl * q;
+            out[L2 + idx] += (dotvec ? -yi : (m > 0 ? -yim[l * L + am] : yim[l * L + am])) * rl * q;
+        }
+        rl *= r;
+    }
+}
+
 struct Oracle {
     int64_t N;
     double E, max_move, eu;
@@ -474,6 +552,164 @@ struct Oracle {
                 }
             }
         return 0;
+    }
+
+    // ---------------------------------------------------------------- checker
+    // Proposal energies of the stored state (positions, cell store, expansions, M, E,
+    // R, dipole terms: exactly the doubles the reference-ordered propose reads),
+    // evaluated on the real type R.  With R = long double (64-bit mantissa) this is
+    // the extended-precision reference the FP64 and mixed GPU paths are measured
+    // against.  Near field, far field and self term follow direct / indirect /
+    // self_term above.  The long-range term is not the reference's difference of two
+    // whole-system energies (kmc_full_long_range.py:843-858, restated in long_range
+    // above): with B the bilinear form of lr_apply_dot, M' = M + dM, E' = E + dE,
+    //   E_lr(M, E) - E_lr(M', E') = -[ cE . dM + cM . dE + B(dM, dE) ],
+    // cM = B(M, .) and cE = B(., E) as coefficient vectors, dM = P2M(new) - P2M(old)
+    // and dE the same for the dot vector, so only quantities of the size of the moved
+    // charge's own contribution are summed.
+    template <typename R>
+    R lr_form_t(const R* Mv, const R* Evv) const {
+        const int L2 = L * L;
+        R dot = 0;
+        for (int row = 0; row < L2; row++) {
+            R a = 0, b = 0;
+            for (int64_t k = lr_indptr[row]; k < lr_indptr[row + 1]; k++) {
+                a += R(lr_data[k]) * Mv[lr_indices[k]];
+                b += R(lr_data[k]) * Mv[lr_indices[k] + L2];
+            }
+            dot += a * Evv[row] + b * Evv[row + L2];
+        }
+        R d = 0;
+        if (L > 1) {
+            const int re11 = 3, re10 = 2, re1n1 = 1;
+            d += (R(dip[0]) * Mv[re11]) * Evv[re11];
+            d += (R(dip[0]) * Mv[re11]) * Evv[re1n1];
+            d -= (R(dip[1]) * Mv[L2 + re11]) * Evv[L2 + re11];
+            d += (R(dip[1]) * Mv[L2 + re11]) * Evv[L2 + re1n1];
+            d += (R(dip[2]) * Mv[re10]) * Evv[re10];
+        }
+        return R(0.5) * dot + R(0.5) * d;
+    }
+    // coefficient vectors cM[j] = B(M, e_j) and cE[k] = B(e_k, E) of the bilinear form
+    template <typename R>
+    void lr_linear_t(std::vector<R>& cM, std::vector<R>& cE) const {
+        const int L2 = L * L, n2 = 2 * L2;
+        std::vector<R> Mr(M.begin(), M.end()), Er(Ev.begin(), Ev.end()), e(n2, R(0));
+        cM.assign(n2, R(0));
+        cE.assign(n2, R(0));
+        for (int j = 0; j < n2; j++) {
+            e[j] = 1;
+            cM[j] = lr_form_t<R>(Mr.data(), e.data());
+            cE[j] = lr_form_t<R>(e.data(), Er.data());
+            e[j] = 0;
+        }
+    }
+    template <typename R>
+    R direct_t(const double* p, R qq, int64_t lcell, int64_t exclude_id) const {
+        R e = 0;
+        for (int k = 0; k < 27; k++) {
+            const int64_t jc = lcell + OFF[k][0] + (int64_t)lsd * (OFF[k][1] + (int64_t)lsd * OFF[k][2]);
+            for (int64_t j = 0; j < occ[jc]; j++) {
+                if (exclude_id >= 0 && pids[jc * occmax + j] == exclude_id) continue;
+                const double* d = &pdata[(jc * occmax + j) * 5];
+                const R dx = R(p[0]) - R(d[0]), dy = R(p[1]) - R(d[1]), dz = R(p[2]) - R(d[2]);
+                e += R(d[3]) / std::sqrt(dx * dx + dy * dy + dz * dz);
+            }
+        }
+        return e * qq;
+    }
+    template <typename R>
+    R indirect_t(const BasisT<R>& Bt, const double* p, R qq, int64_t lcell) const {
+        const double* c = &centres[3 * lcell];
+        R r, th, ph;
+        spherical_t<R>(R(p[0]) - R(c[0]), R(p[1]) - R(c[1]), R(p[2]) - R(c[2]), &r, &th, &ph);
+        return qq * local_eval_t<R>(Bt, &lexp[lcell * 2 * L * L], r, th, ph);
+    }
+    template <typename R>
+    R self_term_t(const double* op, R oq, const double* np_) const {
+        if (bc == BC_FF) return 0;
+        if (bc == BC_FREE) {
+            const R dx = R(op[0]) - R(np_[0]), dy = R(op[1]) - R(np_[1]), dz = R(op[2]) - R(np_[2]);
+            return oq * oq / std::sqrt(dx * dx + dy * dy + dz * dz);
+        }
+        R e27 = 0;
+        for (int k = 0; k < 27; k++) {
+            const R dx = R(E) * OFF[k][0] + R(op[0]) - R(np_[0]);
+            const R dy = R(E) * OFF[k][1] + R(op[1]) - R(np_[1]);
+            const R dz = R(E) * OFF[k][2] + R(op[2]) - R(np_[2]);
+            e27 += R(1) / std::sqrt(dx * dx + dy * dy + dz * dz);
+            if (OFF[k][0] != 0 || OFF[k][1] != 0 || OFF[k][2] != 0) {
+                const R ex = R(E) * OFF[k][0], ey = R(E) * OFF[k][1], ez = R(E) * OFF[k][2];
+                e27 -= R(1) / std::sqrt(ex * ex + ey * ey + ez * ez);
+            }
+        }
+        return oq * oq * e27;
+    }
+    // dE (eu scaled) and optionally comps (T x 5: U1D, U1I, U0D, U0I, USI) on type R
+    template <typename R>
+    void propose_check(int64_t P, const int64_t* pid, const int64_t* es, const double* npos, const double* nq_in,
+                       double* dE, double* comps) const {
+        const BasisT<R> Bt(L);
+        const bool nearfar = bc != BC_FF;
+        const bool lr = have_lr && (bc == BC_PBC || bc == BC_FF);
+        const int n2 = 2 * L * L;
+        std::vector<R> cM, cE;
+        if (lr) lr_linear_t<R>(cM, cE);
+        std::vector<R> U0D(P, R(0)), U0I(P, R(0));
+        std::vector<std::vector<R>> oM(lr ? P : 0), oE(lr ? P : 0);
+#pragma omp parallel for schedule(dynamic)
+        for (int64_t p = 0; p < P; p++) {
+            const int64_t i = pid[p];
+            if (nearfar) {
+                const int64_t lc = store_cell(cell[i]);
+                U0D[p] = direct_t<R>(&pos[3 * i], R(q[i]), lc, i);
+                U0I[p] = indirect_t<R>(Bt, &pos[3 * i], R(q[i]), lc);
+            }
+            if (lr) {
+                oM[p].assign(n2, R(0));
+                oE[p].assign(n2, R(0));
+                R r, th, ph;
+                spherical_t<R>(R(pos[3 * i]), R(pos[3 * i + 1]), R(pos[3 * i + 2]), &r, &th, &ph);
+                p2m_t<R>(Bt, R(q[i]), r, th, ph, oM[p].data(), false);
+                p2m_t<R>(Bt, R(q[i]), r, th, ph, oE[p].data(), true);
+            }
+        }
+#pragma omp parallel for schedule(dynamic)
+        for (int64_t p = 0; p < P; p++) {
+            const int64_t i = pid[p];
+            std::vector<R> dM(n2), dEv(n2);
+            for (int64_t t = es[p]; t < es[p + 1]; t++) {
+                const R qn = nq_in ? R(nq_in[t]) : R(q[i]);
+                R u1d = 0, u1i = 0;
+                if (nearfar) {
+                    int64_t c3[3];
+                    bin(&npos[3 * t], c3);
+                    const int64_t lc = store_cell(lin(c3));
+                    u1d = direct_t<R>(&npos[3 * t], qn, lc, -1);
+                    u1i = indirect_t<R>(Bt, &npos[3 * t], qn, lc);
+                }
+                R usi = self_term_t<R>(&pos[3 * i], R(q[i]), &npos[3 * t]);
+                if (lr) {
+                    for (int j = 0; j < n2; j++) { dM[j] = -oM[p][j]; dEv[j] = -oE[p][j]; }
+                    R r, th, ph;
+                    spherical_t<R>(R(npos[3 * t]), R(npos[3 * t + 1]), R(npos[3 * t + 2]), &r, &th, &ph);
+                    p2m_t<R>(Bt, qn, r, th, ph, dM.data(), false);
+                    p2m_t<R>(Bt, qn, r, th, ph, dEv.data(), true);
+                    R lin_part = 0;
+                    for (int j = 0; j < n2; j++) lin_part += cE[j] * dM[j] + cM[j] * dEv[j];
+                    usi -= lin_part + lr_form_t<R>(dM.data(), dEv.data());
+                }
+                R de;
+                if (bc == BC_FF) de = -usi;
+                else de = ((u1d + u1i) - (U0D[p] + U0I[p])) - usi;
+                dE[t] = (double)(R(eu) * de);
+                if (comps) {
+                    comps[5 * t + 0] = (double)u1d; comps[5 * t + 1] = (double)u1i;
+                    comps[5 * t + 2] = (double)U0D[p]; comps[5 * t + 3] = (double)U0I[p];
+                    comps[5 * t + 4] = (double)usi;
+                }
+            }
+        }
     }
 
     // ---------------------------------------------------------------- accept
@@ -838,6 +1074,15 @@ void orc_get_cells(void* h, int64_t* cells) {
 int32_t orc_propose(void* h, int64_t P, const int64_t* pid, const int64_t* es, const double* npos,
                     const double* nq, double* dE, double* comps) {
     return ((Oracle*)h)->propose(P, pid, es, npos, nq, dE, comps);
+}
+
+// the checker: the proposal energies of the stored state on long double (extended != 0)
+// or double, long-range term in the expanded difference form (Oracle::propose_check)
+void orc_propose_check(void* h, int64_t P, const int64_t* pid, const int64_t* es, const double* npos,
+                       const double* nq, double* dE, double* comps, int32_t extended) {
+    const Oracle* o = (const Oracle*)h;
+    if (extended) o->propose_check<long double>(P, pid, es, npos, nq, dE, comps);
+    else o->propose_check<double>(P, pid, es, npos, nq, dE, comps);
 }
 
 // kmc_mpi_decomp.py:501-616 (setup_move) + kmc_fmm.py:665-762 (propose_with_dats)

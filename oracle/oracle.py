@@ -59,6 +59,7 @@ def lib():
         L.orc_get_cells.argtypes = [P, P]
         L.orc_propose.restype = I32
         L.orc_propose.argtypes = [P, I64, P, P, P, P, P, P]
+        L.orc_propose_check.argtypes = [P, I64, P, P, P, P, P, P, I32]
         L.orc_propose_with_dats.restype = I32
         L.orc_propose_with_dats.argtypes = [P, I64, P, P, P, P, P, P]
         L.orc_accept.restype = I32
@@ -201,6 +202,22 @@ class Oracle:
             raise RuntimeError("Bad proposed move was detected.")
         return (dE, cm) if comps else dE
 
+    def propose_check(self, pids, es, new_pos, new_q=None, extended=True, comps=False):
+        """dE of proposals of the stored state, evaluated in long double (extended) or
+        double, with the long-range term in the expanded difference form (no subtraction
+        of two whole-system energies): the reference the GPU paths are measured against.
+        No validation of the moves."""
+        pids = _i64(pids)
+        es = _i64(es)
+        new_pos = _f64(new_pos).reshape(-1, 3)
+        T = int(es[-1])
+        dE = np.zeros(T)
+        cm = np.zeros((T, 5)) if comps else None
+        nq = None if new_q is None else _f64(new_q)
+        lib().orc_propose_check(self.h, len(pids), _p(pids), _p(es), _p(new_pos), _p(nq), _p(dE), _p(cm),
+                                1 if extended else 0)
+        return (dE, cm) if comps else dE
+
     def propose(self, moves):
         """Tuple API of the reference: returns energy + dE per proposal."""
         pids, es, pos = [], [0], []
@@ -282,6 +299,37 @@ class Oracle:
 
     def accept_sweep_only(self, q, new_pos, old_pos):
         lib().orc_accept_sweep_only(self.h, float(q), _p(_f64(new_pos)), _p(_f64(old_pos)))
+
+
+# ---------------------------------------------------------------------------
+# the one error measure of proposal energies used by the tests and bench.py
+# ---------------------------------------------------------------------------
+# Relative error with the reference magnitude floored at DE_FLOOR (energy units,
+# = 1e-3 kT at kT = 1): a proposal whose |dE| is below the floor is held to an
+# absolute error of bar x DE_FLOOR (1e-13 for the FP64 bar 1e-10, 1e-11 for the
+# mixed bar 1e-8).  An absolute error d in dE changes its Metropolis rate by a factor
+# exp(-d / kT); below the floor the relative error of dE itself says nothing about
+# the rate, while the absolute bound still does.
+DE_FLOOR = 1e-3
+DE_BAR_FP64 = 1e-10
+DE_BAR_MIXED = 1e-8
+
+
+def de_rel_err(a, ref, floor=DE_FLOOR):
+    a = np.asarray(a, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    return np.abs(a - ref) / np.maximum(np.abs(ref), floor)
+
+
+def de_error(a, ref, floor=DE_FLOOR):
+    """max / median absolute and floored relative error of dE values a against ref."""
+    a = np.asarray(a, dtype=np.float64).ravel()
+    ref = np.asarray(ref, dtype=np.float64).ravel()
+    ab = np.abs(a - ref)
+    rel = de_rel_err(a, ref, floor)
+    return {"proposals": int(a.size), "max_abs": float(ab.max()), "median_abs": float(np.median(ab)),
+            "max_rel": float(rel.max()), "median_rel": float(np.median(rel)),
+            "rel_floor": floor, "below_floor": int(np.sum(np.abs(ref) < floor))}
 
 
 # basis primitives (reference layout)

@@ -29,10 +29,11 @@ def ball_moves(rng, pos, pids, k, radius, E, bc):
     return tuple(out)
 
 
-def rel_err(a, b, floor=1e-3):
-    a = np.asarray(a, dtype=np.float64)
-    b = np.asarray(b, dtype=np.float64)
-    return np.abs(a - b) / np.maximum(np.abs(b), floor)
+def rel_err(a, b, floor=None):
+    """The one dE error measure of the tests and bench.py (oracle.de_rel_err):
+    |a - b| / max(|b|, floor), floor 1e-3 energy units unless given."""
+    from oracle import de_rel_err, DE_FLOOR
+    return de_rel_err(a, b, DE_FLOOR if floor is None else floor)
 
 
 def metropolis_pick(dE, u, kT=1.0):
@@ -42,3 +43,26 @@ def metropolis_pick(dE, u, kT=1.0):
     c = np.cumsum(w)
     k = int(np.searchsorted(c, u * c[-1], side="right"))
     return min(k, len(dE) - 1)
+
+
+def move_stream(rng, N, P, k, radius, nsteps):
+    """Seeded synthetic move stream of the benchmark configurations: per step P distinct
+    moving charges, k displacements each uniform in a ball of the radius, one Metropolis
+    uniform."""
+    T = P * k
+    pids = np.stack([rng.choice(N, size=P, replace=False) for _ in range(nsteps)]).astype(np.int64)
+    d = rng.normal(size=(nsteps, T, 3))
+    d /= np.linalg.norm(d, axis=2, keepdims=True)
+    d *= radius * rng.uniform(size=(nsteps, T, 1)) ** (1.0 / 3.0)
+    u = rng.uniform(size=nsteps)
+    return pids, d, u
+
+
+def step_moves(pos, pids_s, disp_s, k, E, bc):
+    """New positions of one step of a move stream (wrapped into the periodic box, or
+    clipped to a free-space box), as kmc_materialize_moves forms them on the device."""
+    npos = pos[np.repeat(pids_s, k)] + disp_s
+    if bc == "free_space":
+        return np.clip(npos, -0.5 * E, 0.5 * E)
+    npos = np.where(npos < -0.5 * E, npos + E, npos)
+    return np.where(npos >= 0.5 * E, npos - E, npos)
