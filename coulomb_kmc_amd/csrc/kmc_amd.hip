@@ -1173,6 +1173,7 @@ static SetupArgs setup_args(kmc_ctx* c, int64_t P, int64_t T, const int64_t* d_p
     a.owner = wk.owner;
     a.err = wk.err;
     a.es_out = wk.es_copy;
+    a.mask = nullptr;
     return a;
 }
 
@@ -1334,8 +1335,10 @@ static int run_propose_core(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, dou
 }
 
 static int launch_setup(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, const int64_t* d_pid, const int64_t* d_es,
-                        const double* d_new_pos, const double* d_new_q, const ProposeWork* w = nullptr) {
-    const SetupArgs a = setup_args(c, P, T, d_pid, d_es, d_new_pos, d_new_q, w);
+                        const double* d_new_pos, const double* d_new_q, const ProposeWork* w = nullptr,
+                        const int64_t* d_mask = nullptr) {
+    SetupArgs a = setup_args(c, P, T, d_pid, d_es, d_new_pos, d_new_q, w);
+    a.mask = d_mask;
     const int64_t nthr = std::max(T, P);
     if (P > 0) hipLaunchKernelGGL(k_setup, dim3((nthr + 255) / 256), dim3(256), 0, s, a);
     HIPCHK(hipGetLastError());
@@ -1656,8 +1659,9 @@ int kmc_set_tiles(kmc_ctx* c, int32_t near_lanes, int32_t sweep_block, int32_t s
         return set_err(KMC_E_ARG, "near_lanes must be one of 0,1,4,8,16,27,32,64 (27: one lane per neighbour leaf, "
                                   "0: the proposals of a moving charge together)");
     if (sweep_pairs < 1 || sweep_pairs > 2) return set_err(KMC_E_ARG, "sweep_pairs must be 1 or 2");
-    if (sweep_block != 0 && (sweep_block < 64 || sweep_block > 256 || sweep_block % 64))
-        return set_err(KMC_E_ARG, "sweep_block must be 0 (fused), 64, 128, 192 or 256");
+    if (sweep_block != 0 && sweep_block != 1 && (sweep_block < 64 || sweep_block > 256 || sweep_block % 64))
+        return set_err(KMC_E_ARG, "sweep_block must be 0 (fused), 1 (4x4x4 leaf blocks with order truncation), "
+                                  "64, 128, 192 or 256");
     if (sweep_split != 1 && sweep_split != 3) return set_err(KMC_E_ARG, "sweep_split must be 1 or 3");
     c->near_lanes = near_lanes;
     c->sweep_block = sweep_block;
@@ -2737,7 +2741,8 @@ int kmc_lattice_step(kmc_ctx* c, void* stream, int64_t M, const double* h_offset
                        c->ppos4.p, c->lat_off.p, d_new_pos, d_masks);
     hipLaunchKernelGGL(k_overlap_mask, dim3((nt + 255) / 256), dim3(256), 0, s, c->geom(), n, (int)M, tol2, wrap,
                        d_new_pos, c->occ.p, c->slot_pos.p, c->slot_id.p, c->occmax, d_masks);
-    int rc = launch_setup(c, s, P, T, c->lat_pid.p, c->lat_es.p, d_new_pos, nullptr);
+    // masked proposals (outside the domain without wrapping, or overlapping) are not moves
+    int rc = launch_setup(c, s, P, T, c->lat_pid.p, c->lat_es.p, d_new_pos, nullptr, nullptr, d_masks);
     if (rc) return rc;
     rc = run_propose_core(c, s, P, T, d_dE, nullptr, precision == KMC_PREC_MIXED, kT, c->rates.p, d_masks, rate_rule);
     if (rc) return rc;

@@ -250,7 +250,7 @@ static constexpr RecTab2 kRec2 = make_rectab2();
 // parallelism); each accumulator still receives the images in order j = 0, 1, ...
 template <int L, int M, int NI>
 __device__ __forceinline__ void sweep_column(const double* zr, const double* ir2, const Cplx* W, double* ar,
-                                             double* ai) {
+                                             double* ai, int lc = L) {
     double e2[NI], e1[NI];
 #pragma unroll
     for (int j = 0; j < NI; j++) {
@@ -259,19 +259,24 @@ __device__ __forceinline__ void sweep_column(const double* zr, const double* ir2
         ar[0] += W[j].re;
         ai[0] += W[j].im;
     }
+    // orders >= lc are negligible (lc wave-uniform).  A guard, not an early exit: the
+    // loop keeps its constant trip count, so it is unrolled and the accumulators stay
+    // in registers
 #pragma unroll
     for (int k = 1; k < L - M; k++) {
         const int l = M + k;
+        if (l < lc) {
 #pragma unroll
-        for (int j = 0; j < NI; j++) {
-            const double e = fma(kRec2.a2[l][M] * zr[j], e1[j], -ir2[j] * e2[j]);
-            e2[j] = e1[j];
-            e1[j] = e;
-        }
+            for (int j = 0; j < NI; j++) {
+                const double e = fma(kRec2.a2[l][M] * zr[j], e1[j], -ir2[j] * e2[j]);
+                e2[j] = e1[j];
+                e1[j] = e;
+            }
 #pragma unroll
-        for (int j = 0; j < NI; j++) {
-            ar[k] = fma(e1[j], W[j].re, ar[k]);
-            ai[k] = fma(e1[j], W[j].im, ai[k]);
+            for (int j = 0; j < NI; j++) {
+                ar[k] = fma(e1[j], W[j].re, ar[k]);
+                ai[k] = fma(e1[j], W[j].im, ai[k]);
+            }
         }
     }
 }
@@ -292,67 +297,75 @@ __device__ __forceinline__ void sweep_store_column(double2* le, const double* ar
 // column M with weight base * s_M * U (U = conj((x+iy)/r^2)^M) into the accumulator row
 template <int L, int M, int NI>
 __device__ __forceinline__ void sweep_weighted_column(const double* zr, const double* ir2, const Cplx* U,
-                                                      const double* base, double* ar, double* ai) {
+                                                      const double* base, double* ar, double* ai, int lc = L) {
     Cplx W[NI];
 #pragma unroll
     for (int j = 0; j < NI; j++) {
         const double f = base[j] * kRec.s[M];
         W[j] = Cplx{f * U[j].re, f * U[j].im};
     }
-    sweep_column<L, M, NI>(zr, ir2, W, ar, ai);
+    if (M < lc) sweep_column<L, M, NI>(zr, ir2, W, ar, ai, lc);
 }
 // low columns P0 + J, J = 0 .. NPE-1 (ascending powers: one multiply per column)
 template <int L, int P0, int NPE, int NI, int J = 0>
 __device__ __forceinline__ void sweep_low_cols(const double* zr, const double* ir2, const Cplx* u, Cplx* U,
-                                               const double* base, double (*ar)[L + 1], double (*ai)[L + 1]) {
+                                               const double* base, double (*ar)[L + 1], double (*ai)[L + 1],
+                                               int lc = L) {
     if constexpr (J < NPE) {
         if constexpr (J > 0) {
 #pragma unroll
             for (int j = 0; j < NI; j++) U[j] = cmul(U[j], u[j]);
         }
-        sweep_weighted_column<L, P0 + J, NI>(zr, ir2, U, base, ar[J], ai[J]);
-        sweep_low_cols<L, P0, NPE, NI, J + 1>(zr, ir2, u, U, base, ar, ai);
+        sweep_weighted_column<L, P0 + J, NI>(zr, ir2, U, base, ar[J], ai[J], lc);
+        sweep_low_cols<L, P0, NPE, NI, J + 1>(zr, ir2, u, U, base, ar, ai, lc);
     }
 }
 // high columns L-1-P0-J for J = NPE-1 down to 0 (ascending powers), skipping the
 // middle column that equals its low partner when L is odd
 template <int L, int P0, int NPE, int NI, int J>
 __device__ __forceinline__ void sweep_high_cols(const double* zr, const double* ir2, const Cplx* u, Cplx* U,
-                                                const double* base, double (*ar)[L + 1], double (*ai)[L + 1]) {
+                                                const double* base, double (*ar)[L + 1], double (*ai)[L + 1],
+                                                int lc = L) {
     if constexpr (J >= 0) {
         constexpr int M1 = P0 + J, M2 = L - 1 - P0 - J;
         if constexpr (M2 != M1)
-            sweep_weighted_column<L, M2, NI>(zr, ir2, U, base, ar[J] + (L - M1), ai[J] + (L - M1));
+            sweep_weighted_column<L, M2, NI>(zr, ir2, U, base, ar[J] + (L - M1), ai[J] + (L - M1), lc);
         if constexpr (J > 0) {
 #pragma unroll
             for (int j = 0; j < NI; j++) U[j] = cmul(U[j], u[j]);
         }
-        sweep_high_cols<L, P0, NPE, NI, J - 1>(zr, ir2, u, U, base, ar, ai);
+        sweep_high_cols<L, P0, NPE, NI, J - 1>(zr, ir2, u, U, base, ar, ai, lc);
     }
 }
 // the column pairs (P, L-1-P), P = P0 .. P0+NP-1, of one lane: the powers of u run up
 // through the low columns and on (one cpow) through the high ones
+// lc: orders l >= lc are skipped (wave-uniform; L: none).  Columns ascend within the
+// low and within the high part, so a column at or above lc ends its part.
 template <int L, int P0, int NP, int NI>
 __device__ __forceinline__ void sweep_group_cols(const double* zr, const double* ir2, const Cplx* u,
-                                                 const double* base, double (*ar)[L + 1], double (*ai)[L + 1]) {
+                                                 const double* base, double (*ar)[L + 1], double (*ai)[L + 1],
+                                                 int lc = L) {
     constexpr int NPAIR = (L + 1) / 2;
     constexpr int NPE = (P0 + NP <= NPAIR) ? NP : NPAIR - P0;   // pairs in this group
     if constexpr (NPE > 0) {
+        if (P0 >= lc) return;
         Cplx U[NI];
 #pragma unroll
         for (int j = 0; j < NI; j++) U[j] = cpow<P0>(u[j]);
-        sweep_low_cols<L, P0, NPE, NI>(zr, ir2, u, U, base, ar, ai);
+        sweep_low_cols<L, P0, NPE, NI>(zr, ir2, u, U, base, ar, ai, lc);
         constexpr int LOW_TOP = P0 + NPE - 1;          // power held in U now
         constexpr int HIGH_BOTTOM = L - 1 - LOW_TOP;    // lowest high column
         if constexpr (HIGH_BOTTOM > LOW_TOP) {
+            if (HIGH_BOTTOM >= lc) return;
 #pragma unroll
             for (int j = 0; j < NI; j++) U[j] = cmul(U[j], cpow<HIGH_BOTTOM - LOW_TOP>(u[j]));
-            sweep_high_cols<L, P0, NPE, NI, NPE - 1>(zr, ir2, u, U, base, ar, ai);
+            sweep_high_cols<L, P0, NPE, NI, NPE - 1>(zr, ir2, u, U, base, ar, ai, lc);
         } else if constexpr (HIGH_BOTTOM == LOW_TOP && NPE > 1) {
             // L odd: the last low column is the middle one; the high columns start one above
+            if (HIGH_BOTTOM + 1 >= lc) return;
 #pragma unroll
             for (int j = 0; j < NI; j++) U[j] = cmul(U[j], u[j]);
-            sweep_high_cols<L, P0, NPE, NI, NPE - 2>(zr, ir2, u, U, base, ar, ai);
+            sweep_high_cols<L, P0, NPE, NI, NPE - 2>(zr, ir2, u, U, base, ar, ai, lc);
         }
     }
 }
@@ -386,7 +399,8 @@ __device__ __forceinline__ void sweep_images(const double* dx, double dy, double
 }
 template <int L, int P0, int NP, int NI>
 __device__ __forceinline__ void sweep_images_xyz(const double* dx, const double* dy, const double* dz,
-                                                 const double* qe, double (*ar)[L + 1], double (*ai)[L + 1]) {
+                                                 const double* qe, double (*ar)[L + 1], double (*ai)[L + 1],
+                                                 int lc = L) {
     double zr[NI], ir2[NI], base[NI];
     Cplx u[NI];
 #pragma unroll
@@ -399,7 +413,7 @@ __device__ __forceinline__ void sweep_images_xyz(const double* dx, const double*
         u[j] = Cplx{dx[j] * ir2[j], -dy[j] * ir2[j]};
         base[j] = qe[j] * ir;
     }
-    sweep_group_cols<L, P0, NP, NI>(zr, ir2, u, base, ar, ai);
+    sweep_group_cols<L, P0, NP, NI>(zr, ir2, u, base, ar, ai, lc);
 }
 
 __device__ __forceinline__ bool near1(int d) { return d < 2 && d > -2; }
@@ -661,6 +675,206 @@ __device__ __forceinline__ void sweep_fused_body(const SweepArgs& a) {
         }
     }
 }
+// ---------------------------------------------------------------------------
+// Accept sweep over 4x4x4 blocks of leaves, orders truncated where they cannot matter.
+//
+// The hop's P2L into a leaf (centre c) from a source image of charge q at s changes
+// the potential the leaf's expansion gives anywhere in the leaf (|x - c| <= rho, the
+// leaf's half diagonal) by the sum over orders l of q |x-c|^l / |s-c|^{l+1} P_l(cos)
+// (addition theorem; |P_l| <= 1).  The orders l >= lc together change it by at most
+//     |q| (rho/D)^lc / (D - rho),   D = min |s - c| over the block's leaf centres,
+// so they are skipped when that bound is below SWEEP_TAU |q| (2^-60 per unit charge,
+// ~1e-18 for potentials of order one): the leaf's expansion then differs from the
+// full P2L by less than the rounding of its own lowest coefficients, and dE of any
+// later proposal by less than 54 SWEEP_TAU |q| per accept.  lc is the same for every
+// leaf of the block, so the truncation is uniform over each wave: a workgroup owns a
+// 4x4x4 block of leaves (lane = leaf, wave = column-pair group, as in the fused
+// layout), whose 64 expansions are 16 runs of 4 consecutive leaves for the staged
+// update.  Images far from the whole block (most of the 27 of each source in a
+// periodic box) keep only their first ~9-12 orders of 16.
+// ---------------------------------------------------------------------------
+constexpr double SWEEP_TAU = 8.673617379884035e-19;   // 2^-60
+
+// smallest lc with |q| (rho/D)^lc / (D - rho) <= SWEEP_TAU |q|, capped at L
+__device__ __forceinline__ int sweep_trunc_order(double D, double rho, int L) {
+    if (!(D > 2.0 * rho)) return L;
+    const double t = rho / D;
+    double bound = 1.0 / (D - rho);
+    int lc = 0;
+    while (lc < L && bound > SWEEP_TAU) {
+        bound *= t;
+        lc++;
+    }
+    return lc;
+}
+
+// group G (column pairs G*NP ..) of the move's 2 x nimg source images for the leaf
+// (tx, ty, tz) of this lane; lcs[k] = truncation order of image k (both sources)
+template <int L, int G, int NP, int NI, bool PER>
+__device__ __forceinline__ void sweep_block_group(const SweepArgs& a, int tx, int ty, int tz, const int* lcs,
+                                                  double (*ar)[L + 1], double (*ai)[L + 1]) {
+    constexpr int P0 = G * NP;
+    const int n = a.g.ncps;
+    const double E = a.g.E;
+    const double cx = a.g.c0 + tx * a.g.w;
+    const double cy = a.g.c0 + ty * a.g.w;
+    const double cz = a.g.c0 + tz * a.g.w;
+    const MoveRec* rc = a.rec;
+    double dx0[2], dy0[2], dz0[2], sq[2];
+    int dcx[2], dcy[2], dcz[2];
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        const double* p = s == 0 ? rc->old_pos : rc->new_pos;
+        const int sc = s == 0 ? rc->old_cell : rc->new_cell;
+        sq[s] = s == 0 ? -1.0 * rc->q : rc->q;
+        dx0[s] = p[0] - cx; dy0[s] = p[1] - cy; dz0[s] = p[2] - cz;
+        dcx[s] = sc % n - tx; dcy[s] = (sc / n) % n - ty; dcz[s] = sc / (n * n) - tz;
+    }
+    constexpr int R = PER ? 1 : 0;
+#pragma unroll 1
+    for (int oz = -R; oz <= R; oz++) {
+#pragma unroll 1
+        for (int oy = -R; oy <= R; oy++) {
+#pragma unroll 1
+            for (int ox = -R; ox <= R; ox++) {
+                const int lc = __builtin_amdgcn_readfirstlane(lcs[(ox + R) + (2 * R + 1) * ((oy + R) + (2 * R + 1) * (oz + R))]);
+                if (lc <= P0) continue;    // every column of this group at or above lc
+                double dx[2], dy[2], dz[2], qe[2];
+#pragma unroll
+                for (int s = 0; s < 2; s++) {
+                    dx[s] = fma((double)ox, E, dx0[s]);
+                    dy[s] = fma((double)oy, E, dy0[s]);
+                    dz[s] = fma((double)oz, E, dz0[s]);
+                    const bool nr = near1(dcx[s] + ox * n) && near1(dcy[s] + oy * n) && near1(dcz[s] + oz * n);
+                    qe[s] = nr ? 0.0 : sq[s];
+                }
+                if constexpr (NI == 2) {
+                    sweep_images_xyz<L, P0, NP, 2>(dx, dy, dz, qe, ar, ai, lc);
+                } else {
+#pragma unroll
+                    for (int s = 0; s < 2; s++)
+                        sweep_images_xyz<L, P0, NP, 1>(dx + s, dy + s, dz + s, qe + s, ar, ai, lc);
+                }
+            }
+        }
+    }
+}
+// one wave's share of the blocked sweep: group G's columns of its lane's leaf, then the
+// staged, coalesced update of the block's expansions together with the other waves.
+// Not inlined: each group's accumulators stay in registers of its own function (inlined
+// into one kernel, identical code of the groups was merged and the accumulators were
+// demoted to scratch memory).
+template <int L, int G, int NP, int NI, bool PER>
+__device__ __forceinline__ void sweep_block_wave(const SweepArgs& a, int tx, int ty, int tz,
+                                                           const int* lcs, double2* stage, int cell_base) {
+    constexpr int W = L + 1;
+    constexpr int NH = nhalf(L);
+    constexpr int ST = stage_stride(NH);
+    const int lane = threadIdx.x;
+    const int n = a.g.ncps;
+    double ar[NP][W], ai[NP][W];
+#pragma unroll
+    for (int j = 0; j < NP; j++)
+#pragma unroll
+        for (int k = 0; k < W; k++) { ar[j][k] = 0.0; ai[j][k] = 0.0; }
+    sweep_block_group<L, G, NP, NI, PER>(a, tx, ty, tz, lcs, ar, ai);
+    sweep_group_stage<L, G * NP, NP, true>(nullptr, ar, ai);
+    const int tid = threadIdx.x + 64 * threadIdx.y;
+    const int nthr = 64 * blockDim.y;
+#pragma unroll 1
+    for (int half = 0; half < 2; half++) {
+        __syncthreads();
+        if ((lane >> 5) == half) sweep_group_stage<L, G * NP, NP, false>(stage + (lane & 31) * ST, ar, ai);
+        __syncthreads();
+        // the half's 32 leaves: 8 runs (y, z) of 4 consecutive leaves along x
+        constexpr int U = 8;
+#pragma unroll 1
+        for (int e0 = tid; e0 < 32 * NH; e0 += nthr * U) {
+            double2 v[U];
+            size_t gi[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int e = e0 + u * nthr;
+                const int j = e / NH, h = e - (e / NH) * NH;             // staged leaf, coefficient
+                const int jy = (j >> 2) & 3, jz = 2 * half + (j >> 4);
+                gi[u] = (size_t)(cell_base + (j & 3) + n * (jy + n * jz)) * NH + h;
+                if (e < 32 * NH) v[u] = a.lexp[gi[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int e = e0 + u * nthr;
+                if (e < 32 * NH) {
+                    const int j = e / NH, h = e - j * NH;
+                    const double2 sv = stage[j * ST + h];
+                    v[u].x += sv.x;
+                    v[u].y += sv.y;
+                    a.lexp[gi[u]] = v[u];
+                }
+            }
+        }
+    }
+}
+template <int L, int NP, int NI, bool PER, int G0>
+__device__ __forceinline__ void sweep_block_dispatch(const SweepArgs& a, int group, int tx, int ty, int tz,
+                                                     const int* lcs, double2* stage, int cell_base) {
+    constexpr int NG = ((L + 1) / 2 + NP - 1) / NP;
+    if constexpr (G0 < NG) {
+        if (group == G0) sweep_block_wave<L, G0, NP, NI, PER>(a, tx, ty, tz, lcs, stage, cell_base);
+        else sweep_block_dispatch<L, NP, NI, PER, G0 + 1>(a, group, tx, ty, tz, lcs, stage, cell_base);
+    }
+}
+
+// grid: one workgroup per 4x4x4 block of leaves (ncps a multiple of 4), 64 x NG threads
+template <int L, int NP, int NI, bool PER>
+__device__ __forceinline__ void sweep_block_body(const SweepArgs& a) {
+    constexpr int NH = nhalf(L);
+    constexpr int ST = stage_stride(NH);
+    constexpr int NIMG = PER ? 27 : 1;
+    __shared__ double2 stage[32 * ST];
+    __shared__ int s_lc[NIMG];
+    const int lane = threadIdx.x;
+    const int group = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    const int n = a.g.ncps, nb = n >> 2;
+    const int b = blockIdx.x;
+    const int bx = b % nb, by = (b / nb) % nb, bz = b / (nb * nb);
+    const int tx = 4 * bx + (lane & 3), ty = 4 * by + ((lane >> 2) & 3), tz = 4 * bz + (lane >> 4);
+    if (!a.rec->valid) return;   // (uniform over the grid: nothing to add)
+    if (group == 0 && lane < NIMG) {
+        // truncation order of image `lane` of both sources against this block
+        const int R = PER ? 1 : 0;
+        const int ox = lane % (2 * R + 1) - R, oy = (lane / (2 * R + 1)) % (2 * R + 1) - R,
+                  oz = lane / ((2 * R + 1) * (2 * R + 1)) - R;
+        const double w = a.g.w;
+        const double rho = 0.8660254037844386 * w;   // half diagonal of a leaf
+        const double hb = 1.5 * w;                    // the 4 leaf centres along an axis span 3w
+        const double mx = a.g.c0 + (4 * bx + 1.5) * w, my = a.g.c0 + (4 * by + 1.5) * w,
+                     mz = a.g.c0 + (4 * bz + 1.5) * w;
+        int lc = 0;
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            const double* p = s == 0 ? a.rec->old_pos : a.rec->new_pos;
+            const double ex = fmax(fabs(p[0] + ox * a.g.E - mx) - hb, 0.0);
+            const double ey = fmax(fabs(p[1] + oy * a.g.E - my) - hb, 0.0);
+            const double ez = fmax(fabs(p[2] + oz * a.g.E - mz) - hb, 0.0);
+            lc = max(lc, sweep_trunc_order(sqrt(fma(ex, ex, fma(ey, ey, ez * ez))), rho, L));
+        }
+        s_lc[lane] = lc;
+    }
+    __syncthreads();
+    const int cell_base = 4 * bx + n * (4 * by + n * 4 * bz);
+    sweep_block_dispatch<L, NP, NI, PER, 0>(a, group, tx, ty, tz, s_lc, stage, cell_base);
+}
+template <int L, int NP, int NI, bool PER>
+__global__ void __launch_bounds__(512, 2) k_sweep_block(SweepArgs a) {
+    sweep_block_body<L, NP, NI, PER>(a);
+}
+template <int L, int NP, int NI, bool PER>
+__global__ void __launch_bounds__(512, 2) kb_sweep_block(const SweepArgs* __restrict__ args) {
+    const SweepArgs& a = args[blockIdx.z];
+    if ((int)blockIdx.x >= (a.g.ncps >> 2) * (a.g.ncps >> 2) * (a.g.ncps >> 2)) return;
+    sweep_block_body<L, NP, NI, PER>(a);
+}
+
 template <int L, int NP, bool PER>
 __global__ void __launch_bounds__(512, ((NP == 1 && L <= 16) ? 4 : 2)) k_sweep_fused(SweepArgs a) {
     sweep_fused_body<L, NP, PER>(a);
@@ -913,6 +1127,7 @@ struct SetupArgs {
     int32_t* owner;
     int* err;
     int64_t* es_out;     // copy of es[0..P] (read by the grouped near-field kernel)
+    const int64_t* mask; // proposals with mask <= 0 are not validated (they get rate 0), or null
 };
 
 // owner p of proposal t: es[p] <= t < es[p+1]; always returns a value in [0, P)
@@ -958,7 +1173,8 @@ __device__ __forceinline__ void setup_body(const SetupArgs& a) {
         a.eexcl[x] = -1;
         const double np3[3] = {px, py, pz};
         const double op3[3] = {o.x, o.y, o.z};
-        for (int d = 0; d < 3; d++) {
+        const int nd = (a.mask && a.mask[x] <= 0) ? 0 : 3;
+        for (int d = 0; d < nd; d++) {
             if (!(np3[d] >= -0.5 * a.g.E && np3[d] <= 0.5 * a.g.E)) bad++;
             double dd = np3[d] - op3[d];
             if (a.g.bc != KMC_BC_FREE_SPACE) {
@@ -1876,8 +2092,29 @@ static void launch_sweep_np(SweepArgs a, int ncells, hipStream_t s, int block, i
         hipLaunchKernelGGL((k_sweep<L, NP, true, 1>), dim3((ncells + block - 1) / block, ng), dim3(block), 0, s, a);
     }
 }
+// the blocked sweep: two column pairs per lane, one source image per recurrence pass
+// (both sources' images together need more than 256 VGPRs)
+template <int L>
+static void launch_sweep_block(SweepArgs a, hipStream_t s) {
+    constexpr int ng = ((L + 1) / 2 + 1) / 2;
+    const int nb = a.g.ncps / 4;
+    if (a.g.bc == KMC_BC_FREE_SPACE)
+        hipLaunchKernelGGL((k_sweep_block<L, 2, 1, false>), dim3(nb * nb * nb), dim3(64, ng), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_sweep_block<L, 2, 1, true>), dim3(nb * nb * nb), dim3(64, ng), 0, s, a);
+}
+static bool sweep_block_fits(const SweepArgs& a, int L) {
+    return a.mode == 0 && a.g.ncps % 4 == 0 && 64 * (((L + 1) / 2 + 1) / 2) <= 512;
+}
 template <int L>
 static void launch_sweep(SweepArgs a, int ncells, hipStream_t s, int block, int np, int split) {
+    // block 1: the blocked sweep with order truncation (the move record only; leaf grids
+    // in multiples of 4 per side); other shapes otherwise
+    if (block == 1 && sweep_block_fits(a, L)) {
+        launch_sweep_block<L>(a, s);
+        return;
+    }
+    if (block == 1) block = 0;
     if (np == 1) launch_sweep_np<L, 1>(a, ncells, s, block, split);
     else launch_sweep_np<L, 2>(a, ncells, s, block, split);
 }

@@ -7,7 +7,9 @@ Tunable shapes (set on a context with KMCFMM.set_tiles / kmc_set_tiles):
   sweep_pairs  column pairs (m, L-1-m) per lane of the accept sweep (1, 2)
   sweep_block  leaves (threads) per workgroup of the accept sweep (64, 128, 192, 256), or
                0: fused layout, 64 leaves x all column groups per workgroup, with the
-               expansion update staged through LDS and coalesced
+               expansion update staged through LDS and coalesced, or 1: blocked layout,
+               a 4x4x4 block of leaves per workgroup with the orders of each source image
+               truncated where they cannot change the block's potential by 2^-60
   sweep_split  waves sharing the 27 images of one leaf (1; or 3, one image layer per
                wave, workgroups of 64 leaves x 3; '27'/PBC only)
 
@@ -26,7 +28,8 @@ TABLE = os.path.join(_HERE, "tiles.json")
 
 NEAR_LANES = (0, 1, 4, 8, 16, 27, 32, 64)
 # (sweep_pairs, sweep_block, sweep_split)
-SWEEP_SHAPES = ((1, 0, 1), (2, 0, 1), (1, 64, 1), (1, 256, 1), (2, 64, 1), (2, 256, 1), (1, 64, 3), (2, 64, 3))
+SWEEP_SHAPES = ((2, 1, 1), (1, 0, 1), (2, 0, 1), (1, 64, 1), (1, 256, 1), (2, 64, 1), (2, 256, 1), (1, 64, 3),
+                (2, 64, 3))
 DEFAULT = {"near_lanes": 27, "sweep_block": 0, "sweep_pairs": 2, "sweep_split": 1}
 KEYS = ("near_lanes", "sweep_block", "sweep_pairs", "sweep_split")
 

@@ -313,8 +313,12 @@ int kmc_check_flags(kmc_ctx* ctx, void* stream, int32_t* h_flags);
  * near_lanes = lanes per evaluation point in the near-field kernel (1,4,8,16,32,64, or
  * 27 = one lane per neighbour leaf); accept sweep: sweep_pairs = column pairs
  * (m, L-1-m) per lane (1, 2), sweep_block = leaves (threads) per workgroup (64, 128,
- * 192, 256), sweep_split = waves sharing one leaf's 27 images (1, or 3: one image
- * layer each, summed in LDS; workgroups of 64 leaves x 3; '27'/PBC only).
+ * 192, 256), or 0: fused (64 consecutive leaves x all column groups per workgroup,
+ * update staged through LDS), or 1: blocked (a 4x4x4 block of leaves per workgroup,
+ * orders of each source image truncated where their sum changes the potential in the
+ * block by less than 2^-60 per unit charge; accepts only, leaf grids with a multiple of
+ * 4 leaves per side, else as 0), sweep_split = waves sharing one leaf's 27 images (1,
+ * or 3: one image layer each, summed in LDS; workgroups of 64 leaves x 3; '27'/PBC only).
  */
 int kmc_set_tiles(kmc_ctx* ctx, int32_t near_lanes, int32_t sweep_block, int32_t sweep_pairs,
                   int32_t sweep_split);

@@ -64,6 +64,9 @@ class DeviceStream:
         self.d_es = torch.zeros(P + 1, dtype=torch.int64, device="cuda")
         self.d_np = torch.zeros(self.T * 3, dtype=torch.float64, device="cuda")
         self.d_dE = torch.zeros(self.T, dtype=torch.float64, device="cuda")
+        # the device step allocates nothing: a first propose of this size sizes the buffers
+        self.materialize()
+        kmc.propose_device(self.d_pid, self.d_es, self.d_np, self.d_dE, precision="fp64")
 
     def materialize(self):
         self.kmc.materialize_moves(self.P, self.k, self.t_pids, self.t_disp, self.counter, self.d_pid, self.d_es,

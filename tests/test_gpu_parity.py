@@ -396,3 +396,41 @@ def test_use_before_initialise_raises():
     with pytest.raises(RuntimeError):
         kmc.propose(((0, pos[0:1]),))
     kmc.free()
+
+
+@pytest.mark.parametrize("bc", ("pbc", "free_space"))
+def test_blocked_sweep_truncation_matches_oracle(bc):
+    """The blocked accept sweep (sweep_block 1: a 4x4x4 block of leaves per workgroup,
+    orders of each source image dropped where they change the block's potential by less
+    than 2^-60 per unit charge) in a box wide enough that most images lose orders:
+    expansions after accepts against the oracle's full P2L sweep (1e-11 of their
+    scale), then dE of new proposals against the extended reference (1e-10)."""
+    from coulomb_kmc_amd import KMCFMM
+    from coulomb_kmc_amd.kmc_fmm import lr_operator_reference_layout
+    N, E, r, L = 3000, 60.0, 5, 16
+    rng = np.random.default_rng(77)
+    pos, q = neutral_system(rng, N, E)
+    kmc = KMCFMM(pos.copy(), q.reshape(-1, 1), E, r=r, l=L, boundary_condition=bc, max_move=1.5,
+                 init_method="direct")
+    kmc.set_tiles(near_lanes=4, sweep_block=1, sweep_pairs=2, sweep_split=1)
+    kmc.initialise()
+    lr = lr_operator_reference_layout(L, E) if bc == "pbc" else None
+    orc = O.Oracle(pos, q, E, r, L, bc=bc, max_move=1.5, lr=lr)
+    orc.init_from_expansions(kmc.get_expansions(), kmc.energy)
+    for step in range(6):
+        pid = int(rng.integers(N))
+        newp = ball_moves(rng, kmc._pos(), [pid], 1, 1.5, E, bc)[0][1][0]
+        kmc.accept((pid, newp))
+        orc.accept(pid, newp)
+    eg, ec = kmc.get_expansions(), orc.get_expansions()
+    err = np.max(np.abs(eg - ec)) / np.max(np.abs(ec))
+    _print("blocked sweep", bc, "expansion err after 6 accepts", err)
+    assert err <= 1e-11
+    pids = rng.choice(N, 64, replace=False)
+    moves = ball_moves(rng, kmc._pos(), pids, 4, 1.5, E, bc)
+    g = np.concatenate(kmc.propose(moves)) - kmc.energy
+    es = np.arange(65, dtype=np.int64) * 4
+    ext = orc.propose_check(pids, es, np.concatenate([m[1] for m in moves]), extended=True)
+    _print("blocked sweep", bc, "dE vs extended", rel_err(g, ext).max())
+    assert rel_err(g, ext).max() <= O.DE_BAR_FP64
+    kmc.free()

@@ -1,0 +1,77 @@
+"""
+Pins the order truncation of the blocked accept sweep on the CPU (no GPU):
+
+The blocked sweep skips orders l >= lc of a source image whose nearest leaf centre in
+the 4x4x4 block is at distance D, with lc the smallest order such that
+|q| (rho/D)^lc / (D - rho) <= 2^-60 |q| (rho: a leaf's half diagonal).  By the addition
+theorem the skipped orders change the potential the leaf's local expansion gives at
+any point of the leaf by at most that bound.  Checked here with the oracle's own P2L
+and local evaluation (the reference's basis, kmc_expansion_tools.py:154-226): the
+potential of the truncated expansion differs from the full one by no more than the
+bound, and the rule is the one the kernel applies (same lc for a range of distances).
+"""
+
+import numpy as np
+
+import oracle as O
+
+TAU = 2.0 ** -60
+
+
+def trunc_order(D, rho, L):
+    """The kernel's rule (sweep_trunc_order in csrc/kmc_device.h)."""
+    if not D > 2.0 * rho:
+        return L
+    t = rho / D
+    bound = 1.0 / (D - rho)
+    lc = 0
+    while lc < L and bound > TAU:
+        bound *= t
+        lc += 1
+    return lc
+
+
+def truncate(lexp, L, lc):
+    """Zero the orders l >= lc of a reference-layout expansion [2 L^2]."""
+    out = np.array(lexp, copy=True)
+    for l in range(lc, L):
+        for m in range(-l, l + 1):
+            out[l * l + l + m] = 0.0
+            out[L * L + l * l + l + m] = 0.0
+    return out
+
+
+def test_truncation_bound_holds():
+    rng = np.random.default_rng(17)
+    L, w = 16, 1.0
+    rho = 0.5 * np.sqrt(3.0) * w
+    seen = set()
+    for D in (2.5, 4.0, 8.0, 20.0, 60.0, 150.0, 400.0):
+        lc = trunc_order(D, rho, L)
+        seen.add(lc)
+        for _ in range(6):
+            d = rng.normal(size=3)
+            src = D * d / np.linalg.norm(d)               # source at distance D from the centre
+            q = rng.choice([-1.0, 1.0])
+            full = O.p2l(L, q, src)
+            trunc = truncate(full, L, lc)
+            bound = abs(q) * (rho / D) ** lc / (D - rho)
+            for _ in range(8):
+                x = rng.uniform(-0.5 * w, 0.5 * w, 3)    # a point in the leaf
+                dphi = O.local_eval(L, full, x) - O.local_eval(L, trunc, x)
+                # FP64 evaluation of O(1/D) terms adds ~1e-16/D of rounding on top
+                assert abs(dphi) <= bound + 4e-16 / D, (D, lc, dphi, bound)
+    # the rule truncates far images and keeps every order of near ones
+    assert trunc_order(2.5, rho, L) == L
+    assert min(seen) < 10
+
+
+def test_truncation_rule_values():
+    rho = 0.5 * np.sqrt(3.0) * 271.4 / 64
+    # bound at lc is below 2^-60, at lc - 1 above (minimality), for the 2M-charge leaf size
+    for D in (15.0, 40.0, 100.0, 200.0, 400.0):
+        lc = trunc_order(D, rho, 16)
+        if lc < 16:
+            assert (rho / D) ** lc / (D - rho) <= TAU
+        if lc > 0:
+            assert (rho / D) ** (lc - 1) / (D - rho) > TAU
