@@ -47,7 +47,7 @@ EXPORTS = (
     "kmc_batch_create", "kmc_batch_destroy", "kmc_batch_set_system", "kmc_batch_prepare", "kmc_batch_step", "kmc_batch_set_streams", "kmc_run_steps",
     "kmc_particle_count", "kmc_set_particle_count", "kmc_charge_energies", "kmc_group_energy",
     "kmc_propose_extract", "kmc_propose_inject", "kmc_extract", "kmc_inject", "kmc_discover_sites",
-    "kmc_lattice_step", "kmc_accept_movedata",
+    "kmc_lattice_step", "kmc_accept_movedata", "kmc_propose_noalloc",
 )
 
 
@@ -132,6 +132,7 @@ def lib():
         "kmc_set_energy": (ctypes.c_int, [P, D]),
         "kmc_ncells": (ctypes.c_int, [P, P]),
         "kmc_propose": (ctypes.c_int, [P, P, I64, I64, P, P, P, P, P, I32]),
+        "kmc_propose_noalloc": (ctypes.c_int, [P, P, I64, I64, P, P, P, P, P, I32]),
         "kmc_propose_with_dats": (ctypes.c_int, [P, P, I64, P, P, P, P, P, P, I32, P]),
         "kmc_accept": (ctypes.c_int, [P, P, I64, P, P]),
         "kmc_step": (ctypes.c_int, [P, P, I64, I64, P, P, P, P, I64, P, D, P, P, I32]),

@@ -24,11 +24,17 @@
 // recurrence documented in coulomb_kmc_amd/harmonics.py (no trigonometry); the
 // coefficient tables are compile-time constants.
 
+#include <mutex>
+
 #include "kmc_device.h"
 
 // ============================================================================
 // kernels independent of the expansion order
 // ============================================================================
+__global__ void k_fill_i32(int* p, int n, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
 
 template <int G, bool MIXED>
 __global__ void __launch_bounds__(256) k_near(EvalArgs a) { near_body<G, MIXED>(a); }
@@ -1095,8 +1101,8 @@ static int ensure_eval(kmc_ctx* c, int64_t T, int64_t P) {
 }
 
 // The evaluation arrays of one propose (T new points, then P old): the context's own
-// buffers (ctx_work, sized by ensure_eval / ensure_lr_work) or device memory handed in
-// by the caller (nothing allocated).
+// buffers (ctx_work, sized by ensure_eval / ensure_lr_work) or a carve of the library's
+// fixed scratch array (kmc_propose_noalloc).
 struct ProposeWork {
     double4* epos4;
     int32_t *ecell, *eexcl, *owner;
@@ -1556,6 +1562,62 @@ static int propose_inject_core(kmc_ctx* c, hipStream_t s, int n, const double* d
 // ============================================================================
 // C ABI
 // ============================================================================
+// ---------------------------------------------------------------------------
+// Fixed scratch of kmc_propose_noalloc: the evaluation arrays of one propose carved
+// from one file-scope device array of KMC_NOALLOC_SCRATCH_BYTES.  Each GPU the
+// library runs on holds its own copy until the process ends.  Host code never uses
+// the array's name as an address: the device address is taken once per device, with
+// that device current, by hipGetSymbolAddress, and only that pointer reaches kernels.
+// ---------------------------------------------------------------------------
+constexpr int KMC_MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[KMC_NOALLOC_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[KMC_MAX_DEVICES] = {};
+
+// the current device's copy of the scratch array
+static int scratch_ptr(void** out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= KMC_MAX_DEVICES) return set_err(KMC_E_ARG, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return set_err(KMC_E_HIP, "hipGetSymbolAddress(scratch) returned null");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return KMC_OK;
+}
+
+// Bytes of scratch a propose of P moving charges and T proposals takes, and (base !=
+// null) the carve of the evaluation arrays from base, each 256-byte aligned.
+static size_t carve_work(const kmc_ctx* c, int64_t P, int64_t T, unsigned char* base, ProposeWork* w) {
+    const size_t ne = (size_t)(T + P), nfar = (size_t)((c->L + 1) / 2);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        unsigned char* p = base ? base + off : nullptr;
+        off += (bytes + 255) & ~(size_t)255;
+        return (void*)p;
+    };
+    ProposeWork k{};
+    k.err = (int*)take(sizeof(int));
+    k.epos4 = (double4*)take(ne * sizeof(double4));
+    k.ecell = (int32_t*)take(ne * sizeof(int32_t));
+    k.eexcl = (int32_t*)take(ne * sizeof(int32_t));
+    k.owner = (int32_t*)take((size_t)T * sizeof(int32_t));
+    k.UD = (double*)take(ne * sizeof(double));
+    k.UI = (double*)take(ne * nfar * sizeof(double));
+    k.es_copy = (int64_t*)take((size_t)(P + 1) * sizeof(int64_t));
+    if (uses_lr(c)) {
+        k.lrd = (double*)take((size_t)12 * T * sizeof(double));
+        k.aold = (double*)take((size_t)12 * P * lr_nbmax(c->L) * sizeof(double));
+        k.cold = (double*)take((size_t)12 * P * sizeof(double));
+    }
+    if (w) *w = k;
+    return off;
+}
+
 extern "C" {
 
 const char* kmc_last_error(void) { return g_err.c_str(); }
@@ -2065,6 +2127,36 @@ int kmc_propose(kmc_ctx* c, void* stream, int64_t P, int64_t T, const int64_t* d
     HIPCHK(hipStreamSynchronize(s));
     if (herr) return fail_err_counter(c, s, KMC_E_BAD_MOVE, "Bad proposed move was detected.");
     return run_propose_core(c, s, P, T, d_dE, nullptr, precision == KMC_PREC_MIXED);
+}
+
+int kmc_propose_noalloc(kmc_ctx* c, void* stream, int64_t P, int64_t T, const int64_t* d_pid, const int64_t* d_es,
+                        const double* d_new_pos, const double* d_new_q, double* d_dE, int32_t precision) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    if (!c->initialised) return set_err(KMC_E_STATE, "Run initialise before this call");
+    if (P < 0 || T < 0 || T >= (int64_t)1 << 30) return set_err(KMC_E_ARG, "bad proposal counts");
+    if (P == 0 || T == 0) return KMC_OK;
+    if (!d_pid || !d_es || !d_new_pos || !d_dE) return set_err(KMC_E_ARG, "null device pointer");
+    const size_t need = carve_work(c, P, T, nullptr, nullptr);
+    if (need > (size_t)KMC_NOALLOC_SCRATCH_BYTES)
+        return set_err(KMC_E_ARG, "kmc_propose_noalloc: " + std::to_string(P) + " moving charges and " +
+                                      std::to_string(T) + " proposals need " + std::to_string(need) +
+                                      " bytes of scratch, more than its fixed " +
+                                      std::to_string((long long)KMC_NOALLOC_SCRATCH_BYTES) + "; use kmc_propose");
+    hipStream_t s = (hipStream_t)stream;
+    void* base = nullptr;
+    int rc = scratch_ptr(&base);
+    if (rc) return rc;
+    ProposeWork w;
+    carve_work(c, P, T, (unsigned char*)base, &w);
+    // the bad-move counter must start at zero (the array keeps the last call's contents)
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, w.err, 1, 0);
+    rc = launch_setup(c, s, P, T, d_pid, d_es, d_new_pos, d_new_q, &w);
+    if (rc) return rc;
+    int herr = 0;
+    HIPCHK(hipMemcpyAsync(&herr, w.err, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (herr) return set_err(KMC_E_BAD_MOVE, "Bad proposed move was detected.");
+    return run_propose_core(c, s, P, T, d_dE, nullptr, precision == KMC_PREC_MIXED, 0.0, nullptr, nullptr, 0, 0, &w);
 }
 
 int kmc_propose_with_dats(kmc_ctx* c, void* stream, int64_t max_prop, const int64_t* d_smc, const int64_t* d_sites,

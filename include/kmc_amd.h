@@ -119,6 +119,25 @@ int kmc_propose(kmc_ctx* ctx, void* stream, int64_t P, int64_t T, const int64_t*
                 int32_t precision);
 
 /*
+ * kmc_propose without allocation.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and errors as kmc_propose, but the evaluation arrays live in
+ * a fixed device array of KMC_NOALLOC_SCRATCH_BYTES that the library holds on each GPU
+ * it runs on until the process ends; the call makes no hipMalloc / hipFree.  Returns
+ * KMC_E_ARG (the message gives the bytes needed) when P moving charges and T proposals
+ * need more scratch than that: (T + P)(48 + 8 ceil(L/2)) + 4 T + 8 P bytes, plus
+ * 96 T + 96 P (nb + 1) for pbc / ff-only (nb = 22 at L = 12, 36 at L = 16), each array
+ * rounded up to 256 bytes: 3.8 MiB for 1024 charges x 8 proposals at L = 12 (pbc),
+ * 21 MiB for 4096 x 8 at L = 16.  Every call on one GPU shares that GPU's array (the
+ * last call's contents stay in it): calls that overlap on one GPU (several host
+ * threads or streams) race.  Synchronises the stream (bad-move check).
+ */
+#define KMC_NOALLOC_SCRATCH_BYTES ((int64_t)32 << 20)
+int kmc_propose_noalloc(kmc_ctx* ctx, void* stream, int64_t P, int64_t T, const int64_t* d_pid,
+                        const int64_t* d_es, const double* d_new_pos, const double* d_new_q, double* d_dE,
+                        int32_t precision);
+
+/*
  * KMCFMM.propose_with_dats (coulomb_kmc/kmc_fmm.py:665-762 with setup_move,
  * coulomb_kmc/kmc_mpi_decomp.py:501-616).  Device arrays: site_max_counts[nsites],
  * current_sites[n], prop_positions[n][max_prop][3], prop_masks[n][max_prop],
