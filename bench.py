@@ -59,6 +59,52 @@ def sweep_flops_per_source(L):
     return 9 * L * (L + 1) // 2 + 6 * (L - 1) + 30
 
 
+SWEEP_TAU = 2.0 ** -60
+
+
+def sweep_work(E, ncps, L, periodic, old, new, blocked):
+    """FP64 flops the accept sweep does for the hop old -> new: over every (leaf, source
+    image) pair that is well separated (the others add nothing), sweep_flops_per_source
+    of the image's order: L, or with the blocked layout (sweep_block 1) the truncation
+    order lc of the image against the leaf's 4x4x4 block (the smallest lc with
+    (rho/D)^lc / (D - rho) <= 2^-60, D the distance from the image to the block's leaf
+    centres, rho a leaf's half diagonal), the rule the kernel applies."""
+    w = E / ncps
+    c0 = -0.5 * E + 0.5 * w
+    rho = 0.5 * np.sqrt(3.0) * w
+    ar = np.arange(ncps)
+    tx, ty, tz = np.meshgrid(ar, ar, ar, indexing="ij")
+    R = 1 if periodic else 0
+    total = 0.0
+    for src in (old, new):
+        src = np.asarray(src, dtype=np.float64)
+        sc = np.minimum(((src + 0.5 * E) / w).astype(int), ncps - 1)
+        for oz in range(-R, R + 1):
+            for oy in range(-R, R + 1):
+                for ox in range(-R, R + 1):
+                    near = ((np.abs(sc[0] + ox * ncps - tx) < 2) & (np.abs(sc[1] + oy * ncps - ty) < 2) &
+                            (np.abs(sc[2] + oz * ncps - tz) < 2))
+                    if blocked:
+                        s = src + np.array([ox, oy, oz]) * E
+                        e = [np.maximum(np.abs(s[d] - (c0 + (4 * (t // 4) + 1.5) * w)) - 1.5 * w, 0.0)
+                             for d, t in enumerate((tx, ty, tz))]
+                        D = np.sqrt(e[0] ** 2 + e[1] ** 2 + e[2] ** 2)
+                        far = D > 2 * rho
+                        Ds = np.where(far, D, 1.0)
+                        bound = np.where(far, 1.0 / np.where(far, Ds - rho, 1.0), np.inf)
+                        lc = np.zeros(D.shape, int)
+                        for _ in range(L):
+                            more = (bound > SWEEP_TAU) & (lc < L)
+                            bound = np.where(more, bound * (rho / Ds), bound)
+                            lc = lc + more
+                        lc = np.where(far, lc, L)
+                    else:
+                        lc = np.full(tx.shape, L)
+                    f = 9 * lc * (lc + 1) // 2 + 6 * (lc - 1) + 30
+                    total += float(np.sum(np.where(near, 0, f)))
+    return total
+
+
 def kernel_rooflines(kt, N, ncells, L, P, T, bc, sweep_flops, sweep_bytes):
     """Per-kernel achieved rate against the attainable one, min(FP64 peak, AI x HBM peak),
     from the algorithmic work per launch (add/mul 1, FMA 2, rsqrt 1):
@@ -364,6 +410,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--near-lanes", type=int, default=0, help="0: autotuned")
+    ap.add_argument("--sweep-block", type=int, default=-1,
+                    help="accept-sweep layout (kmc_set_tiles sweep_block; -1: autotuned): 0 fused, 1 blocked with "
+                         "order truncation, 64..256 leaves per workgroup")
     ap.add_argument("--pipeline", default="auto", choices=("auto", "on", "off", "setup"),
                     help="on: kmc_run_steps with each step's sweep beside the next step's near field and long "
                          "range; setup: beside its set-up and long range only; off: kmc_materialize_moves + "
@@ -415,7 +464,8 @@ def main():
     log("move stream", round(time.time() - t0, 2), "s")
 
     kmc = KMCFMM(pos0.copy(), q.reshape(-1, 1), E, r=r, l=L, boundary_condition=bc, max_move=cfg["radius"])
-    tiles = tune.choose_tiles(L, T, (2 ** (r - 1)) ** 3, near_lanes=args.near_lanes or None)
+    tiles = tune.choose_tiles(L, T, (2 ** (r - 1)) ** 3, near_lanes=args.near_lanes or None,
+                              sweep_block=None if args.sweep_block < 0 else args.sweep_block)
     kmc.set_tiles(**tiles)
     torch.cuda.synchronize()
     t0 = time.time()
@@ -620,28 +670,54 @@ def main():
     kt = {}
     for name in ("near", "far", "lr"):
         kt[name] = kmc.time_kernel(name, reps=20, precision="mixed")
-    kt["sweep"] = kmc.time_kernel("sweep", reps=20)
+    # the accept sweep, timed on three hops of a charge (kmc_sweep_hop sets the move that
+    # kmc_time_kernel re-applies); its flops counted for each hop by sweep_work
+    ncps = 2 ** (r - 1)
+    ncells = ncps ** 3
+    blocked = tiles["sweep_block"] == 1 and ncps % 4 == 0 and bc != "ff-only"
+    hop_rng = np.random.default_rng(cfg["seed"] + 7)
+    t_sum, f_sum = 0.0, 0.0
+    for _ in range(3):
+        old = hop_rng.uniform(-0.5 * E, 0.5 * E, 3)
+        d = hop_rng.normal(size=3)
+        new = old + cfg["radius"] * d / np.linalg.norm(d) * hop_rng.uniform() ** (1.0 / 3.0)
+        new = np.where(new < -0.5 * E, new + E, np.where(new >= 0.5 * E, new - E, new)) if bc != "free_space" \
+            else np.clip(new, -0.5 * E, 0.5 * E)
+        kmc.sweep_hop(1.0, old, new)
+        t_sum += kmc.time_kernel("sweep", reps=10)
+        f_sum += sweep_work(E, ncps, L, bc != "free_space", old, new, blocked)
+    kt["sweep"] = t_sum / 3
+    sweep_flops = f_sum / 3
     log("kernel ms", kt)
-    ncells = (2 ** (r - 1)) ** 3
     nimg = 54 if bc != "free_space" else 2
-    sweep_flops = ncells * nimg * sweep_flops_per_source(L)
     sweep_bytes = 2 * ncells * (L * (L + 1) // 2) * 16
     ach_tflops = sweep_flops / (kt["sweep"] * 1e-3) / 1e12
     ach_gbs = sweep_bytes / (kt["sweep"] * 1e-3) / 1e9
-    roof = {"bound": "valu", "kernel": "k_sweep (accept sweep)", "achieved": ach_tflops,
-            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach_tflops / FP64_PEAK_TFLOPS,
-            "traffic": None, "launch_ms": kt["sweep"], "algorithmic_flops": sweep_flops,
-            "algorithmic_bytes": sweep_bytes, "hbm_GBs": ach_gbs, "hbm_frac": ach_gbs / HBM_PEAK_GBS,
-            "note": "FP64 vector ALU bound (no dense contraction: MFMA unused); peak = MI355X FP64 "
-                    "vector rate; flops = leaves x 54 images x (9 L(L+1)/2 + 6(L-1) + 30)"}
+    ai = sweep_flops / sweep_bytes
+    hbm_bound = ai * HBM_PEAK_GBS / 1e3 < FP64_PEAK_TFLOPS
+    flop_note = ("flops = sum over (leaf, well-separated source image) of 9 lc(lc+1)/2 + 6(lc-1) + 30, lc the "
+                 "image's truncation order against the leaf's 4x4x4 block (mean of 3 hops; all L orders would be "
+                 "{:.3g})".format(ncells * nimg * sweep_flops_per_source(L)) if blocked else
+                 "flops = leaves x {} images x (9 L(L+1)/2 + 6(L-1) + 30)".format(nimg))
+    if hbm_bound:
+        roof = {"bound": "hbm", "kernel": "accept sweep ({})".format("k_sweep_block" if blocked else "k_sweep"),
+                "achieved": ach_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach_gbs / HBM_PEAK_GBS,
+                "fp64_TFLOPs": ach_tflops, "fp64_frac": ach_tflops / FP64_PEAK_TFLOPS}
+    else:
+        roof = {"bound": "valu", "kernel": "accept sweep ({})".format("k_sweep_block" if blocked else "k_sweep"),
+                "achieved": ach_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": ach_tflops / FP64_PEAK_TFLOPS, "hbm_GBs": ach_gbs, "hbm_frac": ach_gbs / HBM_PEAK_GBS}
+    roof.update({"traffic": None, "launch_ms": kt["sweep"], "algorithmic_flops": sweep_flops,
+                 "algorithmic_bytes": sweep_bytes, "flop_per_byte": ai,
+                 "note": "no dense contraction: MFMA unused; bytes = read + write of every leaf's packed "
+                         "expansion; " + flop_note})
     traffic = tune.read_traffic(ROOT, "k_sweep", cfg)
     if traffic is not None:
         roof["traffic"] = traffic
     peaks = measured_peaks()
     log("measured peaks", peaks)
     roof["measured_peaks"] = peaks
-    roof["frac_of_measured"] = ach_tflops / peaks["fp64_fma_TFLOPs"]
-    roof["hbm_frac_of_measured"] = ach_gbs / peaks["copy_GBs"]
+    roof["frac_of_measured"] = (ach_gbs / peaks["copy_GBs"]) if hbm_bound else (ach_tflops / peaks["fp64_fma_TFLOPs"])
 
     mixed, fp64 = results["mixed"], results["fp64"]
     kroof = kernel_rooflines(kt, N, ncells, L, P, T, bc, sweep_flops, sweep_bytes)

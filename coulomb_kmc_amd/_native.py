@@ -133,7 +133,7 @@ def lib():
         "kmc_ncells": (ctypes.c_int, [P, P]),
         "kmc_propose": (ctypes.c_int, [P, P, I64, I64, P, P, P, P, P, I32]),
         "kmc_propose_noalloc": (ctypes.c_int, [P, P, I64, I64, P, P, P, P, P, I32]),
-        "kmc_propose_with_dats": (ctypes.c_int, [P, P, I64, P, P, P, P, P, P, I32, P]),
+        "kmc_propose_with_dats": (ctypes.c_int, [P, P, I64, P, I64, P, P, P, P, P, I32, P]),
         "kmc_accept": (ctypes.c_int, [P, P, I64, P, P]),
         "kmc_step": (ctypes.c_int, [P, P, I64, I64, P, P, P, P, I64, P, D, P, P, I32]),
         "kmc_materialize_moves": (ctypes.c_int, [P, P, I64, I64, I64, P, P, P, P, P, P]),

@@ -36,6 +36,193 @@ __global__ void k_fill_i32(int* p, int n, int v) {
     if (i < n) p[i] = v;
 }
 
+// ============================================================================
+// near field by leaf tiles
+// ============================================================================
+// The evaluation points are counting-sorted by the 4x4x4 tile of leaves that holds
+// them (k_tile_count, k_tile_scan, k_tile_scatter).  One workgroup per tile copies the
+// charge records of the tile's 6x6x6 leaves (the tile and the leaves around it, the
+// periodic images shifted by the box as the reference's halo copies are,
+// coulomb_kmc/kmc_local.py:796-817) into LDS once, then evaluates every point of the
+// tile from there (G lanes per point, the point's 27 leaves in stencil order, the
+// slots of a leaf strided over the lanes, compensated sums): each record is read from
+// memory once per tile instead of once per point whose stencil holds it.  A tile whose
+// leaves hold more records than the LDS takes evaluates its points from memory.
+constexpr int NT_THREADS = 512;
+constexpr int NT_SLOTS = 2048;   // LDS records per tile (36 B each)
+constexpr int NT_HALO = 216;     // 6^3 leaves
+
+__device__ __forceinline__ int tile_of_cell(int c, int n) {
+    const int nb = n >> 2;
+    return ((c % n) >> 2) + nb * ((((c / n) % n) >> 2) + nb * ((c / (n * n)) >> 2));
+}
+__global__ void k_tile_count(int ne, const int32_t* __restrict__ cell, int n, int* __restrict__ tcount) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < ne) atomicAdd(&tcount[tile_of_cell(cell[x], n)], 1);
+}
+// exclusive scan of tcount[nt] into toff[0..nt] and tcur[0..nt); tcount back to zero
+__global__ void __launch_bounds__(1024) k_tile_scan(int nt, int* __restrict__ tcount, int* __restrict__ toff,
+                                                    int* __restrict__ tcur) {
+    __shared__ int sums[1024];
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const int chunk = (nt + nth - 1) / nth;
+    const int b0 = min(nt, tid * chunk), b1 = min(nt, b0 + chunk);
+    int s = 0;
+    for (int i = b0; i < b1; i++) s += tcount[i];
+    sums[tid] = s;
+    __syncthreads();
+    for (int o = 1; o < nth; o <<= 1) {
+        const int v = tid >= o ? sums[tid - o] : 0;
+        __syncthreads();
+        sums[tid] += v;
+        __syncthreads();
+    }
+    int c = tid > 0 ? sums[tid - 1] : 0;
+    for (int i = b0; i < b1; i++) {
+        toff[i] = c;
+        tcur[i] = c;
+        c += tcount[i];
+        tcount[i] = 0;
+    }
+    if (tid == nth - 1) toff[nt] = sums[nth - 1];
+}
+__global__ void k_tile_scatter(int ne, const int32_t* __restrict__ cell, int n, int* __restrict__ tcur,
+                               int* __restrict__ perm) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < ne) perm[atomicAdd(&tcur[tile_of_cell(cell[x], n)], 1)] = x;
+}
+
+// one point from memory (tiles whose leaves overflow the LDS): near_body's loop
+template <int G, bool MIXED>
+__device__ __forceinline__ CSum near_point_global(const EvalArgs& a, int e, int lane) {
+    CSum acc;
+    const double4 p = a.pos4[e];
+    const int c = a.cell[e];
+    const int ex = a.excl[e];
+    const int n = a.g.ncps;
+    const int tx = c % n, ty = (c / n) % n, tz = c / (n * n);
+    const bool periodic = a.g.bc != KMC_BC_FREE_SPACE;
+    for (int k = 0; k < 27; k++) {
+        int ox, oy, oz;
+        stencil(k, ox, oy, oz);
+        int nx = tx + ox, ny = ty + oy, nz = tz + oz;
+        double shx = 0.0, shy = 0.0, shz = 0.0;
+        if (periodic) {
+            if (nx < 0) { nx += n; shx = -a.g.E; } else if (nx >= n) { nx -= n; shx = a.g.E; }
+            if (ny < 0) { ny += n; shy = -a.g.E; } else if (ny >= n) { ny -= n; shy = a.g.E; }
+            if (nz < 0) { nz += n; shz = -a.g.E; } else if (nz >= n) { nz -= n; shz = a.g.E; }
+        } else if (nx < 0 || nx >= n || ny < 0 || ny >= n || nz < 0 || nz >= n) {
+            continue;
+        }
+        const int nc = nx + n * (ny + n * nz);
+        const int occ = min(a.occ[nc], a.occmax);
+        const size_t base = (size_t)nc * a.occmax;
+        for (int s = lane; s < occ; s += G) {
+            const double4 q = a.slot_pos[base + s];
+            const double dx = p.x - (q.x + shx), dy = p.y - (q.y + shy), dz = p.z - (q.z + shz);
+            const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+            if (ex < 0 || a.slot_id[base + s] != ex) acc.add(q.w * rsqrt_sel<MIXED>(r2));
+        }
+    }
+    return acc;
+}
+
+template <int G, bool MIXED>
+__global__ void __launch_bounds__(NT_THREADS) k_near_tile(EvalArgs a, const int* __restrict__ toff,
+                                                          const int* __restrict__ perm) {
+    __shared__ double4 s_pos[NT_SLOTS];
+    __shared__ int s_id[NT_SLOTS];
+    __shared__ int s_start[NT_HALO + 1];
+    __shared__ int s_leaf[NT_HALO];
+    __shared__ double s_sh[NT_HALO][3];
+    __shared__ int s_wsum[NT_THREADS / 64];
+    const int tile = blockIdx.x;
+    const int p0 = toff[tile], p1 = toff[tile + 1];
+    if (p0 == p1) return;   // no point in this tile (uniform over the workgroup)
+    const int n = a.g.ncps, nb = n >> 2;
+    const int bx = tile % nb, by = (tile / nb) % nb, bz = tile / (nb * nb);
+    const bool periodic = a.g.bc != KMC_BC_FREE_SPACE;
+    const int tid = threadIdx.x, lane64 = tid & 63, wv = tid >> 6;
+    // the 216 leaves: global leaf, image shift, occupancy; exclusive prefix of the
+    // occupancies over the leaves (wave scans, then the wave totals)
+    int occ_h = 0;
+    if (tid < NT_HALO) {
+        int g3[3] = {4 * bx + tid % 6 - 1, 4 * by + (tid / 6) % 6 - 1, 4 * bz + tid / 36 - 1};
+        double sh[3] = {0.0, 0.0, 0.0};
+        bool ok = true;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            if (g3[d] < 0) { if (periodic) { g3[d] += n; sh[d] = -a.g.E; } else ok = false; }
+            else if (g3[d] >= n) { if (periodic) { g3[d] -= n; sh[d] = a.g.E; } else ok = false; }
+        }
+        const int leaf = ok ? g3[0] + n * (g3[1] + n * g3[2]) : 0;
+        occ_h = ok ? min(a.occ[leaf], a.occmax) : 0;
+        s_leaf[tid] = leaf;
+        s_sh[tid][0] = sh[0];
+        s_sh[tid][1] = sh[1];
+        s_sh[tid][2] = sh[2];
+    }
+    int v = occ_h;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(v, o, 64);
+        if (lane64 >= o) v += y;
+    }
+    if (lane64 == 63) s_wsum[wv] = v;
+    __syncthreads();
+    int before = 0;
+    for (int j = 0; j < wv; j++) before += s_wsum[j];
+    if (tid < NT_HALO) s_start[tid] = before + v - occ_h;
+    if (tid == NT_HALO - 1) s_start[NT_HALO] = before + v;
+    __syncthreads();
+    const int total = s_start[NT_HALO];
+    const bool staged = total <= NT_SLOTS;
+    if (staged) {
+        // copy the records, each thread a share of the flattened list (leaf by binary search)
+        for (int g = tid; g < total; g += NT_THREADS) {
+            int lo = 0, hi = NT_HALO - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_start[mid] <= g) lo = mid;
+                else hi = mid - 1;
+            }
+            const size_t sidx = (size_t)s_leaf[lo] * a.occmax + (g - s_start[lo]);
+            const double4 q = a.slot_pos[sidx];
+            s_pos[g] = make_double4(q.x + s_sh[lo][0], q.y + s_sh[lo][1], q.z + s_sh[lo][2], q.w);
+            s_id[g] = a.slot_id[sidx];
+        }
+        __syncthreads();
+    }
+    const int lane = tid % G;
+    for (int q = tid / G; q < p1 - p0; q += NT_THREADS / G) {
+        const int e = perm[p0 + q];
+        CSum acc;
+        if (staged) {
+            const double4 p = a.pos4[e];
+            const int c = a.cell[e];
+            const int ex = a.excl[e];
+            const int lx = c % n - 4 * bx, ly = (c / n) % n - 4 * by, lz = c / (n * n) - 4 * bz;
+            for (int k = 0; k < 27; k++) {
+                int ox, oy, oz;
+                stencil(k, ox, oy, oz);
+                const int h = (lx + ox + 1) + 6 * ((ly + oy + 1) + 6 * (lz + oz + 1));
+                const int s1 = s_start[h + 1];
+                for (int j = s_start[h] + lane; j < s1; j += G) {
+                    const double4 r = s_pos[j];
+                    const double dx = p.x - r.x, dy = p.y - r.y, dz = p.z - r.z;
+                    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                    if (ex < 0 || s_id[j] != ex) acc.add(r.w * rsqrt_sel<MIXED>(r2));
+                }
+            }
+        } else {
+            acc = near_point_global<G, MIXED>(a, e, lane);
+        }
+        acc = csum_group<G>(acc);
+        if (lane == 0) a.out[e] = acc.value() * a.pos4[e].w;
+    }
+}
+
+
 template <int G, bool MIXED>
 __global__ void __launch_bounds__(256) k_near(EvalArgs a) { near_body<G, MIXED>(a); }
 // batched form: system blockIdx.z of an argument table (grid sized for the largest)
@@ -308,16 +495,67 @@ __global__ void __launch_bounds__(1024) kb_select(const SelectArgs* __restrict__
     select_body(args[blockIdx.z]);
 }
 
-__global__ void k_dats_count(int n, int64_t max_prop, const int64_t* smc, const int64_t* sites, const int64_t* masks,
-                             int64_t* cnt, int64_t* has) {
+// a charge whose site index lies outside site_max_counts[nsites] takes no proposal and
+// counts into err (the host returns KMC_E_ARG)
+__global__ void k_dats_count(int n, int64_t max_prop, const int64_t* smc, int64_t nsites, const int64_t* sites,
+                             const int64_t* masks, int64_t* cnt, int64_t* has, int* err) {
     const int px = blockIdx.x * blockDim.x + threadIdx.x;
     if (px >= n) return;
-    const int64_t mx = smc[sites[px]];
+    const int64_t site = sites[px];
+    const bool ok = site >= 0 && site < nsites;
+    if (!ok) atomicAdd(err, 1);
+    const int64_t mx = ok ? smc[site] : 0;
     int64_t found = 0;
     for (int64_t m = 0; m < max_prop && found < mx; m++)
         if (masks[(int64_t)px * max_prop + m] > 0) found++;
     cnt[px] = found;
     has[px] = found > 0 ? 1 : 0;
+}
+
+// exclusive scan of n int64 values, three passes over blocks of SCAN_BLOCK values:
+// per-block sums, a scan of the block sums (one workgroup, k_scan_excl), then each
+// block's own scan plus its offset.  *total receives the sum.
+constexpr int SCAN_BLOCK = 1024;
+__global__ void __launch_bounds__(256) k_scan_block_sums(int n, const int64_t* __restrict__ in,
+                                                         int64_t* __restrict__ bsum) {
+    __shared__ int64_t part[4];
+    const int b0 = blockIdx.x * SCAN_BLOCK, tid = threadIdx.x;
+    int64_t v = 0;
+    for (int i = b0 + tid; i < min(n, b0 + SCAN_BLOCK); i += 256) v += in[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((tid & 63) == 0) part[tid >> 6] = v;
+    __syncthreads();
+    if (tid == 0) bsum[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+__global__ void __launch_bounds__(256) k_scan_block_apply(int n, const int64_t* __restrict__ in,
+                                                          const int64_t* __restrict__ boff, int64_t* __restrict__ out) {
+    // each thread 4 consecutive values; wave scans, then the four wave totals
+    __shared__ int64_t wsum[4];
+    const int b0 = blockIdx.x * SCAN_BLOCK, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    int64_t x[4], t = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int i = b0 + 4 * tid + j;
+        x[j] = i < n ? in[i] : 0;
+        t += x[j];
+    }
+    int64_t incl = t;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int64_t y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) wsum[wv] = incl;
+    __syncthreads();
+    int64_t c = boff[blockIdx.x] + incl - t;
+    for (int w = 0; w < wv; w++) c += wsum[w];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int i = b0 + 4 * tid + j;
+        if (i < n) out[i] = c;
+        c += x[j];
+    }
 }
 
 // exclusive scan of n int64 values (single block, sequential chunks per thread)
@@ -355,7 +593,7 @@ __global__ void k_dats_scatter(int n, int64_t max_prop, const int64_t* smc, cons
     const int64_t p = slot[px];
     pid[p] = px;
     es[p] = off[px];
-    const int64_t mx = smc[sites[px]];
+    const int64_t mx = smc[sites[px]];   // in range: k_dats_count gave cnt 0 otherwise
     int64_t found = 0;
     for (int64_t m = 0; m < max_prop && found < mx; m++)
         if (masks[(int64_t)px * max_prop + m] > 0) {
@@ -961,6 +1199,7 @@ static void launch_near_t(EvalArgs a, int G, hipStream_t s) {
     }
 }
 static void launch_near(EvalArgs a, int G, bool mixed, hipStream_t s) {
+    if (G < 0) G = -G;   // the leaf-tiled layout needs the sorted points of a propose
     if (mixed) launch_near_t<true>(a, G, s);
     else launch_near_t<false>(a, G, s);
 }
@@ -1059,9 +1298,10 @@ struct kmc_ctx {
     DBuf<double> UD, UI, lrd, dE1, rates, bsum;
     DBuf<int64_t> one_pid, one_es;
     DBuf<int64_t> es_copy;            // es of the last propose set-up
+    DBuf<int> tcount, toff, tcur, perm;   // leaf-tiled near field (counting sort by tile)
     DBuf<double> one_pos;
     // propose_with_dats work buffers
-    DBuf<int64_t> d_cnt, d_has, d_slot, d_off, d_pid, d_es, d_rate, d_tot;
+    DBuf<int64_t> d_cnt, d_has, d_slot, d_off, d_pid, d_es, d_rate, d_tot, d_scan;
     DBuf<double> d_npos, d_nq;
     // lattice steps: every charge proposes the same M offsets
     DBuf<int64_t> lat_pid, lat_es;
@@ -1071,6 +1311,11 @@ struct kmc_ctx {
 
     Geom geom() const { return Geom{E, w, -0.5 * E + 0.5 * w, ncps, bc}; }
 };
+
+// leaf tiles (4x4x4) of the context's grid, 0 when the grid is not a multiple of 4 per side
+static int near_tiles(const kmc_ctx* c) {
+    return (c->ncps % 4 == 0) ? (c->ncps / 4) * (c->ncps / 4) * (c->ncps / 4) : 0;
+}
 
 static int check_ctx(kmc_ctx* c) {
     if (!c) return set_err(KMC_E_ARG, "null context");
@@ -1097,6 +1342,15 @@ static int ensure_eval(kmc_ctx* c, int64_t T, int64_t P) {
     HIPCHK(c->rates.ensure((size_t)std::max<int64_t>(T, 1)));
     HIPCHK(c->bsum.ensure((size_t)std::max<int64_t>((T + 63) / 64, 1)));
     HIPCHK(c->es_copy.ensure((size_t)P + 1));
+    const int nt = near_tiles(c);
+    if (nt > 0) {
+        const bool fresh = c->tcount.n < (size_t)nt;
+        HIPCHK(c->tcount.ensure(nt));
+        if (fresh) HIPCHK(hipMemset(c->tcount.p, 0, sizeof(int) * nt));   // the scan keeps it zero after
+        HIPCHK(c->toff.ensure((size_t)nt + 1));
+        HIPCHK(c->tcur.ensure(nt));
+        HIPCHK(c->perm.ensure(ne));
+    }
     return KMC_OK;
 }
 
@@ -1109,10 +1363,39 @@ struct ProposeWork {
     double *UD, *UI, *lrd, *aold, *cold;
     int64_t* es_copy;
     int* err;
+    // leaf-tiled near field: per-tile counts (zero between calls), offsets, cursors, and
+    // the evaluation entries sorted by tile
+    int *tcount, *toff, *tcur, *perm;
 };
 static ProposeWork ctx_work(kmc_ctx* c) {
-    return ProposeWork{c->epos4.p, c->ecell.p, c->eexcl.p, c->owner.p, c->UD.p, c->UI.p,
-                       c->lrd.p,   c->aold.p,  c->cold.p,  c->es_copy.p, c->err.p};
+    return ProposeWork{c->epos4.p, c->ecell.p, c->eexcl.p, c->owner.p, c->UD.p,   c->UI.p,   c->lrd.p,
+                       c->aold.p,  c->cold.p,  c->es_copy.p, c->err.p, c->tcount.p, c->toff.p, c->tcur.p,
+                       c->perm.p};
+}
+
+// near field of a propose's evaluation entries: the leaf-tiled kernel (near_lanes < 0,
+// grids with a multiple of 4 leaves per side), the grouped one (0) or the per-point ones
+template <bool MIXED>
+static void launch_near_tiled(const kmc_ctx* c, hipStream_t s, const EvalArgs& a, int G, const ProposeWork& w) {
+    const int n = c->ncps, nt = near_tiles(c), ne = a.n;
+    hipLaunchKernelGGL(k_tile_count, dim3((ne + 255) / 256), dim3(256), 0, s, ne, a.cell, n, w.tcount);
+    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, nt, w.tcount, w.toff, w.tcur);
+    hipLaunchKernelGGL(k_tile_scatter, dim3((ne + 255) / 256), dim3(256), 0, s, ne, a.cell, n, w.tcur, w.perm);
+    switch (G) {
+        case 4: hipLaunchKernelGGL((k_near_tile<4, MIXED>), dim3(nt), dim3(NT_THREADS), 0, s, a, w.toff, w.perm); break;
+        case 8: hipLaunchKernelGGL((k_near_tile<8, MIXED>), dim3(nt), dim3(NT_THREADS), 0, s, a, w.toff, w.perm); break;
+        default: hipLaunchKernelGGL((k_near_tile<16, MIXED>), dim3(nt), dim3(NT_THREADS), 0, s, a, w.toff, w.perm); break;
+    }
+}
+static void launch_near_any(const kmc_ctx* c, hipStream_t s, const EvalArgs& a, bool mixed, int P, int T,
+                            const ProposeWork& w) {
+    if (a.n <= 0) return;
+    if (c->near_lanes < 0 && near_tiles(c) > 0 && w.perm) {
+        if (mixed) launch_near_tiled<true>(c, s, a, -c->near_lanes, w);
+        else launch_near_tiled<false>(c, s, a, -c->near_lanes, w);
+        return;
+    }
+    launch_near_prop(a, c->near_lanes, mixed, P, T, w.es_copy, s);
 }
 
 static EvalArgs eval_args(kmc_ctx* c, int n, const double4* pos4, const int32_t* cell, const int32_t* excl, double* out) {
@@ -1319,7 +1602,7 @@ static int run_propose_core(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, dou
         // set-up and the long range overlap it)
         if (wait_sweep == 2) HIPCHK(hipStreamWaitEvent(s, c->ev_sweep, 0));
         EvalArgs an = eval_args(c, ne, wk.epos4, wk.ecell, wk.eexcl, wk.UD);
-        launch_near_prop(an, c->near_lanes, mixed, (int)P, (int)T, wk.es_copy, s);
+        launch_near_any(c, s, an, mixed, (int)P, (int)T, wk);
         // the far field reads the leaf expansions: after the previous step's sweep
         if (wait_sweep == 1) HIPCHK(hipStreamWaitEvent(s, c->ev_sweep, 0));
         EvalArgs af = eval_args(c, ne, wk.epos4, wk.ecell, wk.eexcl, wk.UI);
@@ -1609,6 +1892,13 @@ static size_t carve_work(const kmc_ctx* c, int64_t P, int64_t T, unsigned char* 
     k.UD = (double*)take(ne * sizeof(double));
     k.UI = (double*)take(ne * nfar * sizeof(double));
     k.es_copy = (int64_t*)take((size_t)(P + 1) * sizeof(int64_t));
+    const int nt = near_tiles(c);
+    if (nt > 0) {
+        k.tcount = (int*)take((size_t)nt * sizeof(int));
+        k.toff = (int*)take((size_t)(nt + 1) * sizeof(int));
+        k.tcur = (int*)take((size_t)nt * sizeof(int));
+        k.perm = (int*)take(ne * sizeof(int));
+    }
     if (uses_lr(c)) {
         k.lrd = (double*)take((size_t)12 * T * sizeof(double));
         k.aold = (double*)take((size_t)12 * P * lr_nbmax(c->L) * sizeof(double));
@@ -1717,9 +2007,11 @@ int kmc_destroy(kmc_ctx* c) {
 int kmc_set_tiles(kmc_ctx* c, int32_t near_lanes, int32_t sweep_block, int32_t sweep_pairs, int32_t sweep_split) {
     if (check_ctx(c)) return KMC_E_ARG;
     if (near_lanes != 0 && near_lanes != 1 && near_lanes != 4 && near_lanes != 8 && near_lanes != 16 &&
-        near_lanes != 27 && near_lanes != 32 && near_lanes != 64)
-        return set_err(KMC_E_ARG, "near_lanes must be one of 0,1,4,8,16,27,32,64 (27: one lane per neighbour leaf, "
-                                  "0: the proposals of a moving charge together)");
+        near_lanes != 27 && near_lanes != 32 && near_lanes != 64 && near_lanes != -4 && near_lanes != -8 &&
+        near_lanes != -16)
+        return set_err(KMC_E_ARG, "near_lanes must be one of 0,1,4,8,16,27,32,64,-4,-8,-16 (27: one lane per "
+                                  "neighbour leaf, 0: the proposals of a moving charge together, -G: points sorted "
+                                  "by 4x4x4 leaf tile, the tile's charges staged in LDS, G lanes per point)");
     if (sweep_pairs < 1 || sweep_pairs > 2) return set_err(KMC_E_ARG, "sweep_pairs must be 1 or 2");
     if (sweep_block != 0 && sweep_block != 1 && (sweep_block < 64 || sweep_block > 256 || sweep_block % 64))
         return set_err(KMC_E_ARG, "sweep_block must be 0 (fused), 1 (4x4x4 leaf blocks with order truncation), "
@@ -2148,8 +2440,13 @@ int kmc_propose_noalloc(kmc_ctx* c, void* stream, int64_t P, int64_t T, const in
     if (rc) return rc;
     ProposeWork w;
     carve_work(c, P, T, (unsigned char*)base, &w);
-    // the bad-move counter must start at zero (the array keeps the last call's contents)
+    // the bad-move counter and the tile counts start at zero (the array keeps the last
+    // call's contents)
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, w.err, 1, 0);
+    if (w.tcount) {
+        const int nt = near_tiles(c);
+        hipLaunchKernelGGL(k_fill_i32, dim3((nt + 255) / 256), dim3(256), 0, s, w.tcount, nt, 0);
+    }
     rc = launch_setup(c, s, P, T, d_pid, d_es, d_new_pos, d_new_q, &w);
     if (rc) return rc;
     int herr = 0;
@@ -2159,12 +2456,22 @@ int kmc_propose_noalloc(kmc_ctx* c, void* stream, int64_t P, int64_t T, const in
     return run_propose_core(c, s, P, T, d_dE, nullptr, precision == KMC_PREC_MIXED, 0.0, nullptr, nullptr, 0, 0, &w);
 }
 
-int kmc_propose_with_dats(kmc_ctx* c, void* stream, int64_t max_prop, const int64_t* d_smc, const int64_t* d_sites,
-                          const double* d_ppos, const int64_t* d_masks, const double* d_pq, double* d_out,
-                          int32_t precision, int64_t* h_total_moves) {
+// exclusive scan of n int64 values over many workgroups (tmp: 2 * ceil(n / SCAN_BLOCK))
+static void scan_excl_i64(hipStream_t s, int n, const int64_t* in, int64_t* out, int64_t* total, int64_t* tmp) {
+    const int nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    hipLaunchKernelGGL(k_scan_block_sums, dim3(nb), dim3(256), 0, s, n, in, tmp);
+    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, s, nb, tmp, tmp + nb, total);
+    hipLaunchKernelGGL(k_scan_block_apply, dim3(nb), dim3(256), 0, s, n, in, tmp + nb, out);
+}
+
+int kmc_propose_with_dats(kmc_ctx* c, void* stream, int64_t max_prop, const int64_t* d_smc, int64_t nsites,
+                          const int64_t* d_sites, const double* d_ppos, const int64_t* d_masks, const double* d_pq,
+                          double* d_out, int32_t precision, int64_t* h_total_moves) {
     if (check_ctx(c)) return KMC_E_ARG;
     if (!c->initialised) return set_err(KMC_E_STATE, "Run initialise before this call");
     if (max_prop < 1) return set_err(KMC_E_ARG, "max_prop must be >= 1");
+    if (nsites < 1 || !d_smc || !d_sites || !d_ppos || !d_masks || !d_out)
+        return set_err(KMC_E_ARG, "bad site_max_counts / null device pointer");
     hipStream_t s = (hipStream_t)stream;
     const int n = (int)c->n;
     HIPCHK(c->d_cnt.ensure(n));
@@ -2172,13 +2479,18 @@ int kmc_propose_with_dats(kmc_ctx* c, void* stream, int64_t max_prop, const int6
     HIPCHK(c->d_slot.ensure(n));
     HIPCHK(c->d_off.ensure(n));
     HIPCHK(c->d_tot.ensure(2));
-    hipLaunchKernelGGL(k_dats_count, dim3((n + 255) / 256), dim3(256), 0, s, n, max_prop, d_smc, d_sites, d_masks,
-                       c->d_cnt.p, c->d_has.p);
-    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, s, n, c->d_cnt.p, c->d_off.p, c->d_tot.p);
-    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, s, n, c->d_has.p, c->d_slot.p, c->d_tot.p + 1);
+    HIPCHK(c->d_scan.ensure((size_t)2 * ((n + SCAN_BLOCK - 1) / SCAN_BLOCK) + 2));
+    HIPCHK(hipMemsetAsync(c->err.p, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_dats_count, dim3((n + 255) / 256), dim3(256), 0, s, n, max_prop, d_smc, nsites, d_sites,
+                       d_masks, c->d_cnt.p, c->d_has.p, c->err.p);
+    scan_excl_i64(s, n, c->d_cnt.p, c->d_off.p, c->d_tot.p, c->d_scan.p);
+    scan_excl_i64(s, n, c->d_has.p, c->d_slot.p, c->d_tot.p + 1, c->d_scan.p);
     int64_t tot[2] = {0, 0};
+    int herr0 = 0;
     HIPCHK(hipMemcpyAsync(tot, c->d_tot.p, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&herr0, c->err.p, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    if (herr0) return fail_err_counter(c, s, KMC_E_ARG, "current_sites holds a site outside site_max_counts");
     const int64_t T = tot[0], P = tot[1];
     if (h_total_moves) *h_total_moves = T;
     if (P == 0 || T == 0) return KMC_OK;
@@ -2339,18 +2651,28 @@ int kmc_run_steps(kmc_ctx* c, void* stream, int64_t nrun, int64_t nsteps_moves, 
     const int64_t mthreads = std::max(T, P + 1);
     const MatArgs ma = mat_args(c, nsteps_moves, P, k, d_pids_all, d_disp_all, d_counter, d_pid, d_es, d_new_pos);
     const SelectArgs sa = select_args(c, P, T, d_pid, d_dE, nsteps_uniform, d_uniform, kT, d_counter, d_selected);
-    for (int64_t it = 0; it < nrun; it++) {
+    int rc = KMC_OK;
+    int64_t done = 0;   // steps whose sweep may be pending on side2
+    for (int64_t it = 0; it < nrun && rc == KMC_OK; it++) {
         hipLaunchKernelGGL(k_materialize, dim3((mthreads + 255) / 256), dim3(256), 0, s, ma);
-        int rc = launch_setup(c, s, P, T, d_pid, d_es, d_new_pos, nullptr);
-        if (rc) return rc;
-        rc = run_propose_core(c, s, P, T, d_dE, nullptr, mixed, kT, c->rates.p, nullptr, 0,
-                              (pipe && it > 0) ? (pipelined == 2 ? 2 : 1) : 0);
-        if (rc) return rc;
-        hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, s, sa);
-        rc = run_accept_tail(c, s, d_counter, pipe);
-        if (rc) return rc;
+        rc = launch_setup(c, s, P, T, d_pid, d_es, d_new_pos, nullptr);
+        if (rc == KMC_OK)
+            rc = run_propose_core(c, s, P, T, d_dE, nullptr, mixed, kT, c->rates.p, nullptr, 0,
+                                  (pipe && it > 0) ? (pipelined == 2 ? 2 : 1) : 0);
+        if (rc == KMC_OK) {
+            hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, s, sa);
+            rc = run_accept_tail(c, s, d_counter, pipe);
+            if (rc == KMC_OK) done = it + 1;
+        }
     }
-    if (pipe && nrun > 0) HIPCHK(hipStreamWaitEvent(s, c->ev_sweep, 0));
+    // join the last deferred sweep into the call's stream, also when a step failed (a
+    // capture must end joined, and later work on s must follow the sweep)
+    if (pipe && done > 0) {
+        const hipError_t e = hipStreamWaitEvent(s, c->ev_sweep, 0);
+        if (rc == KMC_OK && e != hipSuccess)
+            return set_err(KMC_E_HIP, std::string("hipStreamWaitEvent: ") + hipGetErrorString(e));
+    }
+    if (rc) return rc;
     c->mirrors_stale = true;
     HIPCHK(hipGetLastError());
     return KMC_OK;
@@ -2392,7 +2714,7 @@ int kmc_time_kernel(kmc_ctx* c, void* stream, int32_t which, int32_t reps, int32
             c->fns.sweep(sa, (int)c->ncells, s, c->sweep_block, c->sweep_pairs, c->sweep_split);
         } else if (which == 1) {
             EvalArgs an = eval_args(c, (int)(T + P), c->epos4.p, c->ecell.p, c->eexcl.p, c->UD.p);
-            launch_near_prop(an, c->near_lanes, mixed, (int)P, (int)T, c->es_copy.p, s);
+            launch_near_any(c, s, an, mixed, (int)P, (int)T, ctx_work(c));
         } else if (which == 2) {
             EvalArgs af = eval_args(c, (int)(T + P), c->epos4.p, c->ecell.p, c->eexcl.p, c->UI.p);
             c->fns.far(af, s);
@@ -3032,6 +3354,7 @@ int kmc_batch_prepare(kmc_batch* b, double kT, int32_t precision) {
     for (const kmc_batch::Sys& y : b->sys)
         if (y.c->near_lanes != b->near_lanes) b->near_lanes = 27;
     if (b->near_lanes == 0) b->near_lanes = 27;
+    if (b->near_lanes < 0) b->near_lanes = -b->near_lanes;   // batches use the per-point kernels
     for (size_t i = 0; i < b->sys.size(); i++) {
         kmc_batch::Sys& y = b->sys[i];
         if (!y.set) return set_err(KMC_E_STATE, "batch system " + std::to_string(i) + " has no move stream");

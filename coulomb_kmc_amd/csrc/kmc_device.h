@@ -248,9 +248,11 @@ static constexpr RecTab2 kRec2 = make_rectab2();
 // one column m of NI source-images: acc[k] += e_{m+k} * W_j, k = 0 .. L-m-1, j = 0 .. NI-1.
 // The NI recurrences are independent dependency chains (instruction-level
 // parallelism); each accumulator still receives the images in order j = 0, 1, ...
-template <int L, int M, int NI>
+// orders below LC only (LC <= L: the blocked sweep's truncation buckets)
+template <int L, int M, int NI, int LC = L>
 __device__ __forceinline__ void sweep_column(const double* zr, const double* ir2, const Cplx* W, double* ar,
-                                             double* ai, int lc = L) {
+                                             double* ai) {
+    constexpr int LT = LC < L ? LC : L;
     double e2[NI], e1[NI];
 #pragma unroll
     for (int j = 0; j < NI; j++) {
@@ -259,24 +261,19 @@ __device__ __forceinline__ void sweep_column(const double* zr, const double* ir2
         ar[0] += W[j].re;
         ai[0] += W[j].im;
     }
-    // orders >= lc are negligible (lc wave-uniform).  A guard, not an early exit: the
-    // loop keeps its constant trip count, so it is unrolled and the accumulators stay
-    // in registers
 #pragma unroll
-    for (int k = 1; k < L - M; k++) {
+    for (int k = 1; k < LT - M; k++) {
         const int l = M + k;
-        if (l < lc) {
 #pragma unroll
-            for (int j = 0; j < NI; j++) {
-                const double e = fma(kRec2.a2[l][M] * zr[j], e1[j], -ir2[j] * e2[j]);
-                e2[j] = e1[j];
-                e1[j] = e;
-            }
+        for (int j = 0; j < NI; j++) {
+            const double e = fma(kRec2.a2[l][M] * zr[j], e1[j], -ir2[j] * e2[j]);
+            e2[j] = e1[j];
+            e1[j] = e;
+        }
 #pragma unroll
-            for (int j = 0; j < NI; j++) {
-                ar[k] = fma(e1[j], W[j].re, ar[k]);
-                ai[k] = fma(e1[j], W[j].im, ai[k]);
-            }
+        for (int j = 0; j < NI; j++) {
+            ar[k] = fma(e1[j], W[j].re, ar[k]);
+            ai[k] = fma(e1[j], W[j].im, ai[k]);
         }
     }
 }
@@ -295,77 +292,72 @@ __device__ __forceinline__ void sweep_store_column(double2* le, const double* ar
 
 // columns (P, L-1-P) of pair P, NI images, into accumulator row ar/ai (first L-P entries: column P)
 // column M with weight base * s_M * U (U = conj((x+iy)/r^2)^M) into the accumulator row
-template <int L, int M, int NI>
+template <int L, int M, int NI, int LC = L>
 __device__ __forceinline__ void sweep_weighted_column(const double* zr, const double* ir2, const Cplx* U,
-                                                      const double* base, double* ar, double* ai, int lc = L) {
+                                                      const double* base, double* ar, double* ai) {
     Cplx W[NI];
 #pragma unroll
     for (int j = 0; j < NI; j++) {
         const double f = base[j] * kRec.s[M];
         W[j] = Cplx{f * U[j].re, f * U[j].im};
     }
-    if (M < lc) sweep_column<L, M, NI>(zr, ir2, W, ar, ai, lc);
+    if constexpr (M < LC) sweep_column<L, M, NI, LC>(zr, ir2, W, ar, ai);
 }
 // low columns P0 + J, J = 0 .. NPE-1 (ascending powers: one multiply per column)
-template <int L, int P0, int NPE, int NI, int J = 0>
+template <int L, int P0, int NPE, int NI, int LC, int J = 0>
 __device__ __forceinline__ void sweep_low_cols(const double* zr, const double* ir2, const Cplx* u, Cplx* U,
-                                               const double* base, double (*ar)[L + 1], double (*ai)[L + 1],
-                                               int lc = L) {
+                                               const double* base, double (*ar)[L + 1], double (*ai)[L + 1]) {
     if constexpr (J < NPE) {
         if constexpr (J > 0) {
 #pragma unroll
             for (int j = 0; j < NI; j++) U[j] = cmul(U[j], u[j]);
         }
-        sweep_weighted_column<L, P0 + J, NI>(zr, ir2, U, base, ar[J], ai[J], lc);
-        sweep_low_cols<L, P0, NPE, NI, J + 1>(zr, ir2, u, U, base, ar, ai, lc);
+        sweep_weighted_column<L, P0 + J, NI, LC>(zr, ir2, U, base, ar[J], ai[J]);
+        sweep_low_cols<L, P0, NPE, NI, LC, J + 1>(zr, ir2, u, U, base, ar, ai);
     }
 }
 // high columns L-1-P0-J for J = NPE-1 down to 0 (ascending powers), skipping the
 // middle column that equals its low partner when L is odd
-template <int L, int P0, int NPE, int NI, int J>
+template <int L, int P0, int NPE, int NI, int LC, int J>
 __device__ __forceinline__ void sweep_high_cols(const double* zr, const double* ir2, const Cplx* u, Cplx* U,
-                                                const double* base, double (*ar)[L + 1], double (*ai)[L + 1],
-                                                int lc = L) {
+                                                const double* base, double (*ar)[L + 1], double (*ai)[L + 1]) {
     if constexpr (J >= 0) {
         constexpr int M1 = P0 + J, M2 = L - 1 - P0 - J;
         if constexpr (M2 != M1)
-            sweep_weighted_column<L, M2, NI>(zr, ir2, U, base, ar[J] + (L - M1), ai[J] + (L - M1), lc);
+            sweep_weighted_column<L, M2, NI, LC>(zr, ir2, U, base, ar[J] + (L - M1), ai[J] + (L - M1));
         if constexpr (J > 0) {
 #pragma unroll
             for (int j = 0; j < NI; j++) U[j] = cmul(U[j], u[j]);
         }
-        sweep_high_cols<L, P0, NPE, NI, J - 1>(zr, ir2, u, U, base, ar, ai, lc);
+        sweep_high_cols<L, P0, NPE, NI, LC, J - 1>(zr, ir2, u, U, base, ar, ai);
     }
 }
 // the column pairs (P, L-1-P), P = P0 .. P0+NP-1, of one lane: the powers of u run up
 // through the low columns and on (one cpow) through the high ones
-// lc: orders l >= lc are skipped (wave-uniform; L: none).  Columns ascend within the
-// low and within the high part, so a column at or above lc ends its part.
-template <int L, int P0, int NP, int NI>
+// LC: orders l >= LC are left out (the blocked sweep's truncation buckets; L: none).
+// Columns ascend within the low and within the high part, so a column at or above LC
+// ends its part.
+template <int L, int P0, int NP, int NI, int LC = L>
 __device__ __forceinline__ void sweep_group_cols(const double* zr, const double* ir2, const Cplx* u,
-                                                 const double* base, double (*ar)[L + 1], double (*ai)[L + 1],
-                                                 int lc = L) {
+                                                 const double* base, double (*ar)[L + 1], double (*ai)[L + 1]) {
     constexpr int NPAIR = (L + 1) / 2;
     constexpr int NPE = (P0 + NP <= NPAIR) ? NP : NPAIR - P0;   // pairs in this group
-    if constexpr (NPE > 0) {
-        if (P0 >= lc) return;
+    if constexpr (NPE > 0 && P0 < LC) {
         Cplx U[NI];
 #pragma unroll
         for (int j = 0; j < NI; j++) U[j] = cpow<P0>(u[j]);
-        sweep_low_cols<L, P0, NPE, NI>(zr, ir2, u, U, base, ar, ai, lc);
+        sweep_low_cols<L, P0, NPE, NI, LC>(zr, ir2, u, U, base, ar, ai);
         constexpr int LOW_TOP = P0 + NPE - 1;          // power held in U now
         constexpr int HIGH_BOTTOM = L - 1 - LOW_TOP;    // lowest high column
-        if constexpr (HIGH_BOTTOM > LOW_TOP) {
-            if (HIGH_BOTTOM >= lc) return;
+        if constexpr (HIGH_BOTTOM > LOW_TOP && HIGH_BOTTOM < LC) {
 #pragma unroll
             for (int j = 0; j < NI; j++) U[j] = cmul(U[j], cpow<HIGH_BOTTOM - LOW_TOP>(u[j]));
-            sweep_high_cols<L, P0, NPE, NI, NPE - 1>(zr, ir2, u, U, base, ar, ai, lc);
-        } else if constexpr (HIGH_BOTTOM == LOW_TOP && NPE > 1) {
+            sweep_high_cols<L, P0, NPE, NI, LC, NPE - 1>(zr, ir2, u, U, base, ar, ai);
+        } else if constexpr (HIGH_BOTTOM == LOW_TOP && NPE > 1 && HIGH_BOTTOM + 1 < LC) {
             // L odd: the last low column is the middle one; the high columns start one above
-            if (HIGH_BOTTOM + 1 >= lc) return;
 #pragma unroll
             for (int j = 0; j < NI; j++) U[j] = cmul(U[j], u[j]);
-            sweep_high_cols<L, P0, NPE, NI, NPE - 2>(zr, ir2, u, U, base, ar, ai, lc);
+            sweep_high_cols<L, P0, NPE, NI, LC, NPE - 2>(zr, ir2, u, U, base, ar, ai);
         }
     }
 }
@@ -397,10 +389,9 @@ __device__ __forceinline__ void sweep_images(const double* dx, double dy, double
     }
     sweep_group_cols<L, P0, NP, NI>(zr, ir2, u, base, ar, ai);
 }
-template <int L, int P0, int NP, int NI>
+template <int L, int P0, int NP, int NI, int LC = L>
 __device__ __forceinline__ void sweep_images_xyz(const double* dx, const double* dy, const double* dz,
-                                                 const double* qe, double (*ar)[L + 1], double (*ai)[L + 1],
-                                                 int lc = L) {
+                                                 const double* qe, double (*ar)[L + 1], double (*ai)[L + 1]) {
     double zr[NI], ir2[NI], base[NI];
     Cplx u[NI];
 #pragma unroll
@@ -413,7 +404,7 @@ __device__ __forceinline__ void sweep_images_xyz(const double* dx, const double*
         u[j] = Cplx{dx[j] * ir2[j], -dy[j] * ir2[j]};
         base[j] = qe[j] * ir;
     }
-    sweep_group_cols<L, P0, NP, NI>(zr, ir2, u, base, ar, ai, lc);
+    sweep_group_cols<L, P0, NP, NI, LC>(zr, ir2, u, base, ar, ai);
 }
 
 __device__ __forceinline__ bool near1(int d) { return d < 2 && d > -2; }
@@ -708,14 +699,82 @@ __device__ __forceinline__ int sweep_trunc_order(double D, double rho, int L) {
     return lc;
 }
 
+// truncation buckets of the blocked sweep: orders below 8, 10, 12, 14, then all L
+constexpr int SWEEP_NBUCKET = 5;
+__host__ __device__ constexpr int sweep_bucket(int i, int L) {
+    return i < SWEEP_NBUCKET - 1 ? (8 + 2 * i < L ? 8 + 2 * i : L) : L;
+}
+
+// Column pairs of wave G in the blocked sweep: pair G and pair NPAIR-1-G (one pair when
+// they coincide).  With every order kept each wave has 2(L+1) coefficients; with the
+// orders truncated at LC the low columns of the low pairs stay and the high ones go, so
+// the pairing from both ends keeps the waves' shares within a few coefficients of each
+// other (consecutive pairs would leave the waves of high columns idle for far images).
+template <int L, int G>
+struct BlockPairs {
+    static constexpr int NPAIR = (L + 1) / 2;
+    static constexpr int A = G;
+    static constexpr int B = NPAIR - 1 - G;   // == A: one pair
+};
+// one source image for the lane's leaf into wave G's accumulators (row 0: pair A, row 1: pair B)
+template <int L, int G, int LC>
+__device__ __forceinline__ void block_image(double dx, double dy, double dz, double qe, double (*ar)[L + 1],
+                                            double (*ai)[L + 1]) {
+    using PP = BlockPairs<L, G>;
+    double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    r2 = qe != 0.0 ? r2 : 1.0;
+    const double ir = rsqrt_fp64(r2);
+    const double ir2 = ir * ir;
+    const double zr = dz * ir2;
+    const Cplx u = {dx * ir2, -dy * ir2};     // conj((x+iy)/r^2)
+    const double base = qe * ir;
+    sweep_group_cols<L, PP::A, 1, 1, LC>(&zr, &ir2, &u, &base, ar, ai);
+    if constexpr (PP::B > PP::A) sweep_group_cols<L, PP::B, 1, 1, LC>(&zr, &ir2, &u, &base, ar + 1, ai + 1);
+}
+
+// the images of bucket I (truncation order lc in (bucket I-1, bucket I]) of both sources,
+// in stencil order, for the lane's leaf
+template <int L, int G, bool PER, int I>
+__device__ __forceinline__ void sweep_block_images(const SweepArgs& a, const double* dx0, const double* dy0,
+                                                   const double* dz0, const double* sq, const int* dcx,
+                                                   const int* dcy, const int* dcz, const int* lcs,
+                                                   double (*ar)[L + 1], double (*ai)[L + 1]) {
+    constexpr int LC = sweep_bucket(I, L);
+    constexpr int LPREV = I > 0 ? sweep_bucket(I - 1, L) : 0;
+    if constexpr (I < SWEEP_NBUCKET && (I == 0 || LPREV < L)) {
+        if constexpr (G < LC) {
+            constexpr int R = PER ? 1 : 0;
+            constexpr int NIMG = PER ? 27 : 1;
+            const int n = a.g.ncps;
+            const double E = a.g.E;
+#pragma unroll 1
+            for (int k = 0; k < NIMG; k++) {
+                const int lc = __builtin_amdgcn_readfirstlane(lcs[k]);
+                // images of this bucket that reach this wave's columns (its lowest is G)
+                if (lc <= G || lc > LC || (I > 0 && lc <= LPREV)) continue;
+                const int ox = k % (2 * R + 1) - R, oy = (k / (2 * R + 1)) % (2 * R + 1) - R,
+                          oz = k / ((2 * R + 1) * (2 * R + 1)) - R;
+#pragma unroll
+                for (int s = 0; s < 2; s++) {
+                    const bool nr = near1(dcx[s] + ox * n) && near1(dcy[s] + oy * n) && near1(dcz[s] + oz * n);
+                    block_image<L, G, LC>(fma((double)ox, E, dx0[s]), fma((double)oy, E, dy0[s]),
+                                          fma((double)oz, E, dz0[s]), nr ? 0.0 : sq[s], ar, ai);
+                }
+            }
+        }
+        sweep_block_images<L, G, PER, I + 1>(a, dx0, dy0, dz0, sq, dcx, dcy, dcz, lcs, ar, ai);
+    }
+}
+
 // group G (column pairs G*NP ..) of the move's 2 x nimg source images for the leaf
-// (tx, ty, tz) of this lane; lcs[k] = truncation order of image k (both sources)
+// (tx, ty, tz) of this lane; lcs[k] = truncation order of image k (both sources).  The
+// images go bucket by bucket (each bucket's loop a compile-time truncation: unrolled,
+// branch-free columns), in stencil order within a bucket.
 template <int L, int G, int NP, int NI, bool PER>
 __device__ __forceinline__ void sweep_block_group(const SweepArgs& a, int tx, int ty, int tz, const int* lcs,
                                                   double (*ar)[L + 1], double (*ai)[L + 1]) {
-    constexpr int P0 = G * NP;
+    static_assert(NI == 1 && NP == 2, "the blocked sweep: one source image per pass, two column pairs per lane");
     const int n = a.g.ncps;
-    const double E = a.g.E;
     const double cx = a.g.c0 + tx * a.g.w;
     const double cy = a.g.c0 + ty * a.g.w;
     const double cz = a.g.c0 + tz * a.g.w;
@@ -730,43 +789,13 @@ __device__ __forceinline__ void sweep_block_group(const SweepArgs& a, int tx, in
         dx0[s] = p[0] - cx; dy0[s] = p[1] - cy; dz0[s] = p[2] - cz;
         dcx[s] = sc % n - tx; dcy[s] = (sc / n) % n - ty; dcz[s] = sc / (n * n) - tz;
     }
-    constexpr int R = PER ? 1 : 0;
-#pragma unroll 1
-    for (int oz = -R; oz <= R; oz++) {
-#pragma unroll 1
-        for (int oy = -R; oy <= R; oy++) {
-#pragma unroll 1
-            for (int ox = -R; ox <= R; ox++) {
-                const int lc = __builtin_amdgcn_readfirstlane(lcs[(ox + R) + (2 * R + 1) * ((oy + R) + (2 * R + 1) * (oz + R))]);
-                if (lc <= P0) continue;    // every column of this group at or above lc
-                double dx[2], dy[2], dz[2], qe[2];
-#pragma unroll
-                for (int s = 0; s < 2; s++) {
-                    dx[s] = fma((double)ox, E, dx0[s]);
-                    dy[s] = fma((double)oy, E, dy0[s]);
-                    dz[s] = fma((double)oz, E, dz0[s]);
-                    const bool nr = near1(dcx[s] + ox * n) && near1(dcy[s] + oy * n) && near1(dcz[s] + oz * n);
-                    qe[s] = nr ? 0.0 : sq[s];
-                }
-                if constexpr (NI == 2) {
-                    sweep_images_xyz<L, P0, NP, 2>(dx, dy, dz, qe, ar, ai, lc);
-                } else {
-#pragma unroll
-                    for (int s = 0; s < 2; s++)
-                        sweep_images_xyz<L, P0, NP, 1>(dx + s, dy + s, dz + s, qe + s, ar, ai, lc);
-                }
-            }
-        }
-    }
+    sweep_block_images<L, G, PER, 0>(a, dx0, dy0, dz0, sq, dcx, dcy, dcz, lcs, ar, ai);
 }
 // one wave's share of the blocked sweep: group G's columns of its lane's leaf, then the
-// staged, coalesced update of the block's expansions together with the other waves.
-// Not inlined: each group's accumulators stay in registers of its own function (inlined
-// into one kernel, identical code of the groups was merged and the accumulators were
-// demoted to scratch memory).
+// staged, coalesced update of the block's expansions together with the other waves
 template <int L, int G, int NP, int NI, bool PER>
-__device__ __forceinline__ void sweep_block_wave(const SweepArgs& a, int tx, int ty, int tz,
-                                                           const int* lcs, double2* stage, int cell_base) {
+__device__ __forceinline__ void sweep_block_wave(const SweepArgs& a, int tx, int ty, int tz, const int* lcs,
+                                                 double2* stage, int cell_base) {
     constexpr int W = L + 1;
     constexpr int NH = nhalf(L);
     constexpr int ST = stage_stride(NH);
@@ -777,14 +806,19 @@ __device__ __forceinline__ void sweep_block_wave(const SweepArgs& a, int tx, int
     for (int j = 0; j < NP; j++)
 #pragma unroll
         for (int k = 0; k < W; k++) { ar[j][k] = 0.0; ai[j][k] = 0.0; }
+    using PP = BlockPairs<L, G>;
     sweep_block_group<L, G, NP, NI, PER>(a, tx, ty, tz, lcs, ar, ai);
-    sweep_group_stage<L, G * NP, NP, true>(nullptr, ar, ai);
+    sweep_group_stage<L, PP::A, 1, true>(nullptr, ar, ai);
+    if constexpr (PP::B > PP::A) sweep_group_stage<L, PP::B, 1, true>(nullptr, ar + 1, ai + 1);
     const int tid = threadIdx.x + 64 * threadIdx.y;
     const int nthr = 64 * blockDim.y;
 #pragma unroll 1
     for (int half = 0; half < 2; half++) {
         __syncthreads();
-        if ((lane >> 5) == half) sweep_group_stage<L, G * NP, NP, false>(stage + (lane & 31) * ST, ar, ai);
+        if ((lane >> 5) == half) {
+            sweep_group_stage<L, PP::A, 1, false>(stage + (lane & 31) * ST, ar, ai);
+            if constexpr (PP::B > PP::A) sweep_group_stage<L, PP::B, 1, false>(stage + (lane & 31) * ST, ar + 1, ai + 1);
+        }
         __syncthreads();
         // the half's 32 leaves: 8 runs (y, z) of 4 consecutive leaves along x
         constexpr int U = 8;
@@ -1218,10 +1252,13 @@ __device__ __forceinline__ void add_parts(CSum& acc, double sign, const double* 
     for (int j = 0; j < nparts; j++) acc.add(sign * v[(size_t)j * stride + i]);
 }
 
+// Metropolis (rule 0) or exponential (rule 1) rate of an energy change.  An exponential
+// rate beyond 1e300 (dE below -690 kT) is held at 1e300: it still dominates the
+// selection, and a sum of up to ~1e8 such rates stays finite.
 __device__ __forceinline__ double rate_of(double de, double kT, int rule) {
     if (!(de == de) || isinf(de)) return 0.0;
     const double w = exp(-de / kT);
-    if (rule == 1) return isinf(w) ? 0.0 : w;
+    if (rule == 1) return w < 1e300 ? w : 1e300;
     return w < 1.0 ? w : 1.0;
 }
 
@@ -2098,10 +2135,9 @@ template <int L>
 static void launch_sweep_block(SweepArgs a, hipStream_t s) {
     constexpr int ng = ((L + 1) / 2 + 1) / 2;
     const int nb = a.g.ncps / 4;
-    if (a.g.bc == KMC_BC_FREE_SPACE)
-        hipLaunchKernelGGL((k_sweep_block<L, 2, 1, false>), dim3(nb * nb * nb), dim3(64, ng), 0, s, a);
-    else
-        hipLaunchKernelGGL((k_sweep_block<L, 2, 1, true>), dim3(nb * nb * nb), dim3(64, ng), 0, s, a);
+    const dim3 grid(nb * nb * nb), block(64, ng);
+    if (a.g.bc == KMC_BC_FREE_SPACE) hipLaunchKernelGGL((k_sweep_block<L, 2, 1, false>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_sweep_block<L, 2, 1, true>), grid, block, 0, s, a);
 }
 static bool sweep_block_fits(const SweepArgs& a, int L) {
     return a.mode == 0 && a.g.ncps % 4 == 0 && 64 * (((L + 1) / 2 + 1) / 2) <= 512;

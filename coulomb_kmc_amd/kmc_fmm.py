@@ -287,7 +287,8 @@ class KMCFMM(InjectorExtractor):
             d_pq = self._dev(_as_array(prop_charges, 1, np.float64)[:n], torch.float64)
         tm = ctypes.c_int64(0)
         _native.check(self._lib.kmc_propose_with_dats(
-            self._ctx, self._stream(), M, d_smc.data_ptr(), d_sites.data_ptr(), d_ppos.data_ptr(), d_masks.data_ptr(),
+            self._ctx, self._stream(), M, d_smc.data_ptr(), smc.size, d_sites.data_ptr(), d_ppos.data_ptr(),
+            d_masks.data_ptr(),
             None if d_pq is None else d_pq.data_ptr(), d_out.data_ptr(), _native.PREC[self.precision],
             ctypes.byref(tm)), "kmc_propose_with_dats")
         diffs[:n] = d_out.cpu().numpy()

@@ -3,7 +3,9 @@ Tile autotuner for the propose/accept kernels.
 
 Tunable shapes (set on a context with KMCFMM.set_tiles / kmc_set_tiles):
   near_lanes   lanes per evaluation point in the near-field kernel (1, 4, 8, 16, 32, 64;
-               27 = one lane per neighbour leaf)
+               27 = one lane per neighbour leaf; 0 = a moving charge's points together;
+               -G = points sorted by 4x4x4 leaf tile, each tile's charges staged in LDS
+               once, G lanes per point)
   sweep_pairs  column pairs (m, L-1-m) per lane of the accept sweep (1, 2)
   sweep_block  leaves (threads) per workgroup of the accept sweep (64, 128, 192, 256), or
                0: fused layout, 64 leaves x all column groups per workgroup, with the
@@ -26,7 +28,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(_HERE, "tiles.json")
 
-NEAR_LANES = (0, 1, 4, 8, 16, 27, 32, 64)
+NEAR_LANES = (0, 1, 4, 8, 16, 27, 32, 64, -4, -8)
 # (sweep_pairs, sweep_block, sweep_split)
 SWEEP_SHAPES = ((2, 1, 1), (1, 0, 1), (2, 0, 1), (1, 64, 1), (1, 256, 1), (2, 64, 1), (2, 256, 1), (1, 64, 3),
                 (2, 64, 3))

@@ -143,11 +143,11 @@ int kmc_propose_noalloc(kmc_ctx* ctx, void* stream, int64_t P, int64_t T, const 
  * current_sites[n], prop_positions[n][max_prop][3], prop_masks[n][max_prop],
  * prop_charges[n][max_prop] or NULL; writes prop_energy_diffs[n][max_prop] at every
  * masked slot taken (mask > 0, at most site_max_counts[site] per charge).
- * *h_total_moves (optional) receives the number of proposals evaluated.
- * Every current_sites[i] must index d_site_max_counts (not checked on the device).
+ * *h_total_moves (optional) receives the number of proposals evaluated.  A
+ * current_sites[i] outside [0, nsites) returns KMC_E_ARG (checked on the device).
  */
 int kmc_propose_with_dats(kmc_ctx* ctx, void* stream, int64_t max_prop, const int64_t* d_site_max_counts,
-                          const int64_t* d_current_sites, const double* d_prop_positions,
+                          int64_t nsites, const int64_t* d_current_sites, const double* d_prop_positions,
                           const int64_t* d_prop_masks, const double* d_prop_charges,
                           double* d_prop_energy_diffs, int32_t precision, int64_t* h_total_moves);
 
@@ -338,6 +338,9 @@ int kmc_check_flags(kmc_ctx* ctx, void* stream, int32_t* h_flags);
  * block by less than 2^-60 per unit charge; accepts only, leaf grids with a multiple of
  * 4 leaves per side, else as 0), sweep_split = waves sharing one leaf's 27 images (1,
  * or 3: one image layer each, summed in LDS; workgroups of 64 leaves x 3; '27'/PBC only).
+ * near_lanes -4 / -8 / -16: points counting-sorted by 4x4x4 leaf tile, each tile's 6^3
+ * leaves of charges staged in LDS once, G lanes per point (propose paths; grids with a
+ * multiple of 4 leaves per side, else G lanes per point from memory).
  */
 int kmc_set_tiles(kmc_ctx* ctx, int32_t near_lanes, int32_t sweep_block, int32_t sweep_pairs,
                   int32_t sweep_split);

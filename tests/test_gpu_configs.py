@@ -1,6 +1,6 @@
 """
-The five benchmark configurations at their full sizes, on the device step path the
-benchmark times, against the oracle:
+The five benchmark configurations at their full sizes, on the device step path and
+with the tile choices the benchmark uses, against the oracle:
 
 cfg0  free space, N=1000, box 10, 8^3 leaves, L=12, 64 x 4 proposals (ball 1.0,
       clipped), Metropolis at kT=1, 100 steps: every step's dE against the oracle's
@@ -84,6 +84,12 @@ class DeviceStream:
         return npos, self.d_dE.cpu().numpy().copy()
 
 
+def _bench_tiles(kmc, L, T, ncps):
+    """The tile choices bench.py uses for this class (coulomb_kmc_amd/tiles.json)."""
+    from coulomb_kmc_amd import tune
+    kmc.set_tiles(**tune.choose_tiles(L, T, ncps ** 3))
+
+
 def _store_equal(kmc, orc, cells=None):
     occ_g, ids_g, p4_g = kmc.get_store()
     occ_c, ids_c, data_c = orc.get_store()
@@ -126,6 +132,7 @@ def test_cfg0_free_space_1k_100_steps():
     pos, q = neutral_system(rng, N, E)
     pids, disp, us = move_stream(rng, N, P, k, R, nsteps)
     kmc = KMCFMM(pos.copy(), q.reshape(-1, 1), E, r=r, l=L, boundary_condition="free_space", max_move=R)
+    _bench_tiles(kmc, L, P * k, 2 ** (r - 1))
     kmc.initialise()
     orc = O.Oracle(pos, q, E, r, L, bc="free_space", max_move=R)
     orc.initialise()
@@ -151,6 +158,7 @@ def pbc_100k():
     rng = np.random.default_rng(1001)
     pos, q = neutral_system(rng, N, E)
     kmc = KMCFMM(pos.copy(), q.reshape(-1, 1), E, r=r, l=L, boundary_condition="pbc", max_move=R)
+    _bench_tiles(kmc, L, 1024 * 8, 2 ** (r - 1))
     kmc.initialise()
     orc = O.Oracle(pos, q, E, r, L, bc="pbc", max_move=R, lr=lr_operator_reference_layout(L, E))
     orc.init_from_expansions(kmc.get_expansions(), kmc.energy)
@@ -270,6 +278,7 @@ def test_cfg4_pbc_2m():
     rng = np.random.default_rng(1004)
     pos, q = neutral_system(rng, N, E)
     kmc = KMCFMM(pos.copy(), q.reshape(-1, 1), E, r=r, l=L, boundary_condition="pbc", max_move=R)
+    _bench_tiles(kmc, L, 32768 * 8, 2 ** (r - 1))
     kmc.initialise()
     orc = O.Oracle(pos, q, E, r, L, bc="pbc", max_move=R, lr=lr_operator_reference_layout(L, E))
     orc.init_from_expansions(kmc.get_expansions(), kmc.energy)

@@ -434,3 +434,108 @@ def test_blocked_sweep_truncation_matches_oracle(bc):
     _print("blocked sweep", bc, "dE vs extended", rel_err(g, ext).max())
     assert rel_err(g, ext).max() <= O.DE_BAR_FP64
     kmc.free()
+
+
+@pytest.mark.parametrize("bc", BCS)
+@pytest.mark.parametrize("r,N,E", [(2, 300, 6.0), (3, 300, 6.0), (4, 600, 8.0), (3, 1500, 6.0)])
+def test_tiled_near_field(bc, r, N, E):
+    """The leaf-tiled near field (near_lanes -G: points counting-sorted by 4x4x4 leaf
+    tile, one workgroup per tile with the tile's 6^3 leaves staged in LDS) against the
+    per-point kernel with the same lane count and against the oracle: a 2-leaf grid (not
+    tiled: the per-point kernel runs), a 4-leaf grid (one tile whose halo wraps onto
+    itself), an 8-leaf grid, and 1500 charges on 4^3 leaves (more records than the LDS
+    holds: the tile is evaluated from memory)."""
+    kmc, orc, pos, q, rng = make_pair(bc, N=N, E=E, r=r, L=8, max_move=1.0, seed=60 + r)
+    pids = rng.choice(len(q), 40, replace=False)
+    moves = ball_moves(rng, pos, pids, 5, 1.0, E, bc)
+    for G in (4, 8):
+        for prec, tol in (("fp64", 1e-10), ("mixed", 1e-8)):
+            kmc.precision = prec
+            kmc.set_tiles(near_lanes=G)
+            ref = np.concatenate(kmc.propose(moves)) - kmc.energy
+            kmc.set_tiles(near_lanes=-G)
+            g = np.concatenate(kmc.propose(moves)) - kmc.energy
+            c = np.concatenate(orc.propose(moves)) - orc.energy
+            _print("tiled near", bc, r, N, G, prec, rel_err(g, ref).max(), rel_err(g, c).max())
+            assert rel_err(g, ref).max() <= 1e-11
+            assert rel_err(g, c).max() <= tol
+    kmc.free()
+
+
+def test_propose_with_dats_many_scan_blocks_and_bad_site():
+    """propose_with_dats over 5000 charges (the mask compaction's scan spans several
+    workgroups) equals propose on the same moves; a current site outside
+    site_max_counts is refused on the device (KMC_E_ARG) through the C ABI."""
+    import ctypes
+    import torch
+    from coulomb_kmc_amd import _native
+    kmc, orc, pos, q, rng = make_pair("pbc", N=5000, E=14.0, r=4, L=8, max_move=1.0, seed=91)
+    N, M = len(q), 4
+    site_max = np.array([1, 2, 4], dtype=np.int64)
+    sites = rng.integers(0, 3, size=(N, 1)).astype(np.int64)
+    masks = (rng.uniform(size=(N, M)) < 0.5).astype(np.int64)
+    ppos = np.zeros((N, 3 * M))
+    for i in range(N):
+        ppos[i] = ball_moves(rng, pos, [i], M, 1.0, 14.0, "pbc")[0][1].reshape(-1)
+    dg = np.full((N, M), -7.0)
+    kmc.propose_with_dats(site_max, sites, ppos, masks, dg)
+    moves = []
+    for i in range(N):
+        taken = np.nonzero(masks[i])[0][: site_max[sites[i, 0]]]
+        if len(taken):
+            moves.append((i, ppos[i].reshape(M, 3)[taken]))
+    e = kmc.propose(tuple(moves))
+    for (i, mv), vals in zip(moves, e):
+        taken = np.nonzero(masks[i])[0][: site_max[sites[i, 0]]]
+        assert np.allclose(vals - kmc.energy, dg[i, taken], rtol=1e-12, atol=1e-12)
+    dev = lambda a, t: torch.as_tensor(np.ascontiguousarray(a), dtype=t).cuda()
+    bad = sites.copy()
+    bad[17, 0] = 3
+    d_smc, d_sites = dev(site_max, torch.int64), dev(bad.ravel(), torch.int64)
+    d_ppos, d_masks = dev(ppos, torch.float64), dev(masks, torch.int64)
+    d_out = torch.zeros((N, M), dtype=torch.float64, device="cuda")
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    rc = _native.lib().kmc_propose_with_dats(kmc.handle, st, M, d_smc.data_ptr(), 3, d_sites.data_ptr(),
+                                             d_ppos.data_ptr(), d_masks.data_ptr(), None, d_out.data_ptr(), 0, None)
+    assert rc == _native.KMC_E_ARG
+    # the error counter is back at zero: the next call works
+    rc = _native.lib().kmc_propose_with_dats(kmc.handle, st, M, d_smc.data_ptr(), 3,
+                                             dev(sites.ravel(), torch.int64).data_ptr(), d_ppos.data_ptr(),
+                                             d_masks.data_ptr(), None, d_out.data_ptr(), 0, None)
+    assert rc == 0
+    kmc.free()
+
+
+def test_device_step_refuses_bad_move():
+    """A device step whose proposals include one beyond max_move accepts nothing and
+    raises flag 32; the next step with good moves goes on as usual."""
+    import torch
+    kmc, orc, pos, q, rng = make_pair("pbc", N=300, E=6.0, r=4, L=8, max_move=0.5, seed=92)
+    P, k = 8, 2
+    pids = rng.choice(len(q), P, replace=False)
+    good = ball_moves(rng, pos, pids, k, 0.4, 6.0, "pbc")
+    npos = np.concatenate([m[1] for m in good])
+    dev = lambda a, t: torch.as_tensor(np.ascontiguousarray(a), dtype=t).cuda()
+    d_pid, d_es = dev(pids, torch.int64), dev(np.arange(P + 1) * k, torch.int64)
+    d_dE = torch.zeros(P * k, dtype=torch.float64, device="cuda")
+    us = dev(np.array([0.3, 0.6]), torch.float64)
+    counter = torch.zeros(1, dtype=torch.int64, device="cuda")
+    sel = torch.full((2,), -7, dtype=torch.int64, device="cuda")
+    kmc.propose_device(d_pid, d_es, dev(npos, torch.float64), d_dE)   # sizes the buffers
+    e0 = kmc.device_energy()
+    p0, _ = kmc.get_device_particles()
+    bad = npos.copy()
+    bad[3] = pos[pids[1]] + np.array([0.9, 0.0, 0.0])                 # beyond max_move 0.5
+    bad[3] = np.where(bad[3] >= 3.0, bad[3] - 6.0, bad[3])
+    kmc.step_device(d_pid, d_es, dev(bad, torch.float64), d_dE, us, 1.0, counter, sel)
+    torch.cuda.synchronize()
+    assert kmc.check_flags() == 32
+    assert int(sel[0].item()) == -1
+    assert kmc.device_energy() == e0
+    p1, _ = kmc.get_device_particles()
+    assert np.array_equal(p0, p1)
+    kmc.step_device(d_pid, d_es, dev(npos, torch.float64), d_dE, us, 1.0, counter, sel)
+    torch.cuda.synchronize()
+    assert kmc.check_flags() == 0
+    assert int(sel[1].item()) >= 0
+    kmc.free()

@@ -78,20 +78,22 @@ def test_run_steps_equal_single_steps(bc):
                      w["es"], w["new"], w["dE"], pipelined=pipelined)
         _same(_state(kk, w), ref)
         kk.free()
-    # captured: 3 steps per graph, replayed twice with eager work between the replays
-    kk, w = make()
-    s = torch.cuda.Stream()
-    g = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(g, stream=s):
-        kk.run_steps(3, P, k, tabs["pids"], tabs["disp"], tabs["us"], 1.0, w["counter"], w["sel"], w["pid"],
-                     w["es"], w["new"], w["dE"], pipelined=True, stream=s.cuda_stream)
-    for _ in range(2):
-        g.replay()
-        torch.cuda.synchronize()
-        _ = w["counter"].cpu()
-    _same(_state(kk, w), ref)
-    del g
-    kk.free()
+    # captured (both pipelined modes): 3 steps per graph, replayed twice with eager work
+    # between the replays
+    for pipelined in (1, 2):
+        kk, w = make()
+        s = torch.cuda.Stream()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            kk.run_steps(3, P, k, tabs["pids"], tabs["disp"], tabs["us"], 1.0, w["counter"], w["sel"], w["pid"],
+                         w["es"], w["new"], w["dE"], pipelined=pipelined, stream=s.cuda_stream)
+        for _ in range(2):
+            g.replay()
+            torch.cuda.synchronize()
+            _ = w["counter"].cpu()
+        _same(_state(kk, w), ref)
+        del g
+        kk.free()
     ref_k.free()
 
 
