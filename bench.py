@@ -256,15 +256,18 @@ def traffic_from_pmc(workload, fetch_dir, write_dir):
     for which, d, counter in (("fetch", fetch_dir, "FETCH_SIZE"), ("write", write_dir, "WRITE_SIZE")):
         per = _per_dispatch(_pmc_rows(d, counter))
         # the last 5 sweeps are the timed accept sweeps (initialise also launches k_sweep)
-        sweep = [v for (n, i), v in sorted(per.items(), key=lambda kv: int(kv[0][1] or 0))
-                 if n.startswith("void k_sweep")][-5:]
+        sweep_rows = [(n, v) for (n, i), v in sorted(per.items(), key=lambda kv: int(kv[0][1] or 0))
+                      if n.startswith("void k_sweep")][-5:]
+        sweep = [v for _, v in sweep_rows]
+        kname = sweep_rows[-1][0][len("void "):].split("<")[0] if sweep_rows else "k_sweep"
         big = sorted(((v, n) for (n, _), v in per.items() if not n.startswith("void k_")), reverse=True)
         copies = [v for v, n in big if v * 1024 > 0.5 * COPY_PROBE_BYTES]
         if not sweep or not copies:
             raise SystemExit("missing k_sweep or copy probe dispatches under " + d)
         kib = float(np.mean(sweep))
         cal = COPY_PROBE_BYTES / (float(np.median(copies)) * 1024.0)
-        res[which] = {"counter": counter, "k_sweep_KiB_per_dispatch": kib, "dispatches": len(sweep),
+        res[which] = {"counter": counter, "kernel": sweep_rows[-1][0], "KiB_per_dispatch": kib,
+                      "dispatches": len(sweep),
                       "copy_probe_KiB": float(np.median(copies)), "scale": cal,
                       "bytes_per_launch": kib * 1024.0 * cal}
     path = os.path.join(ROOT, "profiles", "traffic.json")
@@ -272,7 +275,7 @@ def traffic_from_pmc(workload, fetch_dir, write_dir):
     if os.path.exists(path):
         with open(path) as f:
             tab = json.load(f)
-    key = "k_sweep|" + cfg["workload"]
+    key = kname + "|" + cfg["workload"]
     tab[key] = {"bytes_per_launch": res["fetch"]["bytes_per_launch"] + res["write"]["bytes_per_launch"],
                 "fetch": res["fetch"], "write": res["write"],
                 "copy_probe_bytes": COPY_PROBE_BYTES}
@@ -326,7 +329,7 @@ def run_batch(args):
         k = KMCFMM(x["pos"].copy(), x["q"].reshape(-1, 1), x["E"], r=x["r"], l=x["L"], boundary_condition=x["bc"],
                    max_move=x["radius"], precision="mixed", init_method="fmm")
         k.initialise()
-        k.set_tiles(near_lanes=near_lanes)
+        k.set_tiles(near_lanes=near_lanes, sweep_block=max(args.sweep_block, 0), sweep_pairs=2, sweep_split=1)
         return k
 
     near_lanes = args.near_lanes or spec["near_lanes"]
@@ -346,10 +349,14 @@ def run_batch(args):
     log("initialised", len(systems), "systems in", round(t_init, 1), "s")
     batch = KMCBatch(systems, kT=1.0, precision="mixed", streams=args.batch_streams, mode=args.batch_mode)
     tiles = {}
+    t0 = time.time()
     if args.batch_mode == "graph":
-        t0 = time.time()
         tiles = batch.autotune(lambda s: make(s.spec), reps=3, log=log)
-        log("autotune", round(time.time() - t0, 1), "s")
+    elif not args.no_tune:
+        choice, tuned = batch.autotune_kernels(log=log)
+        tiles = {"choice": choice, "ms_per_step": {"near_lanes {} sweep_block {}".format(*k): v
+                                                   for k, v in tuned.items()}}
+    log("autotune", round(time.time() - t0, 1), "s")
     batch.capture(G)
     torch.cuda.synchronize()
     for _ in range(warm // G):
@@ -364,6 +371,78 @@ def run_batch(args):
     ms = ev0.elapsed_time(ev1)
     flags = [f for f in batch.check_flags() if f]
     T = batch.proposals_per_step
+
+    # CPU baseline: the oracle on a stated subset of the systems (smallest, median and
+    # largest charge count, and the largest pbc system at L=16), each from its GPU state,
+    # whole steps (propose of its 2048 proposals + Metropolis pick + accept)
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        from coulomb_kmc_amd.kmc_fmm import lr_operator_reference_layout
+        nth = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        O.set_threads(nth)
+        order = np.argsort([x["N"] for x in specs])
+        l16 = [i for i in order if specs[i]["L"] == 16 and specs[i]["bc"] == "pbc"]
+        subset = sorted(set([int(order[0]), int(order[len(order) // 2]), int(order[-1])] + ([int(l16[-1])] if l16 else [])))
+        t_cpu, n_prop, n_acc = 0.0, 0, 0
+        for i in subset:
+            x, bs = specs[i], systems[i]
+            pos_g, _ = bs.kmc.get_device_particles()
+            orc = O.Oracle(pos_g, x["q"], x["E"], x["r"], x["L"], bc=x["bc"], max_move=x["radius"],
+                           lr=lr_operator_reference_layout(x["L"], x["E"]) if x["bc"] == "pbc" else None)
+            orc.init_from_expansions(bs.kmc.get_expansions(), bs.kmc.device_energy())
+            mrng = np.random.default_rng(100 + i)
+            pids, disp, us = move_stream(mrng, dict(N=x["N"], P=x["P"], k=x["k"], radius=x["radius"]), 2)
+            for st in range(2):
+                moves = host_moves(orc.positions(), pids[st], disp[st], x["k"], x["E"], x["bc"])
+                es = np.arange(x["P"] + 1, dtype=np.int64) * x["k"]
+                tc = time.time()
+                dc = orc.propose_arrays(pids[st], es, np.concatenate([m[1] for m in moves]))
+                kc = metropolis_pick(dc, us[st])
+                orc.accept(moves[kc // x["k"]][0], moves[kc // x["k"]][1][kc % x["k"]])
+                t_cpu += time.time() - tc
+                n_prop += x["P"] * x["k"]
+                n_acc += 1
+            del orc
+        cpu = {"value": n_prop / t_cpu, "unit": "proposals/s", "cores": nth, "kind": "port",
+               "accepts_per_s": n_acc / t_cpu,
+               "sample": "2 whole steps (2048 proposals + 1 accept) of each of {} systems ({}; charges {}), "
+                         "{:.1f} s, oracle/kmc_oracle.cpp (FP64, OpenMP); the CPU steps systems one after "
+                         "another".format(len(subset), subset, [specs[i]["N"] for i in subset], t_cpu)}
+        log("cpu baseline", cpu)
+
+    # roofline of the dominant kernel, the batch's accept sweep: every system's move
+    # record set to one hop of a charge (kmc_sweep_hop), then the batch's sweep launches
+    # (one per order / periodicity group) timed with HIP events (kmc_batch_time_sweep);
+    # bytes and flops of each system's sweep counted as for one system (sweep_work)
+    sweep_blocked = tiles.get("choice", {}).get("sweep_block", max(args.sweep_block, 0)) == 1
+    b_sw, f_sw = 0.0, 0.0
+    hop_rng = np.random.default_rng(spec["seed"] + 7)
+    for bs, x in zip(systems, specs):
+        ncps = 2 ** (x["r"] - 1)
+        E = x["E"]
+        old = hop_rng.uniform(-0.5 * E, 0.5 * E, 3)
+        new = old + np.array([0.5, -0.4, 0.3])
+        new = np.where(new < -0.5 * E, new + E, np.where(new >= 0.5 * E, new - E, new)) if x["bc"] == "pbc" \
+            else np.clip(new, -0.5 * E, 0.5 * E)
+        bs.kmc.sweep_hop(1.0, old, new)
+        nh = x["L"] * (x["L"] + 1) // 2
+        b_sw += 2.0 * ncps ** 3 * nh * 16
+        f_sw += sweep_work(E, ncps, x["L"], x["bc"] == "pbc", old, new, blocked=sweep_blocked)
+    torch.cuda.synchronize()
+    t_sw = batch.time_sweep(reps=10)
+    sw_gbs = b_sw / (t_sw * 1e-3) / 1e9
+    sw_tf = f_sw / (t_sw * 1e-3) / 1e12
+    roof = {"bound": "hbm", "kernel": "batched accept sweep ({}, one launch per order / periodicity group)".format(
+                "kb_sweep_block" if sweep_blocked else "kb_sweep_fused"),
+            "achieved": sw_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": sw_gbs / HBM_PEAK_GBS,
+            "traffic": None, "launch_ms": t_sw, "algorithmic_bytes": b_sw, "algorithmic_flops": f_sw,
+            "fp64_TFLOPs": sw_tf, "fp64_frac": sw_tf / FP64_PEAK_TFLOPS,
+            "note": "launch_ms: all groups' sweep launches of one batch step, back to back on one stream "
+                    "(kmc_batch_time_sweep, HIP events); bytes = read + write of every leaf's packed expansion "
+                    "of the 256 systems; flops as bench.sweep_work for one hop per system"}
+    log("batch sweep roofline", roof)
     out = {
         "metric": METRIC,
         "value": T * steps / (ms * 1e-3),
@@ -380,15 +459,17 @@ def run_batch(args):
         "config": {"workload": spec["workload"], "systems": len(systems), "charges": int(sum(x["N"] for x in specs)),
                    "proposals_per_step": T, "graph_steps": G, "streams": args.batch_streams,
                    "mode": args.batch_mode,
-                   "tiles": {"L{}_C{}_T{}".format(*k): v for k, v in tiles.items()} or
-                            "batched kernels: fused sweep, 2 column pairs per lane, near field " +
-                            ("one lane per neighbour leaf" if near_lanes in (0, 27) else
-                             "{} lanes per point".format(near_lanes))},
+                   "tiles": ({"L{}_C{}_T{}".format(*k): v for k, v in tiles.items()}
+                             if args.batch_mode == "graph" else tiles) or
+                            "batched kernels: near field {}, sweep_block as set".format(
+                                "one lane per neighbour leaf" if near_lanes in (0, 27) else
+                                "{} lanes per point".format(near_lanes))},
         "accepts_per_s": len(systems) * steps / (ms * 1e-3),
         "initialise_s": t_init,
         "flags": flags,
-        "roofline": None,
-        "cpu_baseline": None,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "gpu_vs_cpu": (T * steps / (ms * 1e-3)) / cpu["value"] if cpu else None,
     }
     print(json.dumps(out), flush=True)
     batch.free()
@@ -409,6 +490,7 @@ def main():
     ap.add_argument("--graph-steps", type=int, default=10, help="steps captured per HIP graph")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="batch: keep the given tiles (no autotuning)")
     ap.add_argument("--near-lanes", type=int, default=0, help="0: autotuned")
     ap.add_argument("--sweep-block", type=int, default=-1,
                     help="accept-sweep layout (kmc_set_tiles sweep_block; -1: autotuned): 0 fused, 1 blocked with "
@@ -700,18 +782,18 @@ def main():
                  "{:.3g})".format(ncells * nimg * sweep_flops_per_source(L)) if blocked else
                  "flops = leaves x {} images x (9 L(L+1)/2 + 6(L-1) + 30)".format(nimg))
     if hbm_bound:
-        roof = {"bound": "hbm", "kernel": "accept sweep ({})".format("k_sweep_block" if blocked else "k_sweep"),
+        roof = {"bound": "hbm", "kernel": "accept sweep ({})".format("k_sweep_block" if blocked else "k_sweep_fused"),
                 "achieved": ach_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach_gbs / HBM_PEAK_GBS,
                 "fp64_TFLOPs": ach_tflops, "fp64_frac": ach_tflops / FP64_PEAK_TFLOPS}
     else:
-        roof = {"bound": "valu", "kernel": "accept sweep ({})".format("k_sweep_block" if blocked else "k_sweep"),
+        roof = {"bound": "valu", "kernel": "accept sweep ({})".format("k_sweep_block" if blocked else "k_sweep_fused"),
                 "achieved": ach_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": ach_tflops / FP64_PEAK_TFLOPS, "hbm_GBs": ach_gbs, "hbm_frac": ach_gbs / HBM_PEAK_GBS}
     roof.update({"traffic": None, "launch_ms": kt["sweep"], "algorithmic_flops": sweep_flops,
                  "algorithmic_bytes": sweep_bytes, "flop_per_byte": ai,
                  "note": "no dense contraction: MFMA unused; bytes = read + write of every leaf's packed "
                          "expansion; " + flop_note})
-    traffic = tune.read_traffic(ROOT, "k_sweep", cfg)
+    traffic = tune.read_traffic(ROOT, "k_sweep_block" if blocked else "k_sweep_fused", cfg)
     if traffic is not None:
         roof["traffic"] = traffic
     peaks = measured_peaks()

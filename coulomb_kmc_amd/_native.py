@@ -43,9 +43,9 @@ EXPORTS = (
     "kmc_set_expansions", "kmc_get_multipole", "kmc_get_store", "kmc_store_occmax",
     "kmc_get_particles", "kmc_get_energy", "kmc_set_energy", "kmc_ncells", "kmc_propose",
     "kmc_propose_with_dats", "kmc_accept", "kmc_step", "kmc_materialize_moves",
-    "kmc_check_flags", "kmc_set_tiles", "kmc_sweep_hop", "kmc_time_kernel", "kmc_probe_peaks", "kmc_eval_field",
-    "kmc_batch_create", "kmc_batch_destroy", "kmc_batch_set_system", "kmc_batch_prepare", "kmc_batch_step", "kmc_batch_set_streams", "kmc_run_steps",
-    "kmc_particle_count", "kmc_set_particle_count", "kmc_charge_energies", "kmc_group_energy",
+    "kmc_check_flags", "kmc_store_reserve", "kmc_set_tiles", "kmc_sweep_hop", "kmc_time_kernel", "kmc_probe_peaks", "kmc_eval_field",
+    "kmc_batch_create", "kmc_batch_destroy", "kmc_batch_set_system", "kmc_batch_prepare", "kmc_batch_step", "kmc_batch_time_sweep", "kmc_batch_set_streams", "kmc_run_steps",
+    "kmc_particle_count", "kmc_set_particle_count", "kmc_charge_energies", "kmc_charge_potentials", "kmc_group_energy",
     "kmc_propose_extract", "kmc_propose_inject", "kmc_extract", "kmc_inject", "kmc_discover_sites",
     "kmc_lattice_step", "kmc_accept_movedata", "kmc_propose_noalloc",
 )
@@ -138,12 +138,14 @@ def lib():
         "kmc_step": (ctypes.c_int, [P, P, I64, I64, P, P, P, P, I64, P, D, P, P, I32]),
         "kmc_materialize_moves": (ctypes.c_int, [P, P, I64, I64, I64, P, P, P, P, P, P]),
         "kmc_check_flags": (ctypes.c_int, [P, P, P]),
+        "kmc_store_reserve": (ctypes.c_int, [P, P, I32, P]),
         "kmc_eval_field": (ctypes.c_int, [P, P, I64, P, P]),
         "kmc_batch_create": (ctypes.c_int, [ctypes.POINTER(P), P, I32]),
         "kmc_batch_destroy": (ctypes.c_int, [P]),
         "kmc_batch_set_system": (ctypes.c_int, [P, I32, I64, I64, I64, P, P, P, P, P, P, P, P, P]),
         "kmc_batch_prepare": (ctypes.c_int, [P, D, I32]),
         "kmc_batch_step": (ctypes.c_int, [P, P]),
+        "kmc_batch_time_sweep": (ctypes.c_int, [P, P, I32, P]),
         "kmc_batch_set_streams": (ctypes.c_int, [P, I32]),
         "kmc_run_steps": (ctypes.c_int, [P, P, I64, I64, I64, I64, P, P, I64, P, D, P, P, P, P, P, P, I32, I32]),
         "kmc_set_tiles": (ctypes.c_int, [P, I32, I32, I32, I32]),
@@ -153,6 +155,7 @@ def lib():
         "kmc_particle_count": (ctypes.c_int, [P, P]),
         "kmc_set_particle_count": (ctypes.c_int, [P, I64]),
         "kmc_charge_energies": (ctypes.c_int, [P, P, I64, P, P]),
+        "kmc_charge_potentials": (ctypes.c_int, [P, P, I64, P, P]),
         "kmc_group_energy": (ctypes.c_int, [P, P, I64, I64, P, P, P]),
         "kmc_propose_extract": (ctypes.c_int, [P, P, I64, I64, P, I32, P]),
         "kmc_propose_inject": (ctypes.c_int, [P, P, I64, P, P, P]),

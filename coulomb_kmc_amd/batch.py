@@ -9,10 +9,15 @@ Metropolis and accepted.  Two ways to run it:
 * ``mode="kernels"`` (default): kmc_batch_step launches each kernel kind once for the
   whole batch (once per expansion order for the order-specialised kernels), the
   system being blockIdx.z of a table of per-system kernel arguments, so a batch step
-  is ~20 launches whatever the number of systems.  The accept sweep uses the fused
-  layout with two column pairs per lane; the near field runs G lanes per point when
-  every system selects the same G with set_tiles(near_lanes=G), else one lane per
-  neighbour leaf.
+  is ~20 launches whatever the number of systems.  The accept sweep of an (order,
+  periodicity) group runs the blocked layout with order truncation over a flat table
+  of (system, 4x4x4 leaf block) pairs when every system of the group is set to
+  sweep_block 1 (no workgroup sized for a larger system), else the fused layout with
+  two column pairs per lane; the near field runs G lanes per point when every system
+  selects the same G with set_tiles(near_lanes=G), else one lane per neighbour leaf.
+  ff-only systems form groups of their own (long range and cell tables only).
+  ``autotune_kernels`` picks the common near-field lane count and sweep layout by
+  timing whole batch steps.
 * ``mode="graph"``: the systems' own kmc_step calls one after the other, with each
   system's tiles from the autotuner (per order / leaf count / proposal count class).
 
@@ -138,6 +143,36 @@ class KMCBatch:
                 log("batch tiles", key, choice)
         return choices
 
+    def autotune_kernels(self, near_lanes=(4, 8, 16), sweep_blocks=(0, 1), steps=4, log=None):
+        """Kernels mode: times whole batch steps (captured, steps_per_graph 2) for every
+        common near-field lane count and accept-sweep layout and keeps the fastest (the
+        batched kernels take one near_lanes for all systems and one sweep layout per
+        group).  The timed steps advance the systems like any other steps."""
+        import torch
+        if self.mode != "kernels":
+            raise ValueError("autotune_kernels is for kernels mode (graph mode: autotune)")
+        timings = {}
+        for g in near_lanes:
+            for sb in sweep_blocks:
+                for s in self.systems:
+                    s.kmc.set_tiles(near_lanes=g, sweep_block=sb, sweep_pairs=2, sweep_split=1)
+                self.capture(2)
+                self.replay(1)
+                torch.cuda.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                self.replay(max(1, steps // 2))
+                e1.record()
+                e1.synchronize()
+                timings[(g, sb)] = e0.elapsed_time(e1) / (2 * max(1, steps // 2))
+                if log:
+                    log("batch tiles near_lanes {} sweep_block {}: {:.3f} ms/step".format(g, sb, timings[(g, sb)]))
+        best = min(timings, key=timings.get)
+        for s in self.systems:
+            s.kmc.set_tiles(near_lanes=best[0], sweep_block=best[1], sweep_pairs=2, sweep_split=1)
+        self.graph = None
+        return {"near_lanes": best[0], "sweep_block": best[1]}, timings
+
     def prepare(self):
         """Sizes every context's work buffers (and, in kernels mode, uploads the argument
         tables); allocates, so it runs outside capture."""
@@ -175,9 +210,39 @@ class KMCBatch:
         self.steps_per_graph = steps_per_graph
         return g
 
-    def replay(self, n=1):
+    def replay(self, n=1, reserve=False):
+        """n replays of the captured steps.  reserve: before each replay every system gets
+        room for its steps (KMCFMM.reserve_store, one synchronisation per replay) and the
+        graph is captured again when a system's cell tables grew, so no selected move is
+        ever dropped for a full leaf."""
         for _ in range(n):
+            if reserve:
+                self.reserve(self.steps_per_graph)
             self.graph.replay()
+
+    def time_sweep(self, reps=5, stream=None):
+        """Average ms of the batch's accept-sweep launches (kernels mode; re-applies every
+        system's last accepted move, so the leaf expansions change)."""
+        import torch
+        if self._batch is None:
+            raise ValueError("time_sweep is for kernels mode")
+        st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        ms = ctypes.c_double(0.0)
+        _native.check(_native.lib().kmc_batch_time_sweep(self._batch, ctypes.c_void_p(st), int(reps),
+                                                         ctypes.byref(ms)), "kmc_batch_time_sweep")
+        return ms.value
+
+    def reserve(self, slots):
+        """Room for `slots` more steps in every system's cell tables; captures the graph
+        again (same steps per graph) when any grew.  Returns whether any grew."""
+        import torch
+        torch.cuda.synchronize()
+        grew = False
+        for s in self.systems:
+            grew |= s.kmc.reserve_store(slots)
+        if grew and self.graph is not None:
+            self.capture(self.steps_per_graph)
+        return grew
 
     def check_flags(self):
         return [s.kmc.check_flags() for s in self.systems]

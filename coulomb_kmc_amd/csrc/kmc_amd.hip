@@ -36,6 +36,15 @@ __global__ void k_fill_i32(int* p, int n, int v) {
     if (i < n) p[i] = v;
 }
 
+// the largest leaf occupancy: wave maxima of a grid-stride pass, atomicMax into *out
+// (which starts at 0)
+__global__ void k_occ_peak(const int* __restrict__ occ, int ncells, int* __restrict__ out) {
+    int m = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ncells; i += gridDim.x * blockDim.x) m = max(m, occ[i]);
+    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(out, m);
+}
+
 // ============================================================================
 // near field by leaf tiles
 // ============================================================================
@@ -889,6 +898,12 @@ __global__ void k_gather_ids(Geom g, int n, const int64_t* ids, int64_t np, cons
     excl[i] = (int32_t)id;
 }
 
+// unit charge on every entry (the potential at the entries instead of their energy)
+__global__ void k_unit_charges(int n, double4* pos4) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) pos4[i].w = 1.0;
+}
+
 // points with charges as evaluation entries, no exclusion
 __global__ void k_pack_charges(Geom g, int n, const double* pos, const double* q, double4* pos4, int32_t* cell,
                                int32_t* excl) {
@@ -1291,6 +1306,7 @@ struct kmc_ctx {
     // scalars
     DBuf<double> energy;
     DBuf<int> flags, err;
+    DBuf<int> occpeak;                // kmc_store_reserve's largest occupancy
     DBuf<MoveRec> rec;
     // propose work buffers
     DBuf<double4> epos4;
@@ -1767,7 +1783,9 @@ static int lr_add_sources(kmc_ctx* c, hipStream_t s, const double4* src, int k) 
 // leaf expansion and long-range term.  The caller synchronises before the scratch
 // buffers go out of scope.
 static int entry_energies(kmc_ctx* c, hipStream_t s, int n, const double4* pos4, const int32_t* cell,
-                          const int32_t* excl, double* out, DBuf<double>& ud, DBuf<double>& ui, DBuf<double>& lr) {
+                          const int32_t* excl, double* out, DBuf<double>& ud, DBuf<double>& ui, DBuf<double>& lr,
+                          double eu = -1.0) {
+    if (eu < 0.0) eu = c->eu;
     const int nfar = (c->L + 1) / 2;
     const bool near_far = c->bc != KMC_BC_FF_ONLY;
     const bool lrt = uses_lr(c);
@@ -1782,7 +1800,7 @@ static int entry_energies(kmc_ctx* c, hipStream_t s, int n, const double4* pos4,
         hipLaunchKernelGGL(k_eval_lr, dim3((n + 255) / 256), dim3(256), 0, s, c->L, n, pos4, c->gvec.p, lr.p);
     }
     hipLaunchKernelGGL(k_charge_energy, dim3((n + 255) / 256), dim3(256), 0, s, n, pos4, near_far ? ud.p : nullptr,
-                       near_far ? ui.p : nullptr, nfar, lrt ? lr.p : nullptr, c->eu, out);
+                       near_far ? ui.p : nullptr, nfar, lrt ? lr.p : nullptr, eu, out);
     HIPCHK(hipGetLastError());
     return KMC_OK;
 }
@@ -2689,6 +2707,30 @@ int kmc_check_flags(kmc_ctx* c, void* stream, int32_t* h_flags) {
     return KMC_OK;
 }
 
+// room for device-side steps: every leaf gets at least `slots` free slots (the cell
+// tables grow, keeping their contents, to the largest occupancy + slots + the context's
+// headroom when the fullest leaf has fewer); synchronises the stream
+int kmc_store_reserve(kmc_ctx* c, void* stream, int32_t slots, int32_t* h_grew) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    if (!c->initialised) return set_err(KMC_E_STATE, "Run initialise before this call");
+    if (slots < 0 || slots > (1 << 20)) return set_err(KMC_E_ARG, "slots must be in [0, 2^20]");
+    if (h_grew) *h_grew = 0;
+    hipStream_t s = (hipStream_t)stream;
+    HIPCHK(c->occpeak.ensure(1));
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, c->occpeak.p, 1, 0);
+    const int nb = (int)std::min<int64_t>(1024, (c->ncells + 255) / 256);
+    hipLaunchKernelGGL(k_occ_peak, dim3(nb), dim3(256), 0, s, c->occ.p, (int)c->ncells, c->occpeak.p);
+    HIPCHK(hipGetLastError());
+    int peak = 0;
+    HIPCHK(hipMemcpyAsync(&peak, c->occpeak.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (c->occmax - peak >= slots) return KMC_OK;
+    const int rc = grow_store(c, peak + slots + c->occ_headroom, s);
+    if (rc) return rc;
+    if (h_grew) *h_grew = 1;
+    return KMC_OK;
+}
+
 int kmc_time_kernel(kmc_ctx* c, void* stream, int32_t which, int32_t reps, int32_t precision, double* h_ms) {
     if (check_ctx(c)) return KMC_E_ARG;
     if (reps < 1 || !h_ms) return set_err(KMC_E_ARG, "bad arguments");
@@ -2884,7 +2926,9 @@ int kmc_particle_count(kmc_ctx* c, int64_t* out) {
     return KMC_OK;
 }
 
-int kmc_charge_energies(kmc_ctx* c, void* stream, int64_t n, const int64_t* d_ids, double* d_out) {
+// charges' energies (energy_unit applied) or, potential: the potential at each charge
+// from all the others (unit charge, no energy_unit)
+static int charge_entries(kmc_ctx* c, void* stream, int64_t n, const int64_t* d_ids, double* d_out, bool potential) {
     if (check_ctx(c)) return KMC_E_ARG;
     if (!c->initialised) return set_err(KMC_E_STATE, "Run initialise before this call");
     if (n < 0 || n >= (int64_t)1 << 30) return set_err(KMC_E_ARG, "bad charge count");
@@ -2901,13 +2945,22 @@ int kmc_charge_energies(kmc_ctx* c, void* stream, int64_t n, const int64_t* d_id
     HIPCHK(hipMemsetAsync(c->err.p, 0, sizeof(int), s));
     hipLaunchKernelGGL(k_gather_ids, dim3((ni + 255) / 256), dim3(256), 0, s, c->geom(), ni, d_ids, c->n, c->ppos4.p,
                        c->pcell.p, 1.0, pos4.p, cell.p, excl.p, c->err.p);
-    int rc = entry_energies(c, s, ni, pos4.p, cell.p, excl.p, d_out, ud, ui, lr);
+    if (potential) hipLaunchKernelGGL(k_unit_charges, dim3((ni + 255) / 256), dim3(256), 0, s, ni, pos4.p);
+    int rc = entry_energies(c, s, ni, pos4.p, cell.p, excl.p, d_out, ud, ui, lr, potential ? 1.0 : c->eu);
     if (rc) return rc;
     int herr = 0;
     HIPCHK(hipMemcpyAsync(&herr, c->err.p, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     if (herr) return fail_err_counter(c, s, KMC_E_ARG, "charge id out of range");
     return KMC_OK;
+}
+
+int kmc_charge_energies(kmc_ctx* c, void* stream, int64_t n, const int64_t* d_ids, double* d_out) {
+    return charge_entries(c, stream, n, d_ids, d_out, false);
+}
+
+int kmc_charge_potentials(kmc_ctx* c, void* stream, int64_t n, const int64_t* d_ids, double* d_out) {
+    return charge_entries(c, stream, n, d_ids, d_out, true);
 }
 
 int kmc_group_energy(kmc_ctx* c, void* stream, int64_t ngroups, int64_t m, const double* d_pos, const double* d_q,
@@ -3211,15 +3264,21 @@ struct kmc_batch {
         double* dE = nullptr;
         bool set = false;
     };
-    struct Group {                    // systems of one order L and one periodicity
+    struct Group {                    // systems of one order L and one periodicity (ff-only apart)
         int L = 0;
         bool per = false;
+        bool ff = false;              // ff-only systems: long range only (no near / far field, no sweep)
         OrderFns fns{};
         BatchShape shape{}, lr_shape{};
         DBuf<EvalArgs> far;
         DBuf<LR2Args> lr;
         DBuf<SweepArgs> sweep;
         DBuf<TailArgs> tail;
+        // blocked sweep (every system of the group set to sweep_block 1): flat table of
+        // (system, block) pairs
+        bool blocked = false;
+        DBuf<int2> sweep_tab;
+        int sweep_nflat = 0;
     };
     std::vector<Sys> sys;
     std::vector<Group*> groups;
@@ -3256,7 +3315,6 @@ int kmc_batch_create(kmc_batch** out, kmc_ctx* const* ctxs, int32_t n) {
         kmc_ctx* c = ctxs[i];
         if (!c) return set_err(KMC_E_ARG, "null context in batch");
         if (!c->initialised) return set_err(KMC_E_STATE, "Run initialise before batching a context");
-        if (c->bc == KMC_BC_FF_ONLY) return set_err(KMC_E_UNSUPPORTED, "ff-only systems are not batched");
         if (c->ncells % 64) return set_err(KMC_E_UNSUPPORTED, "batched systems need at least 4^3 leaves (r >= 3)");
     }
     kmc_batch* b = new kmc_batch();
@@ -3347,12 +3405,19 @@ int kmc_batch_prepare(kmc_batch* b, double kT, int32_t precision) {
     std::vector<std::vector<LR2Args>> lr;
     std::vector<std::vector<SweepArgs>> sweep;
     std::vector<std::vector<TailArgs>> tail;
+    std::vector<std::vector<int2>> sweep_tab;
     BatchShape all{0, 0, 0, 0, 0};
     // near field: the per-point kernel when every system selects the same lane count
     // (so a batch step sums in the order of the systems' own steps), else one lane per leaf
-    b->near_lanes = b->sys.empty() ? 27 : b->sys[0].c->near_lanes;
-    for (const kmc_batch::Sys& y : b->sys)
-        if (y.c->near_lanes != b->near_lanes) b->near_lanes = 27;
+    // (ff-only systems have no near field and do not take part)
+    bool first = true;
+    b->near_lanes = 27;
+    for (const kmc_batch::Sys& y : b->sys) {
+        if (y.c->bc == KMC_BC_FF_ONLY) continue;
+        if (first) b->near_lanes = y.c->near_lanes;
+        else if (y.c->near_lanes != b->near_lanes) b->near_lanes = 27;
+        first = false;
+    }
     if (b->near_lanes == 0) b->near_lanes = 27;
     if (b->near_lanes < 0) b->near_lanes = -b->near_lanes;   // batches use the per-point kernels
     for (size_t i = 0; i < b->sys.size(); i++) {
@@ -3368,9 +3433,11 @@ int kmc_batch_prepare(kmc_batch* b, double kT, int32_t precision) {
         }
         if (!step_sized(c, P, T)) return set_err(KMC_E_STATE, "batch: work buffers not sized");
         const int ne = (int)(T + P);
+        const bool ff = c->bc == KMC_BC_FF_ONLY;
         mat.push_back(mat_args(c, y.nsteps, P, y.k, y.pids_all, y.disp_all, y.counter, y.pid, y.es, y.new_pos));
         setup.push_back(setup_args(c, P, T, y.pid, y.es, y.new_pos, nullptr));
-        nearA.push_back(eval_args(c, ne, c->epos4.p, c->ecell.p, c->eexcl.p, c->UD.p));
+        // (no points for an ff-only system: its near-field blocks exit at once)
+        nearA.push_back(eval_args(c, ff ? 0 : ne, c->epos4.p, c->ecell.p, c->eexcl.p, c->UD.p));
         comb.push_back(combine_args(c, P, T, y.dE, nullptr, kT, c->rates.p));
         sel.push_back(select_args(c, P, T, y.pid, y.dE, y.nsteps, y.uniform, kT, y.counter, y.selected));
         all.n++;
@@ -3378,23 +3445,36 @@ int kmc_batch_prepare(kmc_batch* b, double kT, int32_t precision) {
         all.max_P = std::max(all.max_P, (int)P);
         all.max_T = std::max(all.max_T, (int)T);
         all.max_cells = std::max(all.max_cells, (int)c->ncells);
-        // group by (order, periodicity)
+        // group by (order, periodicity, ff-only)
         const bool per = c->bc != KMC_BC_FREE_SPACE;
         size_t gi = 0;
-        while (gi < b->groups.size() && !(b->groups[gi]->L == c->L && b->groups[gi]->per == per)) gi++;
+        while (gi < b->groups.size() &&
+               !(b->groups[gi]->L == c->L && b->groups[gi]->per == per && b->groups[gi]->ff == ff))
+            gi++;
         if (gi == b->groups.size()) {
             Group* g = new Group();
             g->L = c->L;
             g->per = per;
+            g->ff = ff;
             g->fns = c->fns;
+            g->blocked = true;
             b->groups.push_back(g);
             far.emplace_back();
             lr.emplace_back();
             sweep.emplace_back();
             tail.emplace_back();
+            sweep_tab.emplace_back();
         }
         Group* g = b->groups[gi];
         far[gi].push_back(eval_args(c, ne, c->epos4.p, c->ecell.p, c->eexcl.p, c->UI.p));
+        // the group sweeps blocked when every system of it asks for it (and has a leaf grid
+        // with a multiple of 4 per side: every batched grid has, being >= 4^3 and 2^k)
+        g->blocked = g->blocked && c->sweep_block == 1 && c->ncps % 4 == 0;
+        if (!ff) {
+            const int nbk = (c->ncps / 4) * (c->ncps / 4) * (c->ncps / 4);
+            const int si = (int)sweep[gi].size();
+            for (int q = 0; q < nbk; q++) sweep_tab[gi].push_back(make_int2(si, q));
+        }
         sweep[gi].push_back(sweep_rec_args(c));
         tail[gi].push_back(tail_args(c, y.counter));
         g->shape.n++;
@@ -3417,6 +3497,8 @@ int kmc_batch_prepare(kmc_batch* b, double kT, int32_t precision) {
     for (size_t gi = 0; gi < b->groups.size(); gi++) {
         Group* g = b->groups[gi];
         HIPCHK(upload(g->far, far[gi]));
+        HIPCHK(upload(g->sweep_tab, sweep_tab[gi]));
+        g->sweep_nflat = (int)sweep_tab[gi].size();
         HIPCHK(upload(g->lr, lr[gi]));
         HIPCHK(upload(g->sweep, sweep[gi]));
         HIPCHK(upload(g->tail, tail[gi]));
@@ -3464,6 +3546,32 @@ static int batch_groups(kmc_batch* b, hipStream_t s, int phase, F fn) {
     return KMC_OK;
 }
 
+// average ms of one batched accept sweep of every group (the sweep launches of a batch
+// step, one after another on the stream), each re-applying its systems' last accepted
+// move (it changes the leaf expansions, as kmc_time_kernel's sweep does)
+int kmc_batch_time_sweep(kmc_batch* b, void* stream, int32_t reps, double* h_ms) {
+    if (!b || reps < 1 || !h_ms) return set_err(KMC_E_ARG, "bad arguments");
+    if (!b->ready) return set_err(KMC_E_STATE, "call kmc_batch_prepare before kmc_batch_time_sweep");
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, s));
+    for (int i = 0; i < reps; i++)
+        for (kmc_batch::Group* g : b->groups)
+            if (!g->ff)
+                g->fns.bsweep(g->sweep.p, g->blocked ? g->sweep_tab.p : nullptr, g->sweep_nflat, g->shape, g->per, s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    HIPCHK(hipEventDestroy(e0));
+    HIPCHK(hipEventDestroy(e1));
+    *h_ms = ms / reps;
+    return KMC_OK;
+}
+
 int kmc_batch_step(kmc_batch* b, void* stream) {
     if (!b) return set_err(KMC_E_ARG, "null batch");
     if (!b->ready) return set_err(KMC_E_STATE, "call kmc_batch_prepare before kmc_batch_step");
@@ -3481,7 +3589,7 @@ int kmc_batch_step(kmc_batch* b, void* stream) {
     if (b->mixed) launch_batch_near<true>(b->near_lanes, a.max_ne, n, b->nearA.p, s);
     else launch_batch_near<false>(b->near_lanes, a.max_ne, n, b->nearA.p, s);
     int rc = batch_groups(b, s, 0, [](kmc_batch::Group* g, hipStream_t t) {
-        g->fns.bfar(g->far.p, g->shape, t);
+        if (!g->ff) g->fns.bfar(g->far.p, g->shape, t);
         if (g->lr_shape.n > 0) g->fns.blr(g->lr.p, g->lr_shape, t);
     });
     if (rc) return rc;
@@ -3491,7 +3599,8 @@ int kmc_batch_step(kmc_batch* b, void* stream) {
         hipLaunchKernelGGL(kb_combine<false>, dim3((a.max_T + 63) / 64, 1, n), dim3(64), 0, s, b->comb.p);
     hipLaunchKernelGGL(kb_select, dim3(1, 1, n), dim3(1024), 0, s, b->sel.p);
     rc = batch_groups(b, s, 1, [](kmc_batch::Group* g, hipStream_t t) {
-        g->fns.bsweep(g->sweep.p, g->shape, g->per, t);
+        if (!g->ff)
+            g->fns.bsweep(g->sweep.p, g->blocked ? g->sweep_tab.p : nullptr, g->sweep_nflat, g->shape, g->per, t);
         g->fns.btail(g->tail.p, g->shape, t);
     });
     if (rc) return rc;

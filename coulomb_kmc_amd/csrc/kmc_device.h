@@ -860,7 +860,7 @@ __device__ __forceinline__ void sweep_block_dispatch(const SweepArgs& a, int gro
 
 // grid: one workgroup per 4x4x4 block of leaves (ncps a multiple of 4), 64 x NG threads
 template <int L, int NP, int NI, bool PER>
-__device__ __forceinline__ void sweep_block_body(const SweepArgs& a) {
+__device__ __forceinline__ void sweep_block_body(const SweepArgs& a, int b) {
     constexpr int NH = nhalf(L);
     constexpr int ST = stage_stride(NH);
     constexpr int NIMG = PER ? 27 : 1;
@@ -869,7 +869,6 @@ __device__ __forceinline__ void sweep_block_body(const SweepArgs& a) {
     const int lane = threadIdx.x;
     const int group = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     const int n = a.g.ncps, nb = n >> 2;
-    const int b = blockIdx.x;
     const int bx = b % nb, by = (b / nb) % nb, bz = b / (nb * nb);
     const int tx = 4 * bx + (lane & 3), ty = 4 * by + ((lane >> 2) & 3), tz = 4 * bz + (lane >> 4);
     if (!a.rec->valid) return;   // (uniform over the grid: nothing to add)
@@ -900,13 +899,16 @@ __device__ __forceinline__ void sweep_block_body(const SweepArgs& a) {
 }
 template <int L, int NP, int NI, bool PER>
 __global__ void __launch_bounds__(512, 2) k_sweep_block(SweepArgs a) {
-    sweep_block_body<L, NP, NI, PER>(a);
+    sweep_block_body<L, NP, NI, PER>(a, (int)blockIdx.x);
 }
+// batches: one flat grid over the (system, block) pairs of a group, tab[blockIdx.x] =
+// (system index in the argument table, block of that system), so no workgroup is
+// launched for blocks a smaller system does not have
 template <int L, int NP, int NI, bool PER>
-__global__ void __launch_bounds__(512, 2) kb_sweep_block(const SweepArgs* __restrict__ args) {
-    const SweepArgs& a = args[blockIdx.z];
-    if ((int)blockIdx.x >= (a.g.ncps >> 2) * (a.g.ncps >> 2) * (a.g.ncps >> 2)) return;
-    sweep_block_body<L, NP, NI, PER>(a);
+__global__ void __launch_bounds__(512, 2) kb_sweep_block(const SweepArgs* __restrict__ args,
+                                                         const int2* __restrict__ tab) {
+    const int2 e = tab[blockIdx.x];
+    sweep_block_body<L, NP, NI, PER>(args[e.x], e.y);
 }
 
 template <int L, int NP, bool PER>
@@ -2186,7 +2188,7 @@ struct BatchShape {
 };
 typedef void (*BFarLaunch)(const EvalArgs*, BatchShape, hipStream_t);
 typedef void (*BLRLaunch)(const LR2Args*, BatchShape, hipStream_t);
-typedef void (*BSweepLaunch)(const SweepArgs*, BatchShape, bool, hipStream_t);
+typedef void (*BSweepLaunch)(const SweepArgs*, const int2*, int, BatchShape, bool, hipStream_t);
 typedef void (*BTailLaunch)(const TailArgs*, BatchShape, hipStream_t);
 
 template <int L>
@@ -2200,10 +2202,16 @@ static void blaunch_lr(const LR2Args* t, BatchShape b, hipStream_t s) {
     hipLaunchKernelGGL(kb_lr_part<L>, dim3((b.max_P + 255) / 256, 12, b.n), dim3(256), 0, s, t);
     hipLaunchKernelGGL(kb_lr_prop2<L>, dim3((b.max_T + 255) / 256, 12, b.n), dim3(256), 0, s, t);
 }
-// fused accept sweep, two column pairs per lane
+// accept sweep of a group: the blocked layout over the flat (system, block) table when
+// one is given (nflat blocks), else the fused layout (grid sized for the largest system)
 template <int L>
-static void blaunch_sweep(const SweepArgs* t, BatchShape b, bool per, hipStream_t s) {
+static void blaunch_sweep(const SweepArgs* t, const int2* tab, int nflat, BatchShape b, bool per, hipStream_t s) {
     constexpr int ng = ((L + 1) / 2 + 1) / 2;
+    if (tab && nflat > 0 && 64 * ng <= 512) {
+        if (per) hipLaunchKernelGGL((kb_sweep_block<L, 2, 1, true>), dim3(nflat), dim3(64, ng), 0, s, t, tab);
+        else hipLaunchKernelGGL((kb_sweep_block<L, 2, 1, false>), dim3(nflat), dim3(64, ng), 0, s, t, tab);
+        return;
+    }
     if (per) hipLaunchKernelGGL((kb_sweep_fused<L, 2, true>), dim3(b.max_cells / 64, 1, b.n), dim3(64, ng), 0, s, t);
     else hipLaunchKernelGGL((kb_sweep_fused<L, 2, false>), dim3(b.max_cells / 64, 1, b.n), dim3(64, ng), 0, s, t);
 }

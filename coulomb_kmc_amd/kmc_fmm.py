@@ -201,7 +201,23 @@ class KMCFMM(InjectorExtractor):
         self._torch()
         _native.check(self._lib.kmc_initialise(self._ctx, self._stream(), ctypes.byref(e)), "kmc_initialise")
         self.energy = e.value
+        # the per-charge potential of the initial state, as ppmd's FMM writes it into
+        # group._kmc_fmm_potential (coulomb_kmc/kmc_fmm.py:942-958); [N, 1], no energy_unit
+        self.potential = self.charge_potentials(np.arange(self.N, dtype=np.int64)).reshape(-1, 1)
         self._profile_inc("initialise", time.time() - t0)
+
+    def charge_potentials(self, ids):
+        """Potential at each charge ids[n] from all the other charges (near field, leaf
+        expansion and, for pbc / ff-only, the long-range term; energy_unit not applied)."""
+        ids = np.ascontiguousarray(np.asarray(ids, dtype=np.int64).ravel())
+        if ids.size == 0:
+            return np.zeros(0)
+        torch = self._torch()
+        d_ids = torch.as_tensor(ids).cuda()
+        d_out = torch.empty(ids.size, dtype=torch.float64, device="cuda")
+        _native.check(self._lib.kmc_charge_potentials(self._ctx, self._stream(), ids.size, d_ids.data_ptr(),
+                                                      d_out.data_ptr()), "kmc_charge_potentials")
+        return d_out.cpu().numpy()
 
     def _check_move_valid(self, old_pos, new_pos):
         # coulomb_kmc/kmc_mpi_decomp.py:312-345
@@ -440,6 +456,16 @@ class KMCFMM(InjectorExtractor):
         _native.check(self._lib.kmc_materialize_moves(self._ctx, s, nsteps, int(P), int(k), d_pids_all.data_ptr(),
                                                       d_disp_all.data_ptr(), d_counter.data_ptr(), d_pid.data_ptr(),
                                                       d_es.data_ptr(), d_new_pos.data_ptr()), "materialize")
+
+    def reserve_store(self, slots, stream=None):
+        """Gives every leaf at least `slots` free slots for the next device-side run (a run
+        of n steps needs n), growing the cell tables when needed
+        (coulomb_kmc/kmc_local.py:558-578).  Returns True when they grew: graphs captured
+        before must be captured again.  Synchronises."""
+        g = ctypes.c_int32(0)
+        s = stream if stream is not None else self._stream()
+        _native.check(self._lib.kmc_store_reserve(self._ctx, s, int(slots), ctypes.byref(g)), "kmc_store_reserve")
+        return bool(g.value)
 
     def check_flags(self):
         f = ctypes.c_int32(0)

@@ -239,6 +239,14 @@ int kmc_set_particle_count(kmc_ctx* ctx, int64_t n);
  * Synchronises the stream.
  */
 int kmc_charge_energies(kmc_ctx* ctx, void* stream, int64_t n, const int64_t* d_ids, double* d_out);
+/*
+ * Potential at each charge d_ids[n] from all the other charges (the same parts as
+ * kmc_charge_energies at unit charge, energy_unit not applied): the per-charge
+ * potential that KMCFMM.initialise has ppmd's FMM write into group._kmc_fmm_potential
+ * (coulomb_kmc/kmc_fmm.py:942-958); 0.5 sum_i q_i phi_i energy_unit is the energy.
+ * Synchronises the stream.
+ */
+int kmc_charge_potentials(kmc_ctx* ctx, void* stream, int64_t n, const int64_t* d_ids, double* d_out);
 
 /*
  * Energy of each of ngroups groups of m charges alone in the domain, energy_unit
@@ -319,13 +327,27 @@ int kmc_lattice_step(kmc_ctx* ctx, void* stream, int64_t M, const double* h_offs
 /*
  * Flags raised by device-side steps (kmc_step, kmc_run_steps, kmc_lattice_step, batches),
  * or-ed into *h_flags and cleared by the call: 1 the moved charge was not found exactly
- * once in its leaf's table, 2 the selected move's target leaf was full
- * (not accepted), 4 no proposal had a positive rate, 8 a moving-charge id out of range,
+ * once in its leaf's table, 2 the selected move's target leaf was full (not accepted;
+ * kmc_store_reserve before the run rules it out), 4 no proposal had a positive rate, 8 a moving-charge id out of range,
  * 16 a synthetic move-stream row out of range, 32 a proposal outside the domain or
  * beyond max_move, or malformed es (nothing accepted in that step; the host propose
  * paths return KMC_E_BAD_MOVE for the same inputs).
  */
 int kmc_check_flags(kmc_ctx* ctx, void* stream, int32_t* h_flags);
+
+/*
+ * Room in the cell tables for device-side steps.  Replaces the growth of the particle
+ * store inside accept (LocalParticleData._resize_particle_store,
+ * coulomb_kmc/kmc_local.py:558-578, called at :321-330 when the target cell is full):
+ * device steps cannot allocate, so between runs the caller reserves room for the next
+ * run.  Every leaf gets at least `slots` free slots (a run of n steps adds at most n
+ * charges to any leaf, so slots = n makes flag 2 impossible in that run); when the
+ * fullest leaf has fewer, the tables grow, contents kept, to its occupancy + slots +
+ * the context's occupancy headroom, and *h_grew is 1: graphs captured before then hold
+ * the old tables and must be captured again (batches: kmc_batch_prepare again).
+ * Synchronises the stream (not capturable).
+ */
+int kmc_store_reserve(kmc_ctx* ctx, void* stream, int32_t slots, int32_t* h_grew);
 
 /*
  * Tile choices of the hot kernels (filled by the autotuner in coulomb_kmc_amd/tune.py):
@@ -368,15 +390,20 @@ int kmc_probe_peaks(void* stream, int64_t bytes, int32_t reps, double* h_out);
  * Batches of independent systems (no reference counterpart: the reference steps one
  * system per process).  The step of every system in the batch is one launch per kernel
  * kind (per expansion order for the order-specialised kernels), the system being
- * blockIdx.z of a table of per-system kernel arguments.  Systems: initialised contexts,
- * not ff-only, >= 4^3 leaves.  Per system, the same device move stream as
+ * blockIdx.z of a table of per-system kernel arguments.  Systems: initialised contexts
+ * with >= 4^3 leaves, of any boundary condition (ff-only systems form groups of their
+ * own that run the long range and the cell-table tail only, as their single steps do).  Per system, the same device move stream as
  * kmc_materialize_moves + kmc_step (pids_all[nsteps*P], disp_all[nsteps*P*k*3],
  * uniform[nsteps], counter, selected[nsteps], and pid[P], es[P+1], new_pos[P*k*3],
  * dE[P*k] work arrays).  kmc_batch_prepare sizes every context's work buffers and
  * uploads the argument tables (it allocates; call it again after host-side accepts
  * or anything else that may reallocate a context's tables); kmc_batch_step allocates
- * nothing and can be captured in a graph.  The accept sweep of a batch always uses
- * the fused layout with two column pairs per lane, the near field 32 lanes per point.
+ * nothing and can be captured in a graph.  The accept sweep of each (order,
+ * periodicity) group uses the blocked layout with order truncation when every system
+ * of the group is set to sweep_block 1 (one flat grid over the groups' (system, 4x4x4
+ * block) pairs, so no workgroup goes to a block a smaller system lacks), else the fused
+ * layout with two column pairs per lane; the near field uses G lanes per point when
+ * every system selects the same G (near_lanes G or -G), else one lane per leaf.
  */
 typedef struct kmc_batch kmc_batch;
 int kmc_batch_create(kmc_batch** out, kmc_ctx* const* ctxs, int32_t n);
@@ -386,6 +413,11 @@ int kmc_batch_set_system(kmc_batch* batch, int32_t i, int64_t nsteps, int64_t P,
                          int64_t* d_pid, int64_t* d_es, double* d_new_pos, double* d_dE);
 int kmc_batch_prepare(kmc_batch* batch, double kT, int32_t precision);
 int kmc_batch_step(kmc_batch* batch, void* stream);
+/* Average ms of the batch's accept-sweep launches (no reference counterpart; the
+ * measurement of the batched sweep): one launch per (order, periodicity) group, run
+ * reps times one after another on the stream, each re-applying every system's last
+ * accepted move (the leaf expansions change, as with kmc_time_kernel's sweep). */
+int kmc_batch_time_sweep(kmc_batch* batch, void* stream, int32_t reps, double* h_ms);
 /* Side streams of a batch (no reference counterpart): with nstreams > 1, the launches
  * that run once per (expansion order, periodicity) group -- far field, long range,
  * accept sweep and cell-table tail -- go to nstreams - 1 library-owned side streams

@@ -18,7 +18,8 @@ def _systems(seed):
     rng = np.random.default_rng(seed)
     out = []
     for i, (N, E, r, L, bc) in enumerate(((600, 8.0, 4, 8, "pbc"), (900, 10.0, 4, 12, "free_space"),
-                                           (1500, 12.0, 5, 8, "27"), (700, 8.0, 4, 12, "pbc"))):
+                                           (1500, 12.0, 5, 8, "27"), (700, 8.0, 4, 12, "pbc"),
+                                           (800, 9.0, 4, 12, "ff-only"))):
         pos, q = neutral_system(rng, N, E)
         k = KMCFMM(pos, q, E, r=r, l=L, boundary_condition=bc, max_move=1.0, precision="mixed")
         k.initialise()
@@ -34,14 +35,14 @@ def _systems(seed):
     return out
 
 
-def _check_batch(streams, mode="graph", near_lanes=27):
+def _check_batch(streams, mode="graph", near_lanes=27, sweep_block=0):
     import torch
     from coulomb_kmc_amd.batch import KMCBatch
     steps = 4
     batched = _systems(5)
     single = _systems(5)
     for s in batched + single:   # the batched kernels' shapes
-        s.kmc.set_tiles(near_lanes=near_lanes, sweep_block=0, sweep_pairs=2, sweep_split=1)
+        s.kmc.set_tiles(near_lanes=near_lanes, sweep_block=sweep_block, sweep_pairs=2, sweep_split=1)
     b = KMCBatch(batched, kT=1.0, precision="mixed", streams=streams, mode=mode)
     b.capture(steps_per_graph=2)
     b.replay(steps // 2)
@@ -76,6 +77,13 @@ def test_batched_kernels_equal_single_system_steps():
 def test_batched_per_point_near_equals_single_system_steps(near_lanes):
     # every system selects the same per-point lane count: the batch uses kb_near<G>
     _check_batch(1, "kernels", near_lanes)
+
+
+@pytest.mark.parametrize("streams", [1, 3])
+def test_batched_blocked_sweep_equals_single_system_steps(streams):
+    # sweep_block 1 everywhere: one blocked-sweep launch per group over its flat
+    # (system, 4x4x4 block) table
+    _check_batch(streams, "kernels", 8, sweep_block=1)
 
 
 @pytest.mark.skip(reason="recording the systems on extra torch side streams crashed the host process inside "

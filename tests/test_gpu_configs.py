@@ -234,7 +234,7 @@ def test_cfg3_batch_256_systems():
         kmc = KMCFMM(x["pos"].copy(), x["q"].reshape(-1, 1), x["E"], r=x["r"], l=x["L"],
                      boundary_condition=x["bc"], max_move=x["radius"], precision="mixed", init_method="fmm")
         kmc.initialise()
-        kmc.set_tiles(near_lanes=spec["near_lanes"], sweep_block=0, sweep_pairs=2, sweep_split=1)
+        kmc.set_tiles(near_lanes=spec["near_lanes"], sweep_block=1, sweep_pairs=2, sweep_split=1)
         p, d, u = moves[i]
         return BatchSystem(kmc, x["P"], x["k"], _dev(p.reshape(-1), torch.int64), _dev(d.reshape(-1), torch.float64),
                            _dev(u, torch.float64))

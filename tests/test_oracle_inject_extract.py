@@ -19,6 +19,7 @@ import pytest
 
 import oracle as O
 from ewald import ewald_energy, free_space_energy, nearest27_energy
+from helpers import neutral_system
 
 from coulomb_kmc_amd.kmc_fmm import lr_operator_reference_layout
 
@@ -126,3 +127,25 @@ def test_charge_energy_free_space_brute_force():
     i = int(ids[0])
     keep = np.arange(N) != i
     assert abs(-ce[0] - (free_space_energy(pos[keep], q[keep]) - e_all)) <= 1e-5 * abs(ce[0])
+
+
+@pytest.mark.parametrize("bc", ["free_space", "27", "pbc", "ff-only"])
+def test_half_sum_of_charge_energies_is_the_energy(bc):
+    """0.5 sum_i U_i = E for the oracle's per-charge energies (the identity the
+    per-charge potential of initialise is checked by); free space also against the
+    brute-force potential within the expansion truncation."""
+    from coulomb_kmc_amd.kmc_fmm import lr_operator_reference_layout
+    rng = np.random.default_rng(21)
+    N, E = 300, 6.0
+    pos, q = neutral_system(rng, N, E)
+    lr = lr_operator_reference_layout(12, E) if bc in ("pbc", "ff-only") else None
+    o = O.Oracle(pos, q, E, 4, 12, bc=bc, max_move=1.0, lr=lr)
+    e = o.initialise()
+    u = o.charge_energy(np.arange(N))
+    assert abs(0.5 * u.sum() - e) <= 1e-13 * abs(e)
+    if bc == "free_space":
+        d = pos[:, None, :] - pos[None, :, :]
+        r = np.sqrt((d * d).sum(-1))
+        np.fill_diagonal(r, np.inf)
+        phi = (q[None, :] / r).sum(axis=1)
+        assert np.max(np.abs(u / q - phi)) <= 1e-5 * np.max(np.abs(phi))
