@@ -492,6 +492,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="batch: keep the given tiles (no autotuning)")
     ap.add_argument("--near-lanes", type=int, default=0, help="0: autotuned")
+    ap.add_argument("--near-order", type=int, default=-1,
+                    help="near-field entry order (kmc_set_near_order; -1: autotuned): 0 proposals, 1 leaf tiles")
     ap.add_argument("--sweep-block", type=int, default=-1,
                     help="accept-sweep layout (kmc_set_tiles sweep_block; -1: autotuned): 0 fused, 1 blocked with "
                          "order truncation, 64..256 leaves per workgroup")
@@ -547,7 +549,8 @@ def main():
 
     kmc = KMCFMM(pos0.copy(), q.reshape(-1, 1), E, r=r, l=L, boundary_condition=bc, max_move=cfg["radius"])
     tiles = tune.choose_tiles(L, T, (2 ** (r - 1)) ** 3, near_lanes=args.near_lanes or None,
-                              sweep_block=None if args.sweep_block < 0 else args.sweep_block)
+                              sweep_block=None if args.sweep_block < 0 else args.sweep_block,
+                              near_order=None if args.near_order < 0 else args.near_order)
     kmc.set_tiles(**tiles)
     torch.cuda.synchronize()
     t0 = time.time()

@@ -43,7 +43,7 @@ EXPORTS = (
     "kmc_set_expansions", "kmc_get_multipole", "kmc_get_store", "kmc_store_occmax",
     "kmc_get_particles", "kmc_get_energy", "kmc_set_energy", "kmc_ncells", "kmc_propose",
     "kmc_propose_with_dats", "kmc_accept", "kmc_step", "kmc_materialize_moves",
-    "kmc_check_flags", "kmc_store_reserve", "kmc_set_tiles", "kmc_sweep_hop", "kmc_time_kernel", "kmc_probe_peaks", "kmc_eval_field",
+    "kmc_check_flags", "kmc_store_reserve", "kmc_set_tiles", "kmc_set_near_order", "kmc_profile_enable", "kmc_profile_read", "kmc_sweep_hop", "kmc_time_kernel", "kmc_probe_peaks", "kmc_eval_field",
     "kmc_batch_create", "kmc_batch_destroy", "kmc_batch_set_system", "kmc_batch_prepare", "kmc_batch_step", "kmc_batch_time_sweep", "kmc_batch_set_streams", "kmc_run_steps",
     "kmc_particle_count", "kmc_set_particle_count", "kmc_charge_energies", "kmc_charge_potentials", "kmc_group_energy",
     "kmc_propose_extract", "kmc_propose_inject", "kmc_extract", "kmc_inject", "kmc_discover_sites",
@@ -149,6 +149,9 @@ def lib():
         "kmc_batch_set_streams": (ctypes.c_int, [P, I32]),
         "kmc_run_steps": (ctypes.c_int, [P, P, I64, I64, I64, I64, P, P, I64, P, D, P, P, P, P, P, P, I32, I32]),
         "kmc_set_tiles": (ctypes.c_int, [P, I32, I32, I32, I32]),
+        "kmc_set_near_order": (ctypes.c_int, [P, I32]),
+        "kmc_profile_enable": (ctypes.c_int, [P, I32]),
+        "kmc_profile_read": (ctypes.c_int, [P, P]),
         "kmc_sweep_hop": (ctypes.c_int, [P, P, D, P, P]),
         "kmc_time_kernel": (ctypes.c_int, [P, P, I32, I32, I32, P]),
         "kmc_probe_peaks": (ctypes.c_int, [P, I64, I32, P]),

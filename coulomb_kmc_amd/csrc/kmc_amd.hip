@@ -234,6 +234,10 @@ __global__ void __launch_bounds__(NT_THREADS) k_near_tile(EvalArgs a, const int*
 
 template <int G, bool MIXED>
 __global__ void __launch_bounds__(256) k_near(EvalArgs a) { near_body<G, MIXED>(a); }
+template <int G, bool MIXED>
+__global__ void __launch_bounds__(256) k_near_sorted(EvalArgs a, const int* __restrict__ perm) {
+    near_sorted_body<G, MIXED>(a, perm);
+}
 // batched form: system blockIdx.z of an argument table (grid sized for the largest)
 template <int G, bool MIXED>
 __global__ void __launch_bounds__(256) kb_near(const EvalArgs* __restrict__ args) {
@@ -898,6 +902,32 @@ __global__ void k_gather_ids(Geom g, int n, const int64_t* ids, int64_t np, cons
     excl[i] = (int32_t)id;
 }
 
+// near-field pair evaluations of the entries (the reference's direct-sum division /
+// square-root count): the occupancies of the leaves around each entry's leaf
+__global__ void k_pair_count(EvalArgs a, unsigned long long* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned int cnt = 0;
+    if (i < a.n) {
+        const int c = a.cell[i], n = a.g.ncps;
+        const int tx = c % n, ty = (c / n) % n, tz = c / (n * n);
+        for (int k = 0; k < 27; k++) {
+            int ox, oy, oz;
+            stencil(k, ox, oy, oz);
+            int nx = tx + ox, ny = ty + oy, nz = tz + oz;
+            if (a.g.bc != KMC_BC_FREE_SPACE) {
+                nx = (nx + n) % n;
+                ny = (ny + n) % n;
+                nz = (nz + n) % n;
+            } else if (nx < 0 || nx >= n || ny < 0 || ny >= n || nz < 0 || nz >= n) {
+                continue;
+            }
+            cnt += (unsigned int)min(a.occ[nx + n * (ny + n * nz)], a.occmax);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+    if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(out, (unsigned long long)cnt);
+}
+
 // unit charge on every entry (the potential at the entries instead of their energy)
 __global__ void k_unit_charges(int n, double4* pos4) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1264,6 +1294,7 @@ struct kmc_ctx {
     int occ_headroom = 8;
     int occmax = 0;
     int near_lanes = 27;
+    int near_order = 0;               // 1: per-point near field over the entries sorted by leaf tile
     int sweep_block = 0;
     int sweep_pairs = 2;
     int sweep_split = 1;
@@ -1285,6 +1316,19 @@ struct kmc_ctx {
     // (batches recorded against the old layout refuse to step)
     int64_t generation = 0;
     OrderFns fns{};
+    // kmc_profile_*: device time of the hot kernels of host-called (not captured) proposes
+    // and accepts, HIP event pairs summed when read
+    struct ProfPair {
+        hipEvent_t a, b;
+        int kind;
+        int64_t count;
+    };
+    bool prof = false;
+    std::vector<ProfPair> prof_pending;
+    std::vector<hipEvent_t> prof_free;
+    double prof_ms[4] = {0, 0, 0, 0};       // near field, far field, long range, accept sweep
+    int64_t prof_count[4] = {0, 0, 0, 0};   // evaluation points (near, far), proposals (long range), sweeps
+    DBuf<unsigned long long> prof_pairs;    // near-field pair evaluations (one rsqrt each)
 
     // particles
     DBuf<double4> ppos4;
@@ -1403,9 +1447,33 @@ static void launch_near_tiled(const kmc_ctx* c, hipStream_t s, const EvalArgs& a
         default: hipLaunchKernelGGL((k_near_tile<16, MIXED>), dim3(nt), dim3(NT_THREADS), 0, s, a, w.toff, w.perm); break;
     }
 }
+// the counting sort of the entries by leaf tile, then the per-point kernel over that order
+template <bool MIXED>
+static void launch_near_sorted(const kmc_ctx* c, hipStream_t s, const EvalArgs& a, int G, const ProposeWork& w) {
+    const int n = c->ncps, nt = near_tiles(c), ne = a.n;
+    hipLaunchKernelGGL(k_tile_count, dim3((ne + 255) / 256), dim3(256), 0, s, ne, a.cell, n, w.tcount);
+    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, nt, w.tcount, w.toff, w.tcur);
+    hipLaunchKernelGGL(k_tile_scatter, dim3((ne + 255) / 256), dim3(256), 0, s, ne, a.cell, n, w.tcur, w.perm);
+    const dim3 grid((unsigned)(((long long)ne * G + 255) / 256));
+    switch (G) {
+        case 1: hipLaunchKernelGGL((k_near_sorted<1, MIXED>), grid, dim3(256), 0, s, a, w.perm); break;
+        case 4: hipLaunchKernelGGL((k_near_sorted<4, MIXED>), grid, dim3(256), 0, s, a, w.perm); break;
+        case 8: hipLaunchKernelGGL((k_near_sorted<8, MIXED>), grid, dim3(256), 0, s, a, w.perm); break;
+        case 16: hipLaunchKernelGGL((k_near_sorted<16, MIXED>), grid, dim3(256), 0, s, a, w.perm); break;
+        case 32: hipLaunchKernelGGL((k_near_sorted<32, MIXED>), grid, dim3(256), 0, s, a, w.perm); break;
+        default: hipLaunchKernelGGL((k_near_sorted<64, MIXED>), grid, dim3(256), 0, s, a, w.perm); break;
+    }
+}
 static void launch_near_any(const kmc_ctx* c, hipStream_t s, const EvalArgs& a, bool mixed, int P, int T,
                             const ProposeWork& w) {
     if (a.n <= 0) return;
+    const int G = c->near_lanes;
+    if (c->near_order == 1 && (G == 1 || G == 4 || G == 8 || G == 16 || G == 32 || G == 64) && near_tiles(c) > 0 &&
+        w.perm) {
+        if (mixed) launch_near_sorted<true>(c, s, a, G, w);
+        else launch_near_sorted<false>(c, s, a, G, w);
+        return;
+    }
     if (c->near_lanes < 0 && near_tiles(c) > 0 && w.perm) {
         if (mixed) launch_near_tiled<true>(c, s, a, -c->near_lanes, w);
         else launch_near_tiled<false>(c, s, a, -c->near_lanes, w);
@@ -1594,6 +1662,49 @@ static int join_side(kmc_ctx* c, hipStream_t s, int i) {
     return KMC_OK;
 }
 
+// ---- kernel profile (kmc_profile_*)
+static bool prof_on(const kmc_ctx* c, hipStream_t s) {
+    if (!c->prof) return false;
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &st) == hipSuccess && st == hipStreamCaptureStatusNone;
+}
+static hipEvent_t prof_mark(kmc_ctx* c, hipStream_t s) {
+    hipEvent_t e = nullptr;
+    if (!c->prof_free.empty()) {
+        e = c->prof_free.back();
+        c->prof_free.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+        return nullptr;
+    }
+    if (hipEventRecord(e, s) != hipSuccess) {
+        c->prof_free.push_back(e);
+        return nullptr;
+    }
+    return e;
+}
+// waits for every pending pair and adds it to the sums
+static int prof_collect(kmc_ctx* c) {
+    for (auto& q : c->prof_pending) {
+        HIPCHK(hipEventSynchronize(q.b));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, q.a, q.b));
+        c->prof_ms[q.kind] += ms;
+        c->prof_count[q.kind] += q.count;
+        c->prof_free.push_back(q.a);
+        c->prof_free.push_back(q.b);
+    }
+    c->prof_pending.clear();
+    return KMC_OK;
+}
+static void prof_add(kmc_ctx* c, int kind, hipEvent_t a, hipEvent_t b, int64_t count) {
+    if (a && b) c->prof_pending.push_back(kmc_ctx::ProfPair{a, b, kind, count});
+    else {
+        if (a) c->prof_free.push_back(a);
+        if (b) c->prof_free.push_back(b);
+    }
+    if (c->prof_pending.size() >= 4096) prof_collect(c);   // (synchronises)
+}
+
 // w: the evaluation arrays (null: the context's own, sized here as needed for the long
 // range); with w given nothing is allocated
 static int run_propose_core(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, double* d_out, const int64_t* rate_loc,
@@ -1605,12 +1716,15 @@ static int run_propose_core(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, dou
         c->last_T = T;
     }
     const bool lr = (c->bc == KMC_BC_PBC || c->bc == KMC_BC_FF_ONLY) && c->have_lr;
+    const bool prof = !w && prof_on(c, s);   // (the no-allocation path records no events)
     if (lr) {
         int rc = w ? KMC_OK : ensure_lr_work(c, P);
         if (rc) return rc;
         rc = fork_side(c, s, 0);
         if (rc) return rc;
+        hipEvent_t e0 = prof ? prof_mark(c, c->side) : nullptr;
         c->fns.lr(lr_args(c, P, T, w), c->side);
+        if (prof) prof_add(c, 2, e0, prof_mark(c, c->side), T);
     }
     const ProposeWork wk = w ? *w : ctx_work(c);
     if (c->bc != KMC_BC_FF_ONLY) {
@@ -1618,11 +1732,19 @@ static int run_propose_core(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, dou
         // set-up and the long range overlap it)
         if (wait_sweep == 2) HIPCHK(hipStreamWaitEvent(s, c->ev_sweep, 0));
         EvalArgs an = eval_args(c, ne, wk.epos4, wk.ecell, wk.eexcl, wk.UD);
+        hipEvent_t e0 = prof ? prof_mark(c, s) : nullptr;
         launch_near_any(c, s, an, mixed, (int)P, (int)T, wk);
+        if (prof) {
+            prof_add(c, 0, e0, prof_mark(c, s), ne);
+            if (c->prof_pairs.ensure(1) == hipSuccess && ne > 0)
+                hipLaunchKernelGGL(k_pair_count, dim3((ne + 255) / 256), dim3(256), 0, s, an, c->prof_pairs.p);
+        }
         // the far field reads the leaf expansions: after the previous step's sweep
         if (wait_sweep == 1) HIPCHK(hipStreamWaitEvent(s, c->ev_sweep, 0));
         EvalArgs af = eval_args(c, ne, wk.epos4, wk.ecell, wk.eexcl, wk.UI);
+        hipEvent_t e1 = prof ? prof_mark(c, s) : nullptr;
         c->fns.far(af, s);
+        if (prof) prof_add(c, 1, e1, prof_mark(c, s), ne);
     }
     if (lr) {
         int rc = join_side(c, s, 0);
@@ -1663,15 +1785,20 @@ static int run_accept_tail(kmc_ctx* c, hipStream_t s, int64_t* d_counter, bool d
     if (rc) return rc;
     c->fns.md(ta, c->side);
     const SweepArgs sa = sweep_rec_args(c);
+    const bool prof = prof_on(c, s);
     if (defer_sweep) {
         // the sweep on side2; s waits only for the tail, and the next step's far field
         // waits for ev_sweep (the caller joins ev_sweep into s after the last step)
         HIPCHK(hipEventRecord(c->ev_fork2, s));
         HIPCHK(hipStreamWaitEvent(c->side2, c->ev_fork2, 0));
+        hipEvent_t e0 = prof ? prof_mark(c, c->side2) : nullptr;
         c->fns.sweep(sa, (int)c->ncells, c->side2, c->sweep_block, c->sweep_pairs, c->sweep_split);
+        if (prof) prof_add(c, 3, e0, prof_mark(c, c->side2), 1);
         HIPCHK(hipEventRecord(c->ev_sweep, c->side2));
     } else {
+        hipEvent_t e0 = prof ? prof_mark(c, s) : nullptr;
         c->fns.sweep(sa, (int)c->ncells, s, c->sweep_block, c->sweep_pairs, c->sweep_split);
+        if (prof) prof_add(c, 3, e0, prof_mark(c, s), 1);
     }
     rc = join_side(c, s, 1);
     if (rc) return rc;
@@ -2018,6 +2145,11 @@ int kmc_destroy(kmc_ctx* c) {
     if (c->ev_fork2) hipEventDestroy(c->ev_fork2);
     if (c->ev_sweep) hipEventDestroy(c->ev_sweep);
     if (c->side2) hipStreamDestroy(c->side2);
+    for (auto& q : c->prof_pending) {
+        hipEventDestroy(q.a);
+        hipEventDestroy(q.b);
+    }
+    for (hipEvent_t e : c->prof_free) hipEventDestroy(e);
     delete c;
     return KMC_OK;
 }
@@ -2039,6 +2171,49 @@ int kmc_set_tiles(kmc_ctx* c, int32_t near_lanes, int32_t sweep_block, int32_t s
     c->sweep_block = sweep_block;
     c->sweep_pairs = sweep_pairs;
     c->sweep_split = sweep_split;
+    return KMC_OK;
+}
+
+int kmc_profile_enable(kmc_ctx* c, int32_t on) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    int rc = prof_collect(c);
+    if (rc) return rc;
+    for (int k = 0; k < 4; k++) {
+        c->prof_ms[k] = 0.0;
+        c->prof_count[k] = 0;
+    }
+    HIPCHK(c->prof_pairs.ensure(1));
+    HIPCHK(hipMemset(c->prof_pairs.p, 0, sizeof(unsigned long long)));
+    c->prof = on != 0;
+    return KMC_OK;
+}
+
+int kmc_profile_read(kmc_ctx* c, double* h_out) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    if (!h_out) return set_err(KMC_E_ARG, "null output");
+    int rc = prof_collect(c);
+    if (rc) return rc;
+    unsigned long long pairs = 0;
+    if (c->prof_pairs.p) {
+        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipMemcpy(&pairs, c->prof_pairs.p, sizeof(pairs), hipMemcpyDeviceToHost));
+    }
+    h_out[0] = c->prof_ms[0];
+    h_out[1] = (double)c->prof_count[0];
+    h_out[2] = (double)pairs;
+    h_out[3] = c->prof_ms[1];
+    h_out[4] = (double)c->prof_count[1];
+    h_out[5] = c->prof_ms[2];
+    h_out[6] = (double)c->prof_count[2];
+    h_out[7] = c->prof_ms[3];
+    h_out[8] = (double)c->prof_count[3];
+    return KMC_OK;
+}
+
+int kmc_set_near_order(kmc_ctx* c, int32_t order) {
+    if (check_ctx(c)) return KMC_E_ARG;
+    if (order != 0 && order != 1) return set_err(KMC_E_ARG, "near order must be 0 (proposal order) or 1 (leaf tiles)");
+    c->near_order = order;
     return KMC_OK;
 }
 

@@ -1017,14 +1017,12 @@ __global__ void __launch_bounds__(512) kb_far(const EvalArgs* __restrict__ args)
 // G lanes per evaluation point, neighbour slots strided over the G lanes,
 // shuffle reduction inside the lane group.
 // ============================================================================
+// lane `lane` of the G lanes of evaluation entry i: its share of the slots of the 27
+// leaves around the entry's leaf
 template <int G, bool MIXED>
-__device__ __forceinline__ void near_body(const EvalArgs& a) {
-    const int gid = (blockIdx.x * blockDim.x + threadIdx.x);
-    const int i = gid / G;
-    const int lane = gid % G;
-    const bool active = i < a.n;
+__device__ __forceinline__ CSum near_point(const EvalArgs& a, int i, int lane) {
     CSum acc;
-    if (active) {
+    {
         const double4 p = a.pos4[i];
         const int c = a.cell[i];
         const int ex = a.excl[i];
@@ -1056,8 +1054,40 @@ __device__ __forceinline__ void near_body(const EvalArgs& a) {
             }
         }
     }
+    return acc;
+}
+template <int G, bool MIXED>
+__device__ __forceinline__ void near_body(const EvalArgs& a) {
+    const int gid = (blockIdx.x * blockDim.x + threadIdx.x);
+    const int i = gid / G;
+    const int lane = gid % G;
+    const bool active = i < a.n;
+    CSum acc;
+    if (active) acc = near_point<G, MIXED>(a, i, lane);
     if constexpr (G > 1) acc = csum_group<G>(acc);
     if (active && lane == 0) a.out[i] = acc.value() * a.pos4[i].w;
+}
+
+// The per-point near field over the entries in leaf-tile order (perm: the counting sort
+// of the entries by 4x4x4 tile of leaves).  Entry perm[i] is summed exactly as near_body
+// sums entry perm[i], so the results equal k_near<G>'s bit for bit; consecutive entries
+// share most of their 27 leaves.  Workgroups go to the XCDs round-robin (b, b + 8, ... on
+// one XCD), so block b takes logical block lb, which gives each XCD one contiguous range
+// of the sorted entries and keeps their shared leaves in that XCD's L2.
+template <int G, bool MIXED>
+__device__ __forceinline__ void near_sorted_body(const EvalArgs& a, const int* __restrict__ perm) {
+    const int nblk = (int)gridDim.x, b = (int)blockIdx.x;
+    const int q = nblk >> 3, r = nblk & 7, x = b & 7, slot = b >> 3;
+    const int lb = (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + slot;
+    const int gid = lb * (int)blockDim.x + (int)threadIdx.x;
+    const int i = gid / G;
+    const int lane = gid % G;
+    const bool active = i < a.n;
+    const int e = active ? perm[i] : 0;
+    CSum acc;
+    if (active) acc = near_point<G, MIXED>(a, e, lane);
+    if constexpr (G > 1) acc = csum_group<G>(acc);
+    if (active && lane == 0) a.out[e] = acc.value() * a.pos4[e].w;
 }
 
 // near field, one lane per neighbour leaf: 32 lanes per evaluation point, lanes
@@ -1451,7 +1481,8 @@ struct LR2Args {
     const double* gvec;
     const double4* epos4;
     const int32_t* owner;
-    double* aold;              // [12][P][NBMAX]  a_o per (block, component), row per moving charge
+    double* aold;              // [12][NBMAX][P]  a_o per (block, component), entry-major: the lanes
+                               // (moving charges / their proposals) read and write adjacent words
     double* cpart;             // [12][P]   c_o per (block, component)
     double* lpart;             // [12][T]   v.a_o + 0.5 v G v per (block, component)
 };
@@ -1484,7 +1515,7 @@ __device__ __forceinline__ void lr_block_part(const LR2Args& a, const double* __
         const double4 xo = a.epos4[a.T + p];
         double v[NB];
         lr_block_values<L, B, COMP>(xo.x, xo.y, xo.z, xo.w, v);
-        double* ao = a.aold + ((size_t)Y * a.P + p) * NBMAX;
+        double* ao = a.aold + (size_t)Y * NBMAX * a.P + p;   // entry i at ao[i * P]
         double c = 0.0;
 #pragma unroll
         for (int i = 0; i < NB; i++) {
@@ -1492,7 +1523,7 @@ __device__ __forceinline__ void lr_block_part(const LR2Args& a, const double* __
             double yy = 0.0;
 #pragma unroll
             for (int j = 0; j < NB; j++) yy = fma(G[i * NB + j], v[j], yy);
-            ao[i] = g[h] - yy;
+            ao[(size_t)i * a.P] = g[h] - yy;
             c += v[i] * fma(0.5, yy, -g[h]);
         }
         a.cpart[(size_t)Y * a.P + p] = c;
@@ -1518,11 +1549,11 @@ __device__ __forceinline__ void lr_block_prop(const LR2Args& a, const double* __
         const double4 xn = a.epos4[t];
         double v[NB];
         lr_block_values<L, B, COMP>(xn.x, xn.y, xn.z, xn.w, v);
-        const double* ao = a.aold + ((size_t)Y * a.P + p) * NBMAX;
+        const double* ao = a.aold + (size_t)Y * NBMAX * a.P + p;
         double acc = 0.0;
 #pragma unroll
         for (int i = 0; i < NB; i++) {
-            double s = fma(0.5 * G[i * NB + i], v[i], ao[i]);
+            double s = fma(0.5 * G[i * NB + i], v[i], ao[(size_t)i * a.P]);
 #pragma unroll
             for (int j = 0; j < i; j++) s = fma(G[i * NB + j], v[j], s);
             acc = fma(v[i], s, acc);

@@ -167,15 +167,55 @@ class KMCFMM(InjectorExtractor):
     def _profile_inc(self, key, inc):
         self.profile[key] = self.profile.get(key, 0.0) + inc
 
+    def profile_kernels(self, on=True):
+        """Turns the per-kernel device profile on (or off) and clears it: HIP event pairs
+        around the near field, far field, long range and accept sweep of every propose /
+        accept called from the host (none under graph capture)."""
+        _native.check(self._lib.kmc_profile_enable(self._ctx, int(bool(on))), "kmc_profile_enable")
+
+    def kernel_profile(self):
+        """The device profile as the reference's counters (coulomb_kmc/kmc_local.py:466-492,
+        kmc_octal.py:176-183, 352-358), also merged into ``profile``: seconds, counts and
+        G rates.  The near field evaluates new and old positions in one launch, so
+        c_direct_time is the reference's c_direct_new_time + c_direct_old_time and
+        c_direct_div_count its two div counts together.  Flop counts: far field 9 nh per
+        point, accept sweep 54 images x (9 nh + 6 (L - 1) + 30) per leaf (every order;
+        the blocked sweep skips truncated orders, so its rate is an upper count)."""
+        out = np.zeros(9)
+        _native.check(self._lib.kmc_profile_read(self._ctx, out.ctypes.data_as(ctypes.c_void_p)),
+                      "kmc_profile_read")
+        nh = self.L * (self.L + 1) // 2
+        d = {
+            "c_direct_time": out[0] * 1e-3,
+            "c_direct_exec_count": int(out[1]),
+            "c_direct_div_count": int(out[2]),
+            "c_indirect_time": out[3] * 1e-3,
+            "c_indirect_exec_count": int(out[4]),
+            "c_long_range_time": out[5] * 1e-3,
+            "c_long_range_exec_count": int(out[6]),
+            "c_accept_time": out[7] * 1e-3,
+            "c_accept_exec_count": int(out[8]),
+            "c_accept_flop_count_local_create": 9 * nh + 6 * (self.L - 1) + 30,
+            "c_accept_exec_count_local_create": int(out[8]) * self.ncells * 54,
+        }
+        d["c_direct_div_sqrt_grate"] = d["c_direct_div_count"] / d["c_direct_time"] / 1e9 if out[0] > 0 else 0.0
+        d["c_indirect_gflop_rate"] = (d["c_indirect_exec_count"] * 9 * nh / d["c_indirect_time"] / 1e9
+                                      if out[3] > 0 else 0.0)
+        d["c_accept_gflop_rate"] = (d["c_accept_flop_count_local_create"] * d["c_accept_exec_count_local_create"]
+                                    / d["c_accept_time"] / 1e9 if out[7] > 0 else 0.0)
+        self.profile.update(d)
+        return d
+
     @property
     def handle(self):
         """The C-ABI context (kmc_ctx*)."""
         return self._ctx
 
-    def set_tiles(self, near_lanes=27, sweep_block=0, sweep_pairs=2, sweep_split=1):
+    def set_tiles(self, near_lanes=27, sweep_block=0, sweep_pairs=2, sweep_split=1, near_order=0):
         _native.check(self._lib.kmc_set_tiles(self._ctx, int(near_lanes), int(sweep_block), int(sweep_pairs),
                                               int(sweep_split)),
                       "kmc_set_tiles")
+        _native.check(self._lib.kmc_set_near_order(self._ctx, int(near_order)), "kmc_set_near_order")
 
     # ------------------------------------------------------------------ API
     def initialise(self):

@@ -6,6 +6,9 @@ Tunable shapes (set on a context with KMCFMM.set_tiles / kmc_set_tiles):
                27 = one lane per neighbour leaf; 0 = a moving charge's points together;
                -G = points sorted by 4x4x4 leaf tile, each tile's charges staged in LDS
                once, G lanes per point)
+  near_order   0: the per-point near field in the proposals' order; 1: over the points
+               counting-sorted by 4x4x4 leaf tile, one contiguous range per XCD (same
+               sums bit for bit)
   sweep_pairs  column pairs (m, L-1-m) per lane of the accept sweep (1, 2)
   sweep_block  leaves (threads) per workgroup of the accept sweep (64, 128, 192, 256), or
                0: fused layout, 64 leaves x all column groups per workgroup, with the
@@ -29,11 +32,12 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(_HERE, "tiles.json")
 
 NEAR_LANES = (0, 1, 4, 8, 16, 27, 32, 64, -4, -8)
+NEAR_SORTED_LANES = (4, 8, 16)   # candidates with near_order 1
 # (sweep_pairs, sweep_block, sweep_split)
 SWEEP_SHAPES = ((2, 1, 1), (1, 0, 1), (2, 0, 1), (1, 64, 1), (1, 256, 1), (2, 64, 1), (2, 256, 1), (1, 64, 3),
                 (2, 64, 3))
-DEFAULT = {"near_lanes": 27, "sweep_block": 0, "sweep_pairs": 2, "sweep_split": 1}
-KEYS = ("near_lanes", "sweep_block", "sweep_pairs", "sweep_split")
+DEFAULT = {"near_lanes": 27, "sweep_block": 0, "sweep_pairs": 2, "sweep_split": 1, "near_order": 0}
+KEYS = ("near_lanes", "sweep_block", "sweep_pairs", "sweep_split", "near_order")
 
 
 def _pow2(x):
@@ -72,18 +76,22 @@ def autotune(kmc, T, reps=10, precision="mixed", save=False, log=None):
     """Time every candidate on kmc (a propose must have run); returns (choice, timings)."""
     timings = {"near": {}, "sweep": {}}
     cur = choose_tiles(kmc.L, T, kmc.ncells)
-    for g in NEAR_LANES:
-        kmc.set_tiles(**dict(cur, near_lanes=g))
+    cands = [(g, 0) for g in NEAR_LANES]
+    if round(kmc.ncells ** (1.0 / 3.0)) % 4 == 0:
+        cands += [(g, 1) for g in NEAR_SORTED_LANES]
+    for g, o in cands:
+        kmc.set_tiles(**dict(cur, near_lanes=g, near_order=o))
         kmc.time_kernel("near", reps=2, precision=precision)
-        timings["near"][g] = kmc.time_kernel("near", reps=reps, precision=precision)
-    best_g = min(timings["near"], key=timings["near"].get)
+        timings["near"]["{}/{}".format(g, o)] = kmc.time_kernel("near", reps=reps, precision=precision)
+    best_g, best_o = (int(x) for x in min(timings["near"], key=timings["near"].get).split("/"))
     for npair, block, split in sweep_candidates(kmc.boundary_condition.value != "free_space"):
-        kmc.set_tiles(near_lanes=best_g, sweep_block=block, sweep_pairs=npair, sweep_split=split)
+        kmc.set_tiles(near_lanes=best_g, sweep_block=block, sweep_pairs=npair, sweep_split=split, near_order=best_o)
         kmc.time_kernel("sweep", reps=1)
         timings["sweep"]["{}x{}x{}".format(npair, block, split)] = kmc.time_kernel("sweep", reps=reps)
     best = min(timings["sweep"], key=timings["sweep"].get)
     npair, block, split = (int(x) for x in best.split("x"))
-    choice = {"near_lanes": int(best_g), "sweep_block": block, "sweep_pairs": npair, "sweep_split": split}
+    choice = {"near_lanes": int(best_g), "sweep_block": block, "sweep_pairs": npair, "sweep_split": split,
+              "near_order": int(best_o)}
     kmc.set_tiles(**choice)
     if log:
         log("autotune", _key(kmc.L, T, kmc.ncells), choice, timings)

@@ -366,6 +366,30 @@ int kmc_store_reserve(kmc_ctx* ctx, void* stream, int32_t slots, int32_t* h_grew
  */
 int kmc_set_tiles(kmc_ctx* ctx, int32_t near_lanes, int32_t sweep_block, int32_t sweep_pairs,
                   int32_t sweep_split);
+/*
+ * Order of the near field's evaluation entries (a tile choice, with kmc_set_tiles):
+ * 0 the proposals' order; 1 counting-sorted by 4x4x4 leaf tile, each XCD taking one
+ * contiguous range of the sorted entries, for the per-point kernels (near_lanes 1, 4, 8,
+ * 16, 32, 64; propose paths; leaf grids with a multiple of 4 per side, else order 0).
+ * The sums equal order 0's bit for bit.
+ */
+int kmc_set_near_order(kmc_ctx* ctx, int32_t order);
+
+/*
+ * Per-kernel profile of host-called proposes and accepts (the reference's counters:
+ * c_direct_new_time / c_direct_old_time / c_direct_*_div_count,
+ * coulomb_kmc/kmc_local.py:466-492; c_indirect_time / c_indirect_exec_count,
+ * kmc_octal.py:352-358; c_accept_time, kmc_octal.py:176-183).  kmc_profile_enable
+ * (on = 1 / 0) clears the sums and turns the recording on / off: HIP event pairs
+ * around the near field, far field, long range and accept sweep of every call that is
+ * not under graph capture (kmc_propose_noalloc records none), and a count of the
+ * near field's pair evaluations.  kmc_profile_read (synchronises) writes h_out[9]:
+ * near-field ms, its evaluation points, its pair evaluations; far-field ms, points;
+ * long-range ms, proposals; accept-sweep ms, sweeps.  The near field evaluates the
+ * new and old positions in one launch (the reference's direct_new and direct_old).
+ */
+int kmc_profile_enable(kmc_ctx* ctx, int32_t on);
+int kmc_profile_read(kmc_ctx* ctx, double* h_out);
 
 /* Accept sweep only, for a hop of charge q from h_old to h_new (timing / tests). */
 int kmc_sweep_hop(kmc_ctx* ctx, void* stream, double q, const double* h_old_pos, const double* h_new_pos);

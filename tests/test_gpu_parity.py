@@ -462,6 +462,37 @@ def test_tiled_near_field(bc, r, N, E):
     kmc.free()
 
 
+@pytest.mark.parametrize("bc", BCS)
+@pytest.mark.parametrize("r,N,E", [(2, 300, 6.0), (3, 300, 6.0), (5, 3000, 16.0)])
+def test_sorted_near_field(bc, r, N, E):
+    """The per-point near field over the points sorted by leaf tile (near_order 1, the
+    blocks remapped so that each XCD takes one contiguous range) equals the per-point
+    kernel in the proposals' order bit for bit, for 1, 4, 8 and 16 lanes per point (1 lane:
+    fewer than 8 workgroups; 16 lanes: a block count that is not a multiple of 8), on a
+    2-leaf grid (not tiled: order 0 runs), one tile and 8^3 tiles; and the oracle's
+    extended-precision checker to the parity bars."""
+    kmc, orc, pos, q, rng = make_pair(bc, N=N, E=E, r=r, L=8, max_move=1.0, seed=70 + r)
+    pids = rng.choice(len(q), min(200, len(q) // 2), replace=False)
+    moves = ball_moves(rng, pos, pids, 5, 1.0, E, bc)
+    # the extended-precision checker on the GPU's own leaf expansions (the oracle's
+    # independently initialised ones differ by their FP64 rounding, ~1e-13 absolute,
+    # which the relative measure's 1e-3 floor would count)
+    orc.init_from_expansions(kmc.get_expansions(), kmc.energy)
+    c = orc.propose_check(pids, np.arange(len(pids) + 1, dtype=np.int64) * 5, np.concatenate([m[1] for m in moves]),
+                          extended=True)
+    for G in (1, 4, 8, 16):
+        for prec, tol in (("fp64", 1e-10), ("mixed", 1e-8)):
+            kmc.precision = prec
+            kmc.set_tiles(near_lanes=G, near_order=0)
+            ref = np.concatenate(kmc.propose(moves))
+            kmc.set_tiles(near_lanes=G, near_order=1)
+            g = np.concatenate(kmc.propose(moves))
+            _print("sorted near", bc, r, N, G, prec, rel_err(g - kmc.energy, c).max())
+            assert np.array_equal(g, ref)
+            assert rel_err(g - kmc.energy, c).max() <= tol
+    kmc.free()
+
+
 def test_propose_with_dats_many_scan_blocks_and_bad_site():
     """propose_with_dats over 5000 charges (the mask compaction's scan spans several
     workgroups) equals propose on the same moves; a current site outside
