@@ -105,22 +105,44 @@ def sweep_work(E, ncps, L, periodic, old, new, blocked):
     return total
 
 
-def kernel_rooflines(kt, N, ncells, L, P, T, bc, sweep_flops, sweep_bytes):
+def distinct_leaves(pts, E, ncps, periodic):
+    """(leaves holding a point, leaves in the union of the points' 27-leaf stencils)."""
+    w = E / ncps
+    c = np.clip(((np.asarray(pts) + 0.5 * E) / w).astype(np.int64), 0, ncps - 1)
+    own = np.unique(c[:, 0] + ncps * (c[:, 1] + ncps * c[:, 2]))
+    seen = np.zeros(ncps ** 3, dtype=bool)
+    ox = own % ncps
+    oy = (own // ncps) % ncps
+    oz = own // (ncps * ncps)
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                x, y, z = ox + dx, oy + dy, oz + dz
+                if periodic:
+                    x, y, z = x % ncps, y % ncps, z % ncps
+                    ok = np.ones(len(x), dtype=bool)
+                else:
+                    ok = (x >= 0) & (x < ncps) & (y >= 0) & (y < ncps) & (z >= 0) & (z < ncps)
+                seen[(x + ncps * (y + ncps * z))[ok]] = True
+    return len(own), int(seen.sum())
+
+
+def kernel_rooflines(kt, N, ncells, L, P, T, bc, sweep_flops, sweep_bytes, leaves=None):
     """Per-kernel achieved rate against the attainable one, min(FP64 peak, AI x HBM peak),
     from the algorithmic work per launch (add/mul 1, FMA 2, rsqrt 1):
     near: 10 flops per pair, 27 rho pairs per evaluation point (T + P points, rho = N /
     leaves); bytes 32 per point in + 8 out, and 32 per charge of the leaves the points'
-    stencils cover (at most all leaves; taken as min(leaves, 27 P) leaves);
-    far: 9 nh flops per point; bytes 40 per point and 16 nh per leaf holding a point
-    (taken as min(leaves, P));
+    stencils cover (leaves[1], counted on the host from a step's points);
+    far: 9 nh flops per point; bytes 40 per point and 16 nh per distinct leaf holding a
+    point (leaves[0]);
     sweep: as in the roofline line.  The long-range kernels are bound by LDS, not by
     HBM or the FP64 rate, and are reported as durations only."""
     nh = L * (L + 1) // 2
     rho = N / ncells
     ne = T + P
     work = {
-        "near": (ne * 10.0 * 27 * rho, ne * 40.0 + min(ncells, 27 * P) * rho * 32.0),
-        "far": (ne * 9.0 * nh, ne * 40.0 + min(ncells, P) * 16.0 * nh),
+        "near": (ne * 10.0 * 27 * rho, ne * 40.0 + (leaves[1] if leaves else min(ncells, 27 * P)) * rho * 32.0),
+        "far": (ne * 9.0 * nh, ne * 40.0 + (leaves[0] if leaves else min(ncells, P)) * 16.0 * nh),
         "sweep": (float(sweep_flops), float(sweep_bytes)),
     }
     if bc == "ff-only":
@@ -174,6 +196,63 @@ def host_moves(pos, pids_s, disp_s, k, E, bc):
             npos = np.where(npos >= 0.5 * E, npos - E, npos)
         moves.append((int(pid), npos))
     return moves
+
+
+def time_propose_with_dats(kmc, pids_s, disp_s, k, E, bc, reps=5):
+    """kmc_propose_with_dats at this workload's size, on device arrays [N][k] (the dats
+    form of one step's proposals: the moving charges' masks set, the others clear);
+    wall time per call (the call synchronises for its bad-move check) and the same
+    proposals through kmc_propose for comparison."""
+    import ctypes
+    import torch
+    from coulomb_kmc_amd import _native
+    N = kmc.N
+    pos, _ = kmc.get_device_particles()
+    moves = host_moves(pos, pids_s, disp_s, k, E, bc)
+    ppos = np.zeros((N, k, 3))
+    masks = np.zeros((N, k), dtype=np.int64)
+    for pid, npos in moves:
+        ppos[pid] = npos
+        masks[pid] = 1
+    dev = lambda a, t: torch.as_tensor(np.ascontiguousarray(a), dtype=t).cuda()
+    d_smc = dev(np.array([k], dtype=np.int64), torch.int64)
+    d_sites = torch.zeros(N, dtype=torch.int64, device="cuda")
+    d_ppos, d_masks = dev(ppos.reshape(-1), torch.float64), dev(masks.reshape(-1), torch.int64)
+    d_out = torch.zeros(N * k, dtype=torch.float64, device="cuda")
+    lib = _native.lib()
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    tm = ctypes.c_int64(0)
+
+    def call():
+        _native.check(lib.kmc_propose_with_dats(kmc.handle, st, k, d_smc.data_ptr(), 1, d_sites.data_ptr(),
+                                                d_ppos.data_ptr(), d_masks.data_ptr(), None, d_out.data_ptr(),
+                                                _native.PREC[kmc.precision], ctypes.byref(tm)),
+                      "kmc_propose_with_dats")
+    call()
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for _ in range(reps):
+        call()
+    torch.cuda.synchronize()
+    ms = (time.time() - t0) / reps * 1e3
+    # the same proposals through kmc_propose (device arrays, no dats set-up)
+    P = len(pids_s)
+    d_pid = dev(np.asarray(pids_s, dtype=np.int64), torch.int64)
+    d_es = dev(np.arange(P + 1, dtype=np.int64) * k, torch.int64)
+    d_np = dev(np.concatenate([m[1] for m in moves]).reshape(-1), torch.float64)
+    d_dE = torch.zeros(P * k, dtype=torch.float64, device="cuda")
+    kmc.propose_device(d_pid, d_es, d_np, d_dE)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for _ in range(reps):
+        kmc.propose_device(d_pid, d_es, d_np, d_dE)
+    torch.cuda.synchronize()
+    ms_p = (time.time() - t0) / reps * 1e3
+    same = bool(np.array_equal(d_out.view(N, k)[d_pid].cpu().numpy().reshape(-1), d_dE.cpu().numpy()))
+    return {"ms_per_call": ms, "proposals": int(tm.value), "charges": N, "max_prop": k,
+            "propose_ms_per_call": ms_p, "equal_to_propose": same,
+            "note": "wall time per call with the device synchronisation it makes; dats arrays [N][max_prop] "
+                    "on the device, the moving charges' masks set"}
 
 
 def metropolis_pick(dE, u, kT=1.0):
@@ -804,8 +883,16 @@ def main():
     roof["measured_peaks"] = peaks
     roof["frac_of_measured"] = (ach_gbs / peaks["copy_GBs"]) if hbm_bound else (ach_tflops / peaks["fp64_fma_TFLOPs"])
 
+    dats = time_propose_with_dats(kmc, pids_all[0], disp_all[0], k, E, bc)
+    log("propose_with_dats", dats)
     mixed, fp64 = results["mixed"], results["fp64"]
-    kroof = kernel_rooflines(kt, N, ncells, L, P, T, bc, sweep_flops, sweep_bytes)
+    pos_now, _ = kmc.get_device_particles()
+    mv0 = host_moves(pos_now, pids_all[0], disp_all[0], k, E, bc)
+    pts0 = np.concatenate([np.concatenate([m[1] for m in mv0]), pos_now[np.asarray(pids_all[0])]])
+    leaves = distinct_leaves(pts0, E, 2 ** (r - 1), bc != "free_space")
+    kroof = kernel_rooflines(kt, N, ncells, L, P, T, bc, sweep_flops, sweep_bytes, leaves)
+    kroof["distinct_leaves"] = {"holding_a_point": leaves[0], "in_the_stencils": leaves[1],
+                                "note": "one step's T + P evaluation points"}
     # the headline path: mixed only when both paths hold their bars against the extended
     # reference; otherwise the FP64 path (whose bar is the tighter one), or nothing
     # checked at all without the oracle (--no-cpu)
@@ -843,6 +930,7 @@ def main():
         "dE_error_mixed_vs_fp64": derr,
         "kernel_ms": kt,
         "kernel_roofline": kroof,
+        "propose_with_dats": dats,
         "initialise_s": t_init,
         "roofline": roof,
         "cpu_baseline": cpu,
