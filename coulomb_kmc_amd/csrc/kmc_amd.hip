@@ -1495,6 +1495,8 @@ static EvalArgs eval_args(kmc_ctx* c, int n, const double4* pos4, const int32_t*
     a.slot_pos = c->slot_pos.p;
     a.slot_id = c->slot_id.p;
     a.occmax = c->occmax;
+    a.ppos4 = c->ppos4.p;
+    a.np = (int)c->n;
     return a;
 }
 

@@ -943,6 +943,8 @@ struct EvalArgs {
     const double4* slot_pos;
     const int32_t* slot_id;
     int occmax;
+    const double4* ppos4;        // particle positions (x, y, z, q) by id
+    int np;                      // particles
 };
 
 // columns M1 and M2 (M2 == M1: one column) of  Re sum_l L_lm R_l^m(d)  with weight
@@ -1026,11 +1028,23 @@ __global__ void __launch_bounds__(512) kb_far(const EvalArgs* __restrict__ args)
 // leaves around the entry's leaf
 template <int G, bool MIXED>
 __device__ __forceinline__ CSum near_point(const EvalArgs& a, int i, int lane) {
+    // The excluded charge (the moving one) is found by its position: its slot holds
+    // exactly the position in ppos4, and no other charge can sit at that same point
+    // (their pair energy would be infinite), so the slots' ids are never loaded - a
+    // 4-byte id load touches as many cache lines as the 32-byte record load.
     CSum acc;
     {
         const double4 p = a.pos4[i];
         const int c = a.cell[i];
         const int ex = a.excl[i];
+        const bool has_ex = ex >= 0 && ex < a.np;
+        double ex_x = 0.0, ex_y = 0.0, ex_z = 0.0;
+        if (has_ex) {
+            const double4 e4 = a.ppos4[ex];
+            ex_x = e4.x;
+            ex_y = e4.y;
+            ex_z = e4.z;
+        }
         const int n = a.g.ncps;
         const int tx = c % n, ty = (c / n) % n, tz = c / (n * n);
         const bool periodic = a.g.bc != KMC_BC_FREE_SPACE;
@@ -1055,7 +1069,7 @@ __device__ __forceinline__ CSum near_point(const EvalArgs& a, int i, int lane) {
                 const double dx = px - q.x, dy = py - q.y, dz = pz - q.z;
                 const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
                 const double contrib = q.w * rsqrt_sel<MIXED>(r2);
-                if (ex < 0 || a.slot_id[base + s] != ex) acc.add(contrib);
+                if (!(has_ex && q.x == ex_x && q.y == ex_y && q.z == ex_z)) acc.add(contrib);
             }
         }
     }
