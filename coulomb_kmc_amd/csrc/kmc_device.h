@@ -1460,9 +1460,10 @@ struct LRTab {
     static constexpr LRBlocks T = make_lr_blocks(L);
 };
 
-// entries of block B of v = q conj(R(x)): real parts (COMP 0) or imaginary parts (COMP 1)
-template <int L, int B, int COMP>
-__device__ __forceinline__ void lr_block_values(double x, double y, double z, double q, double* v) {
+// entries of block B of v = q conj(R(x)): vr = real parts, vi = imaginary parts, from
+// one column recurrence b_l^m per m of the block's class (values bb * q Re/Im(C_m))
+template <int L, int B>
+__device__ __forceinline__ void lr_block_values2(double x, double y, double z, double q, double* vr, double* vi) {
     constexpr int C = B / 2, PAR = B % 2;
     const double r2 = fma(x, x, fma(y, y, z * z));
     const Cplx w = {x, -y};
@@ -1474,14 +1475,18 @@ __device__ __forceinline__ void lr_block_values(double x, double y, double z, do
 #pragma unroll
     for (int m = 0; m < L; m++) {
         if (m_class(m) != C) continue;
-        const double qc = q * (COMP == 0 ? Cm.re : Cm.im);
+        const double qr = q * Cm.re, qi = q * Cm.im;
         double b2 = 0.0, b1 = 0.0;
 #pragma unroll
         for (int l = m; l < L; l++) {
             const double bb = (l == m) ? kRec.s[m] : fma(kRec.a[l][m] * z, b1, -(kRec.b[l][m] * r2) * b2);
             b2 = b1;
             b1 = bb;
-            if ((l & 1) == PAR) v[k++] = bb * qc;
+            if ((l & 1) == PAR) {
+                vr[k] = bb * qr;
+                vi[k] = bb * qi;
+                k++;
+            }
         }
         Cm = cmul(Cm, step);
     }
@@ -1506,129 +1511,116 @@ struct LR2Args {
     double* lpart;             // [12][T]   v.a_o + 0.5 v G v per (block, component)
 };
 
-// the G tile of (block B, component COMP) into LDS
-template <int L, int B, int COMP>
-__device__ __forceinline__ void lr_stage_g(const double* __restrict__ gblk, double* Gs) {
-    constexpr LRBlocks T = LRTab<L>::T;
-    constexpr int NB = T.n[B];
-    const double* G = gblk + T.off[B] + COMP * NB * NB;
-    for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) Gs[e] = G[e];
-    __syncthreads();
-}
-
-// per moving charge p: a_o = g - G v_o over one component of block B, its share of c_o
-template <int L, int B, int COMP>
+// per moving charge p, both components of block B: a_o = g - G v_o, and c_o.  The
+// block's coefficients are wave-uniform: read straight from memory they become scalar
+// loads (s_load into SGPRs), one per several FMAs.
+template <int L, int B>
 __device__ __forceinline__ void lr_block_part(const LR2Args& a, const double* __restrict__ gblk,
-                                              const double* __restrict__ g, double* Gs) {
+                                              const double* __restrict__ g) {
     constexpr LRBlocks T = LRTab<L>::T;
     constexpr int NB = T.n[B];
     constexpr int NBMAX = lr_nbmax(L);
-    constexpr int Y = 2 * B + COMP;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.P) return;
     if constexpr (NB > 0) {
-        // the block's coefficients are wave-uniform: read straight from memory they
-        // become scalar loads (s_load into SGPRs), one per several FMAs
-        const double* __restrict__ G = gblk + T.off[B] + COMP * NB * NB;
-        (void)Gs;
-        const int p = blockIdx.x * blockDim.x + threadIdx.x;
-        if (p >= a.P) return;
         const double4 xo = a.epos4[a.T + p];
-        double v[NB];
-        lr_block_values<L, B, COMP>(xo.x, xo.y, xo.z, xo.w, v);
-        double* ao = a.aold + (size_t)Y * NBMAX * a.P + p;   // entry i at ao[i * P]
-        double c = 0.0;
+        double vr[NB], vi[NB];
+        lr_block_values2<L, B>(xo.x, xo.y, xo.z, xo.w, vr, vi);
 #pragma unroll
-        for (int i = 0; i < NB; i++) {
-            const int h = 2 * hidx(T.l[B][i], T.m[B][i]) + COMP;
-            double yy = 0.0;
+        for (int comp = 0; comp < 2; comp++) {
+            const double* __restrict__ G = gblk + T.off[B] + comp * NB * NB;
+            const double* v = comp == 0 ? vr : vi;
+            const int Y = 2 * B + comp;
+            double* ao = a.aold + (size_t)Y * NBMAX * a.P + p;   // entry i at ao[i * P]
+            double c = 0.0;
 #pragma unroll
-            for (int j = 0; j < NB; j++) yy = fma(G[i * NB + j], v[j], yy);
-            ao[(size_t)i * a.P] = g[h] - yy;
-            c += v[i] * fma(0.5, yy, -g[h]);
+            for (int i = 0; i < NB; i++) {
+                const int h = 2 * hidx(T.l[B][i], T.m[B][i]) + comp;
+                double yy = 0.0;
+#pragma unroll
+                for (int j = 0; j < NB; j++) yy = fma(G[i * NB + j], v[j], yy);
+                ao[(size_t)i * a.P] = g[h] - yy;
+                c += v[i] * fma(0.5, yy, -g[h]);
+            }
+            a.cpart[(size_t)Y * a.P + p] = c;
         }
-        a.cpart[(size_t)Y * a.P + p] = c;
     } else {
-        const int p = blockIdx.x * blockDim.x + threadIdx.x;
-        if (p < a.P) a.cpart[(size_t)Y * a.P + p] = 0.0;
+        a.cpart[(size_t)(2 * B) * a.P + p] = 0.0;
+        a.cpart[(size_t)(2 * B + 1) * a.P + p] = 0.0;
     }
 }
 
-// per proposal t: v_n . a_o + 0.5 v_n G v_n over one component of block B
-template <int L, int B, int COMP>
-__device__ __forceinline__ void lr_block_prop(const LR2Args& a, const double* __restrict__ gblk, double* Gs) {
+// per proposal t, both components of block B: v_n . a_o + 0.5 v_n G v_n
+template <int L, int B>
+__device__ __forceinline__ void lr_block_prop(const LR2Args& a, const double* __restrict__ gblk) {
     constexpr LRBlocks T = LRTab<L>::T;
     constexpr int NB = T.n[B];
     constexpr int NBMAX = lr_nbmax(L);
-    constexpr int Y = 2 * B + COMP;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.T) return;
     if constexpr (NB > 0) {
-        const double* __restrict__ G = gblk + T.off[B] + COMP * NB * NB;   // wave-uniform: scalar loads
-        (void)Gs;
-        const int t = blockIdx.x * blockDim.x + threadIdx.x;
-        if (t >= a.T) return;
         const int p = a.owner[t];
         const double4 xn = a.epos4[t];
-        double v[NB];
-        lr_block_values<L, B, COMP>(xn.x, xn.y, xn.z, xn.w, v);
-        const double* ao = a.aold + (size_t)Y * NBMAX * a.P + p;
-        double acc = 0.0;
+        double vr[NB], vi[NB];
+        lr_block_values2<L, B>(xn.x, xn.y, xn.z, xn.w, vr, vi);
 #pragma unroll
-        for (int i = 0; i < NB; i++) {
-            double s = fma(0.5 * G[i * NB + i], v[i], ao[(size_t)i * a.P]);
+        for (int comp = 0; comp < 2; comp++) {
+            const double* __restrict__ G = gblk + T.off[B] + comp * NB * NB;   // wave-uniform: scalar loads
+            const double* v = comp == 0 ? vr : vi;
+            const int Y = 2 * B + comp;
+            const double* ao = a.aold + (size_t)Y * NBMAX * a.P + p;
+            double acc = 0.0;
 #pragma unroll
-            for (int j = 0; j < i; j++) s = fma(G[i * NB + j], v[j], s);
-            acc = fma(v[i], s, acc);
+            for (int i = 0; i < NB; i++) {
+                double s = fma(0.5 * G[i * NB + i], v[i], ao[(size_t)i * a.P]);
+#pragma unroll
+                for (int j = 0; j < i; j++) s = fma(G[i * NB + j], v[j], s);
+                acc = fma(v[i], s, acc);
+            }
+            a.lpart[(size_t)Y * a.T + t] = acc;
         }
-        a.lpart[(size_t)Y * a.T + t] = acc;
     } else {
-        const int t = blockIdx.x * blockDim.x + threadIdx.x;
-        if (t < a.T) a.lpart[(size_t)Y * a.T + t] = 0.0;
+        a.lpart[(size_t)(2 * B) * a.T + t] = 0.0;
+        a.lpart[(size_t)(2 * B + 1) * a.T + t] = 0.0;
     }
 }
 
-template <int L, int Y>
-__device__ __forceinline__ void lr_part_dispatch(int y, const LR2Args& a, const double* gblk, const double* g,
-                                                 double* Gs) {
-    if constexpr (Y < 12) {
-        if (y == Y) lr_block_part<L, Y / 2, Y % 2>(a, gblk, g, Gs);
-        else lr_part_dispatch<L, Y + 1>(y, a, gblk, g, Gs);
+template <int L, int B>
+__device__ __forceinline__ void lr_part_dispatch(int b, const LR2Args& a, const double* gblk, const double* g) {
+    if constexpr (B < 6) {
+        if (b == B) lr_block_part<L, B>(a, gblk, g);
+        else lr_part_dispatch<L, B + 1>(b, a, gblk, g);
     }
 }
-template <int L, int Y>
-__device__ __forceinline__ void lr_prop_dispatch(int y, const LR2Args& a, const double* gblk, double* Gs) {
-    if constexpr (Y < 12) {
-        if (y == Y) lr_block_prop<L, Y / 2, Y % 2>(a, gblk, Gs);
-        else lr_prop_dispatch<L, Y + 1>(y, a, gblk, Gs);
+template <int L, int B>
+__device__ __forceinline__ void lr_prop_dispatch(int b, const LR2Args& a, const double* gblk) {
+    if constexpr (B < 6) {
+        if (b == B) lr_block_prop<L, B>(a, gblk);
+        else lr_prop_dispatch<L, B + 1>(b, a, gblk);
     }
 }
 
-// grid.y = (block, component) pair Y; the workgroup stages that G tile in LDS once and
-// its lanes (one per moving charge / proposal) read it as broadcasts
+// grid.y = block B of the operator (6); a lane per moving charge / proposal does both
+// components (real and imaginary parts) of its block
 template <int L>
 __global__ void __launch_bounds__(256) k_lr_part(LR2Args a, const double* __restrict__ gblk,
                                                  const double* __restrict__ gvec) {
-    constexpr int NBMAX = lr_nbmax(L);
-    __shared__ double Gs[NBMAX * NBMAX];
-    lr_part_dispatch<L, 0>((int)blockIdx.y, a, gblk, gvec, Gs);
+    lr_part_dispatch<L, 0>((int)blockIdx.y, a, gblk, gvec);
 }
 
 template <int L>
 __global__ void __launch_bounds__(256) k_lr_prop2(LR2Args a, const double* __restrict__ gblk) {
-    constexpr int NBMAX = lr_nbmax(L);
-    __shared__ double Gs[NBMAX * NBMAX];
-    lr_prop_dispatch<L, 0>((int)blockIdx.y, a, gblk, Gs);
+    lr_prop_dispatch<L, 0>((int)blockIdx.y, a, gblk);
 }
 template <int L>
 __global__ void __launch_bounds__(256) kb_lr_part(const LR2Args* __restrict__ args) {
-    constexpr int NBMAX = lr_nbmax(L);
-    __shared__ double Gs[NBMAX * NBMAX];
     const LR2Args& a = args[blockIdx.z];
-    lr_part_dispatch<L, 0>((int)blockIdx.y, a, a.gblk, a.gvec, Gs);
+    lr_part_dispatch<L, 0>((int)blockIdx.y, a, a.gblk, a.gvec);
 }
 template <int L>
 __global__ void __launch_bounds__(256) kb_lr_prop2(const LR2Args* __restrict__ args) {
-    constexpr int NBMAX = lr_nbmax(L);
-    __shared__ double Gs[NBMAX * NBMAX];
     const LR2Args& a = args[blockIdx.z];
-    lr_prop_dispatch<L, 0>((int)blockIdx.y, a, a.gblk, Gs);
+    lr_prop_dispatch<L, 0>((int)blockIdx.y, a, a.gblk);
 }
 
 
@@ -2217,8 +2209,8 @@ static void launch_far(EvalArgs a, hipStream_t s) {
 template <int L>
 static void launch_lr(LR2Args a, hipStream_t s) {
     if (a.T <= 0 || a.P <= 0) return;
-    hipLaunchKernelGGL(k_lr_part<L>, dim3((a.P + 255) / 256, 12), dim3(256), 0, s, a, a.gblk, a.gvec);
-    hipLaunchKernelGGL(k_lr_prop2<L>, dim3((a.T + 255) / 256, 12), dim3(256), 0, s, a, a.gblk);
+    hipLaunchKernelGGL(k_lr_part<L>, dim3((a.P + 255) / 256, 6), dim3(256), 0, s, a, a.gblk, a.gvec);
+    hipLaunchKernelGGL(k_lr_prop2<L>, dim3((a.T + 255) / 256, 6), dim3(256), 0, s, a, a.gblk);
 }
 template <int L>
 static void launch_mp(int n, int chunk, const double4* p, double* partial, int nblk, hipStream_t s) {
@@ -2249,8 +2241,8 @@ static void blaunch_far(const EvalArgs* t, BatchShape b, hipStream_t s) {
 }
 template <int L>
 static void blaunch_lr(const LR2Args* t, BatchShape b, hipStream_t s) {
-    hipLaunchKernelGGL(kb_lr_part<L>, dim3((b.max_P + 255) / 256, 12, b.n), dim3(256), 0, s, t);
-    hipLaunchKernelGGL(kb_lr_prop2<L>, dim3((b.max_T + 255) / 256, 12, b.n), dim3(256), 0, s, t);
+    hipLaunchKernelGGL(kb_lr_part<L>, dim3((b.max_P + 255) / 256, 6, b.n), dim3(256), 0, s, t);
+    hipLaunchKernelGGL(kb_lr_prop2<L>, dim3((b.max_T + 255) / 256, 6, b.n), dim3(256), 0, s, t);
 }
 // accept sweep of a group: the blocked layout over the flat (system, block) table when
 // one is given (nflat blocks), else the fused layout (grid sized for the largest system)
