@@ -839,6 +839,7 @@ def main():
     ncps = 2 ** (r - 1)
     ncells = ncps ** 3
     blocked = tiles["sweep_block"] == 1 and ncps % 4 == 0 and bc != "ff-only"
+    sweep_kernel = "k_sweep_block" if blocked else "k_sweep_fused"
     hop_rng = np.random.default_rng(cfg["seed"] + 7)
     t_sum, f_sum = 0.0, 0.0
     for _ in range(3):
@@ -864,18 +865,18 @@ def main():
                  "{:.3g})".format(ncells * nimg * sweep_flops_per_source(L)) if blocked else
                  "flops = leaves x {} images x (9 L(L+1)/2 + 6(L-1) + 30)".format(nimg))
     if hbm_bound:
-        roof = {"bound": "hbm", "kernel": "accept sweep ({})".format("k_sweep_block" if blocked else "k_sweep_fused"),
+        roof = {"bound": "hbm", "kernel": "accept sweep ({})".format(sweep_kernel),
                 "achieved": ach_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach_gbs / HBM_PEAK_GBS,
                 "fp64_TFLOPs": ach_tflops, "fp64_frac": ach_tflops / FP64_PEAK_TFLOPS}
     else:
-        roof = {"bound": "valu", "kernel": "accept sweep ({})".format("k_sweep_block" if blocked else "k_sweep_fused"),
+        roof = {"bound": "valu", "kernel": "accept sweep ({})".format(sweep_kernel),
                 "achieved": ach_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": ach_tflops / FP64_PEAK_TFLOPS, "hbm_GBs": ach_gbs, "hbm_frac": ach_gbs / HBM_PEAK_GBS}
     roof.update({"traffic": None, "launch_ms": kt["sweep"], "algorithmic_flops": sweep_flops,
                  "algorithmic_bytes": sweep_bytes, "flop_per_byte": ai,
                  "note": "no dense contraction: MFMA unused; bytes = read + write of every leaf's packed "
                          "expansion; " + flop_note})
-    traffic = tune.read_traffic(ROOT, "k_sweep_block" if blocked else "k_sweep_fused", cfg)
+    traffic = tune.read_traffic(ROOT, sweep_kernel, cfg)
     if traffic is not None:
         roof["traffic"] = traffic
     peaks = measured_peaks()

@@ -796,23 +796,17 @@ __device__ __forceinline__ void sweep_block_group(const SweepArgs& a, int tx, in
     }
     sweep_block_images<L, G, PER, 0>(a, dx0, dy0, dz0, sq, dcx, dcy, dcz, lcs, ar, ai);
 }
-// one wave's share of the blocked sweep: group G's columns of its lane's leaf, then the
-// staged, coalesced update of the block's expansions together with the other waves
-template <int L, int G, int NP, int NI, bool PER>
-__device__ __forceinline__ void sweep_block_wave(const SweepArgs& a, int tx, int ty, int tz, const int* lcs,
-                                                 double2* stage, int cell_base) {
-    constexpr int W = L + 1;
+// the scaled sums of wave G's columns for the lane's leaf, staged through LDS with the
+// other waves' (two halves of 32 leaves), added to the block's expansions with
+// coalesced loads and stores
+template <int L, int G>
+__device__ __forceinline__ void sweep_block_update(const SweepArgs& a, double (*ar)[L + 1], double (*ai)[L + 1],
+                                                   double2* stage, int cell_base) {
     constexpr int NH = nhalf(L);
     constexpr int ST = stage_stride(NH);
     const int lane = threadIdx.x;
     const int n = a.g.ncps;
-    double ar[NP][W], ai[NP][W];
-#pragma unroll
-    for (int j = 0; j < NP; j++)
-#pragma unroll
-        for (int k = 0; k < W; k++) { ar[j][k] = 0.0; ai[j][k] = 0.0; }
     using PP = BlockPairs<L, G>;
-    sweep_block_group<L, G, NP, NI, PER>(a, tx, ty, tz, lcs, ar, ai);
     sweep_group_stage<L, PP::A, 1, true>(nullptr, ar, ai);
     if constexpr (PP::B > PP::A) sweep_group_stage<L, PP::B, 1, true>(nullptr, ar + 1, ai + 1);
     const int tid = threadIdx.x + 64 * threadIdx.y;
@@ -853,6 +847,21 @@ __device__ __forceinline__ void sweep_block_wave(const SweepArgs& a, int tx, int
         }
     }
 }
+
+// one wave's share of the blocked sweep: group G's columns of its lane's leaf, then the
+// staged, coalesced update of the block's expansions together with the other waves
+template <int L, int G, int NP, int NI, bool PER>
+__device__ __forceinline__ void sweep_block_wave(const SweepArgs& a, int tx, int ty, int tz, const int* lcs,
+                                                 double2* stage, int cell_base) {
+    constexpr int W = L + 1;
+    double ar[NP][W], ai[NP][W];
+#pragma unroll
+    for (int j = 0; j < NP; j++)
+#pragma unroll
+        for (int k = 0; k < W; k++) { ar[j][k] = 0.0; ai[j][k] = 0.0; }
+    sweep_block_group<L, G, NP, NI, PER>(a, tx, ty, tz, lcs, ar, ai);
+    sweep_block_update<L, G>(a, ar, ai, stage, cell_base);
+}
 template <int L, int NP, int NI, bool PER, int G0>
 __device__ __forceinline__ void sweep_block_dispatch(const SweepArgs& a, int group, int tx, int ty, int tz,
                                                      const int* lcs, double2* stage, int cell_base) {
@@ -861,6 +870,30 @@ __device__ __forceinline__ void sweep_block_dispatch(const SweepArgs& a, int gro
         if (group == G0) sweep_block_wave<L, G0, NP, NI, PER>(a, tx, ty, tz, lcs, stage, cell_base);
         else sweep_block_dispatch<L, NP, NI, PER, G0 + 1>(a, group, tx, ty, tz, lcs, stage, cell_base);
     }
+}
+
+// truncation order of source image k (both sources) against the 4x4x4 block (bx, by, bz):
+// the smallest order whose tail bound is below SWEEP_TAU for every leaf of the block
+template <int L, bool PER>
+__device__ __forceinline__ int sweep_block_lc(const SweepArgs& a, int bx, int by, int bz, int k) {
+    const int R = PER ? 1 : 0;
+    const int ox = k % (2 * R + 1) - R, oy = (k / (2 * R + 1)) % (2 * R + 1) - R,
+              oz = k / ((2 * R + 1) * (2 * R + 1)) - R;
+    const double w = a.g.w;
+    const double rho = 0.8660254037844386 * w;   // half diagonal of a leaf
+    const double hb = 1.5 * w;                    // the 4 leaf centres along an axis span 3w
+    const double mx = a.g.c0 + (4 * bx + 1.5) * w, my = a.g.c0 + (4 * by + 1.5) * w,
+                 mz = a.g.c0 + (4 * bz + 1.5) * w;
+    int lc = 0;
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        const double* p = s == 0 ? a.rec->old_pos : a.rec->new_pos;
+        const double ex = fmax(fabs(p[0] + ox * a.g.E - mx) - hb, 0.0);
+        const double ey = fmax(fabs(p[1] + oy * a.g.E - my) - hb, 0.0);
+        const double ez = fmax(fabs(p[2] + oz * a.g.E - mz) - hb, 0.0);
+        lc = max(lc, sweep_trunc_order(sqrt(fma(ex, ex, fma(ey, ey, ez * ez))), rho, L));
+    }
+    return lc;
 }
 
 // grid: one workgroup per 4x4x4 block of leaves (ncps a multiple of 4), 64 x NG threads
@@ -877,31 +910,12 @@ __device__ __forceinline__ void sweep_block_body(const SweepArgs& a, int b) {
     const int bx = b % nb, by = (b / nb) % nb, bz = b / (nb * nb);
     const int tx = 4 * bx + (lane & 3), ty = 4 * by + ((lane >> 2) & 3), tz = 4 * bz + (lane >> 4);
     if (!a.rec->valid) return;   // (uniform over the grid: nothing to add)
-    if (group == 0 && lane < NIMG) {
-        // truncation order of image `lane` of both sources against this block
-        const int R = PER ? 1 : 0;
-        const int ox = lane % (2 * R + 1) - R, oy = (lane / (2 * R + 1)) % (2 * R + 1) - R,
-                  oz = lane / ((2 * R + 1) * (2 * R + 1)) - R;
-        const double w = a.g.w;
-        const double rho = 0.8660254037844386 * w;   // half diagonal of a leaf
-        const double hb = 1.5 * w;                    // the 4 leaf centres along an axis span 3w
-        const double mx = a.g.c0 + (4 * bx + 1.5) * w, my = a.g.c0 + (4 * by + 1.5) * w,
-                     mz = a.g.c0 + (4 * bz + 1.5) * w;
-        int lc = 0;
-#pragma unroll
-        for (int s = 0; s < 2; s++) {
-            const double* p = s == 0 ? a.rec->old_pos : a.rec->new_pos;
-            const double ex = fmax(fabs(p[0] + ox * a.g.E - mx) - hb, 0.0);
-            const double ey = fmax(fabs(p[1] + oy * a.g.E - my) - hb, 0.0);
-            const double ez = fmax(fabs(p[2] + oz * a.g.E - mz) - hb, 0.0);
-            lc = max(lc, sweep_trunc_order(sqrt(fma(ex, ex, fma(ey, ey, ez * ez))), rho, L));
-        }
-        s_lc[lane] = lc;
-    }
+    if (group == 0 && lane < NIMG) s_lc[lane] = sweep_block_lc<L, PER>(a, bx, by, bz, lane);
     __syncthreads();
     const int cell_base = 4 * bx + n * (4 * by + n * 4 * bz);
     sweep_block_dispatch<L, NP, NI, PER, 0>(a, group, tx, ty, tz, s_lc, stage, cell_base);
 }
+
 template <int L, int NP, int NI, bool PER>
 __global__ void __launch_bounds__(512, 2) k_sweep_block(SweepArgs a) {
     sweep_block_body<L, NP, NI, PER>(a, (int)blockIdx.x);
@@ -915,6 +929,7 @@ __global__ void __launch_bounds__(512, 2) kb_sweep_block(const SweepArgs* __rest
     const int2 e = tab[blockIdx.x];
     sweep_block_body<L, NP, NI, PER>(args[e.x], e.y);
 }
+
 
 template <int L, int NP, bool PER>
 __global__ void __launch_bounds__(512, ((NP == 1 && L <= 16) ? 4 : 2)) k_sweep_fused(SweepArgs a) {
