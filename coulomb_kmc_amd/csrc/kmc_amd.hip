@@ -55,9 +55,13 @@ __global__ void k_occ_peak(const int* __restrict__ occ, int ncells, int* __restr
 // coulomb_kmc/kmc_local.py:796-817) into LDS once, then evaluates every point of the
 // tile from there (G lanes per point, the point's 27 leaves in stencil order, the
 // slots of a leaf strided over the lanes, compensated sums): each record is read from
-// memory once per tile instead of once per point whose stencil holds it.  A tile whose
-// leaves hold more records than the LDS takes evaluates its points from memory.
-constexpr int NT_THREADS = 512;
+// memory once per tile instead of once per point whose stencil holds it.  The halo is
+// staged leaf by leaf in x-fastest order, so a point's 27 leaves are 9 runs of 3
+// consecutive leaves along x, each one contiguous range of the staged records, which
+// the point's G lanes stride over.  A tile whose leaves hold more records than the LDS
+// takes evaluates its points from memory.  Workgroups of 1024 threads: two fit on a
+// CU (the LDS), 32 waves.
+constexpr int NT_THREADS = 1024;
 constexpr int NT_SLOTS = 2048;   // LDS records per tile (36 B each)
 constexpr int NT_HALO = 216;     // 6^3 leaves
 
@@ -65,11 +69,15 @@ __device__ __forceinline__ int tile_of_cell(int c, int n) {
     const int nb = n >> 2;
     return ((c % n) >> 2) + nb * ((((c / n) % n) >> 2) + nb * ((c / (n * n)) >> 2));
 }
+// The per-tile counters (counts, cursors) are TSTRIDE ints apart, one per 128-byte
+// line: every evaluation entry increments its tile's counter atomically, and counters
+// packed into a few lines put those atomics on a few memory channels.
+constexpr int TSTRIDE = 32;
 __global__ void k_tile_count(int ne, const int32_t* __restrict__ cell, int n, int* __restrict__ tcount) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x < ne) atomicAdd(&tcount[tile_of_cell(cell[x], n)], 1);
+    if (x < ne) atomicAdd(&tcount[tile_of_cell(cell[x], n) * TSTRIDE], 1);
 }
-// exclusive scan of tcount[nt] into toff[0..nt] and tcur[0..nt); tcount back to zero
+// exclusive scan of the counts into toff[0..nt] and the cursors; the counts back to zero
 __global__ void __launch_bounds__(1024) k_tile_scan(int nt, int* __restrict__ tcount, int* __restrict__ toff,
                                                     int* __restrict__ tcur) {
     __shared__ int sums[1024];
@@ -77,7 +85,7 @@ __global__ void __launch_bounds__(1024) k_tile_scan(int nt, int* __restrict__ tc
     const int chunk = (nt + nth - 1) / nth;
     const int b0 = min(nt, tid * chunk), b1 = min(nt, b0 + chunk);
     int s = 0;
-    for (int i = b0; i < b1; i++) s += tcount[i];
+    for (int i = b0; i < b1; i++) s += tcount[i * TSTRIDE];
     sums[tid] = s;
     __syncthreads();
     for (int o = 1; o < nth; o <<= 1) {
@@ -89,16 +97,16 @@ __global__ void __launch_bounds__(1024) k_tile_scan(int nt, int* __restrict__ tc
     int c = tid > 0 ? sums[tid - 1] : 0;
     for (int i = b0; i < b1; i++) {
         toff[i] = c;
-        tcur[i] = c;
-        c += tcount[i];
-        tcount[i] = 0;
+        tcur[i * TSTRIDE] = c;
+        c += tcount[i * TSTRIDE];
+        tcount[i * TSTRIDE] = 0;
     }
     if (tid == nth - 1) toff[nt] = sums[nth - 1];
 }
 __global__ void k_tile_scatter(int ne, const int32_t* __restrict__ cell, int n, int* __restrict__ tcur,
                                int* __restrict__ perm) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x < ne) perm[atomicAdd(&tcur[tile_of_cell(cell[x], n)], 1)] = x;
+    if (x < ne) perm[atomicAdd(&tcur[tile_of_cell(cell[x], n) * TSTRIDE], 1)] = x;
 }
 
 // one point from memory (tiles whose leaves overflow the LDS): near_body's loop
@@ -211,12 +219,18 @@ __global__ void __launch_bounds__(NT_THREADS) k_near_tile(EvalArgs a, const int*
             const int c = a.cell[e];
             const int ex = a.excl[e];
             const int lx = c % n - 4 * bx, ly = (c / n) % n - 4 * by, lz = c / (n * n) - 4 * bz;
-            for (int k = 0; k < 27; k++) {
-                int ox, oy, oz;
-                stencil(k, ox, oy, oz);
-                const int h = (lx + ox + 1) + 6 * ((ly + oy + 1) + 6 * (lz + oz + 1));
-                const int s1 = s_start[h + 1];
-                for (int j = s_start[h] + lane; j < s1; j += G) {
+            // the 9 x-runs (leaves lx-1 .. lx+1 of halo row (ly+oy, lz+oz)): their ranges first
+            int r0[9], r1[9];
+#pragma unroll
+            for (int k = 0; k < 9; k++) {
+                const int h = lx + 6 * ((ly + k % 3) + 6 * (lz + k / 3));
+                r0[k] = s_start[h];
+                r1[k] = s_start[h + 3];
+            }
+#pragma unroll
+            for (int k = 0; k < 9; k++) {
+                const int e1 = r1[k];
+                for (int j = r0[k] + lane; j < e1; j += G) {
                     const double4 r = s_pos[j];
                     const double dx = p.x - r.x, dy = p.y - r.y, dz = p.z - r.z;
                     const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
@@ -1404,11 +1418,11 @@ static int ensure_eval(kmc_ctx* c, int64_t T, int64_t P) {
     HIPCHK(c->es_copy.ensure((size_t)P + 1));
     const int nt = near_tiles(c);
     if (nt > 0) {
-        const bool fresh = c->tcount.n < (size_t)nt;
-        HIPCHK(c->tcount.ensure(nt));
-        if (fresh) HIPCHK(hipMemset(c->tcount.p, 0, sizeof(int) * nt));   // the scan keeps it zero after
+        const bool fresh = c->tcount.n < (size_t)nt * TSTRIDE;
+        HIPCHK(c->tcount.ensure((size_t)nt * TSTRIDE));
+        if (fresh) HIPCHK(hipMemset(c->tcount.p, 0, sizeof(int) * nt * TSTRIDE));   // the scan keeps it zero after
         HIPCHK(c->toff.ensure((size_t)nt + 1));
-        HIPCHK(c->tcur.ensure(nt));
+        HIPCHK(c->tcur.ensure((size_t)nt * TSTRIDE));
         HIPCHK(c->perm.ensure(ne));
     }
     return KMC_OK;
@@ -2041,9 +2055,9 @@ static size_t carve_work(const kmc_ctx* c, int64_t P, int64_t T, unsigned char* 
     k.es_copy = (int64_t*)take((size_t)(P + 1) * sizeof(int64_t));
     const int nt = near_tiles(c);
     if (nt > 0) {
-        k.tcount = (int*)take((size_t)nt * sizeof(int));
+        k.tcount = (int*)take((size_t)nt * TSTRIDE * sizeof(int));
         k.toff = (int*)take((size_t)(nt + 1) * sizeof(int));
-        k.tcur = (int*)take((size_t)nt * sizeof(int));
+        k.tcur = (int*)take((size_t)nt * TSTRIDE * sizeof(int));
         k.perm = (int*)take(ne * sizeof(int));
     }
     if (uses_lr(c)) {
@@ -2640,7 +2654,7 @@ int kmc_propose_noalloc(kmc_ctx* c, void* stream, int64_t P, int64_t T, const in
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, w.err, 1, 0);
     if (w.tcount) {
         const int nt = near_tiles(c);
-        hipLaunchKernelGGL(k_fill_i32, dim3((nt + 255) / 256), dim3(256), 0, s, w.tcount, nt, 0);
+        hipLaunchKernelGGL(k_fill_i32, dim3((nt * TSTRIDE + 255) / 256), dim3(256), 0, s, w.tcount, nt * TSTRIDE, 0);
     }
     rc = launch_setup(c, s, P, T, d_pid, d_es, d_new_pos, d_new_q, &w);
     if (rc) return rc;
