@@ -1514,9 +1514,13 @@ static EvalArgs eval_args(kmc_ctx* c, int n, const double4* pos4, const int32_t*
     return a;
 }
 
+// row chunks of each operator block in k_lr_part: with few moving charges (under 4096)
+// one lane per (charge, block) leaves most of the device idle
+static int lr_split(int64_t P, const ProposeWork* = nullptr) { return P < 4096 ? LR_SPLIT_MAX : 1; }
+
 static int ensure_lr_work(kmc_ctx* c, int64_t P) {
     HIPCHK(c->aold.ensure((size_t)12 * P * lr_nbmax(c->L)));
-    HIPCHK(c->cold.ensure((size_t)12 * P));
+    HIPCHK(c->cold.ensure((size_t)12 * P * lr_split(P, nullptr)));
     return KMC_OK;
 }
 
@@ -1531,6 +1535,7 @@ static LR2Args lr_args(kmc_ctx* c, int64_t P, int64_t T, const ProposeWork* w = 
     la.owner = wk.owner;
     la.aold = wk.aold;
     la.cpart = wk.cold;
+    la.rsplit = lr_split(P, w);
     la.lpart = wk.lrd;
     return la;
 }
@@ -1582,6 +1587,7 @@ static CombineArgs combine_args(kmc_ctx* c, int64_t P, int64_t T, double* d_out,
     ca.nfar = (c->L + 1) / 2;
     ca.lpart = lr ? wk.lrd : nullptr;
     ca.cpart = lr ? wk.cold : nullptr;
+    ca.ncpart = 12 * lr_split(P, w);
     ca.P = (int)P;
     ca.rate_loc = rate_loc;
     ca.out = d_out;
@@ -1630,7 +1636,8 @@ static SweepArgs sweep_rec_args(kmc_ctx* c) {
 // work buffers sized for P moving charges and T proposals (a step allocates nothing)
 static bool step_sized(const kmc_ctx* c, int64_t P, int64_t T) {
     const bool lr_sized =
-        !c->have_lr || ((size_t)12 * P * lr_nbmax(c->L) <= c->aold.n && (size_t)12 * P <= c->cold.n);
+        !c->have_lr || ((size_t)12 * P * lr_nbmax(c->L) <= c->aold.n &&
+                        (size_t)12 * P * lr_split(P, nullptr) <= c->cold.n);
     return (size_t)(T + P) <= c->epos4.n && (size_t)T <= c->owner.n && (size_t)T <= c->rates.n &&
            (size_t)((T + 63) / 64) <= c->bsum.n && (size_t)(P + 1) <= c->es_copy.n && lr_sized;
 }
@@ -2063,7 +2070,7 @@ static size_t carve_work(const kmc_ctx* c, int64_t P, int64_t T, unsigned char* 
     if (uses_lr(c)) {
         k.lrd = (double*)take((size_t)12 * T * sizeof(double));
         k.aold = (double*)take((size_t)12 * P * lr_nbmax(c->L) * sizeof(double));
-        k.cold = (double*)take((size_t)12 * P * sizeof(double));
+        k.cold = (double*)take((size_t)12 * P * lr_split(P) * sizeof(double));
     }
     if (w) *w = k;
     return off;
@@ -2952,7 +2959,7 @@ int kmc_time_kernel(kmc_ctx* c, void* stream, int32_t which, int32_t reps, int32
             EvalArgs af = eval_args(c, (int)(T + P), c->epos4.p, c->ecell.p, c->eexcl.p, c->UI.p);
             c->fns.far(af, s);
         } else if (which == 3) {
-            if (c->have_lr && (size_t)(12 * P) <= c->cold.n) c->fns.lr(lr_args(c, P, T), s);
+            if (c->have_lr && (size_t)12 * P * lr_split(P, nullptr) <= c->cold.n) c->fns.lr(lr_args(c, P, T), s);
         } else {
             return set_err(KMC_E_ARG, "unknown kernel id");
         }

@@ -1302,7 +1302,8 @@ struct CombineArgs {
     const double* UI;            // far-field partials [nfar][T+P]
     int nfar;
     const double* lpart;         // long range per proposal partials [12][T], or null
-    const double* cpart;         // long range per moving charge partials [12][P]
+    const double* cpart;         // long range per moving charge partials [ncpart][P]
+    int ncpart;                  // 12 x the long range's row chunks
     int P;
     const int64_t* rate_loc;     // output index, or null (identity)
     double* out;
@@ -1367,7 +1368,7 @@ __device__ __forceinline__ double combine_one(const CombineArgs& a, int t) {
     tot.add(-q2 * e27.c);
     if (a.lpart) {
         add_parts(tot, 1.0, a.lpart, 12, a.T, t);
-        add_parts(tot, 1.0, a.cpart, 12, a.P, ow);
+        add_parts(tot, 1.0, a.cpart, a.ncpart, a.P, ow);
     }
     const double de = a.eu * tot.value();
     const int64_t o = a.rate_loc ? a.rate_loc[t] : (int64_t)t;
@@ -1522,22 +1523,28 @@ struct LR2Args {
     const int32_t* owner;
     double* aold;              // [12][NBMAX][P]  a_o per (block, component), entry-major: the lanes
                                // (moving charges / their proposals) read and write adjacent words
-    double* cpart;             // [12][P]   c_o per (block, component)
+    double* cpart;             // [12][rsplit][P]   c_o per (block, component, row chunk)
+    int rsplit;                // row chunks of each block in k_lr_part (1..LR_SPLIT_MAX)
     double* lpart;             // [12][T]   v.a_o + 0.5 v G v per (block, component)
 };
 
-// per moving charge p, both components of block B: a_o = g - G v_o, and c_o.  The
+// per moving charge p, both components of block B, rows [i0, i1) of chunk r of the
+// block's a.rsplit row chunks: a_o = g - G v_o, and the chunk's share of c_o.  With few
+// moving charges the chunks give the kernel enough lanes to fill the device.  The
 // block's coefficients are wave-uniform: read straight from memory they become scalar
 // loads (s_load into SGPRs), one per several FMAs.
+constexpr int LR_SPLIT_MAX = 4;
 template <int L, int B>
 __device__ __forceinline__ void lr_block_part(const LR2Args& a, const double* __restrict__ gblk,
-                                              const double* __restrict__ g) {
+                                              const double* __restrict__ g, int r) {
     constexpr LRBlocks T = LRTab<L>::T;
     constexpr int NB = T.n[B];
     constexpr int NBMAX = lr_nbmax(L);
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.P) return;
+    if (p >= a.P || r >= a.rsplit) return;
+    const int R = a.rsplit;
     if constexpr (NB > 0) {
+        const int i0 = r * NB / R, i1 = (r + 1) * NB / R;
         const double4 xo = a.epos4[a.T + p];
         double vr[NB], vi[NB];
         lr_block_values2<L, B>(xo.x, xo.y, xo.z, xo.w, vr, vi);
@@ -1550,6 +1557,7 @@ __device__ __forceinline__ void lr_block_part(const LR2Args& a, const double* __
             double c = 0.0;
 #pragma unroll
             for (int i = 0; i < NB; i++) {
+                if (i < i0 || i >= i1) continue;
                 const int h = 2 * hidx(T.l[B][i], T.m[B][i]) + comp;
                 double yy = 0.0;
 #pragma unroll
@@ -1557,11 +1565,11 @@ __device__ __forceinline__ void lr_block_part(const LR2Args& a, const double* __
                 ao[(size_t)i * a.P] = g[h] - yy;
                 c += v[i] * fma(0.5, yy, -g[h]);
             }
-            a.cpart[(size_t)Y * a.P + p] = c;
+            a.cpart[((size_t)Y * R + r) * a.P + p] = c;
         }
     } else {
-        a.cpart[(size_t)(2 * B) * a.P + p] = 0.0;
-        a.cpart[(size_t)(2 * B + 1) * a.P + p] = 0.0;
+        a.cpart[((size_t)(2 * B) * R + r) * a.P + p] = 0.0;
+        a.cpart[((size_t)(2 * B + 1) * R + r) * a.P + p] = 0.0;
     }
 }
 
@@ -1601,10 +1609,11 @@ __device__ __forceinline__ void lr_block_prop(const LR2Args& a, const double* __
 }
 
 template <int L, int B>
-__device__ __forceinline__ void lr_part_dispatch(int b, const LR2Args& a, const double* gblk, const double* g) {
+__device__ __forceinline__ void lr_part_dispatch(int b, int r, const LR2Args& a, const double* gblk,
+                                                 const double* g) {
     if constexpr (B < 6) {
-        if (b == B) lr_block_part<L, B>(a, gblk, g);
-        else lr_part_dispatch<L, B + 1>(b, a, gblk, g);
+        if (b == B) lr_block_part<L, B>(a, gblk, g, r);
+        else lr_part_dispatch<L, B + 1>(b, r, a, gblk, g);
     }
 }
 template <int L, int B>
@@ -1615,12 +1624,13 @@ __device__ __forceinline__ void lr_prop_dispatch(int b, const LR2Args& a, const 
     }
 }
 
-// grid.y = block B of the operator (6); a lane per moving charge / proposal does both
-// components (real and imaginary parts) of its block
+// a lane per moving charge / proposal does both components (real and imaginary parts)
+// of its block; k_lr_part: grid.y = (block B of the operator (6)) x (row chunk, rg of
+// them); k_lr_prop2: grid.y = block B
 template <int L>
 __global__ void __launch_bounds__(256) k_lr_part(LR2Args a, const double* __restrict__ gblk,
-                                                 const double* __restrict__ gvec) {
-    lr_part_dispatch<L, 0>((int)blockIdx.y, a, gblk, gvec);
+                                                 const double* __restrict__ gvec, int rg) {
+    lr_part_dispatch<L, 0>((int)blockIdx.y / rg, (int)blockIdx.y % rg, a, gblk, gvec);
 }
 
 template <int L>
@@ -1630,7 +1640,7 @@ __global__ void __launch_bounds__(256) k_lr_prop2(LR2Args a, const double* __res
 template <int L>
 __global__ void __launch_bounds__(256) kb_lr_part(const LR2Args* __restrict__ args) {
     const LR2Args& a = args[blockIdx.z];
-    lr_part_dispatch<L, 0>((int)blockIdx.y, a, a.gblk, a.gvec);
+    lr_part_dispatch<L, 0>((int)blockIdx.y / LR_SPLIT_MAX, (int)blockIdx.y % LR_SPLIT_MAX, a, a.gblk, a.gvec);
 }
 template <int L>
 __global__ void __launch_bounds__(256) kb_lr_prop2(const LR2Args* __restrict__ args) {
@@ -2224,7 +2234,8 @@ static void launch_far(EvalArgs a, hipStream_t s) {
 template <int L>
 static void launch_lr(LR2Args a, hipStream_t s) {
     if (a.T <= 0 || a.P <= 0) return;
-    hipLaunchKernelGGL(k_lr_part<L>, dim3((a.P + 255) / 256, 6), dim3(256), 0, s, a, a.gblk, a.gvec);
+    hipLaunchKernelGGL(k_lr_part<L>, dim3((a.P + 255) / 256, 6 * a.rsplit), dim3(256), 0, s, a, a.gblk, a.gvec,
+                       a.rsplit);
     hipLaunchKernelGGL(k_lr_prop2<L>, dim3((a.T + 255) / 256, 6), dim3(256), 0, s, a, a.gblk);
 }
 template <int L>
@@ -2256,7 +2267,7 @@ static void blaunch_far(const EvalArgs* t, BatchShape b, hipStream_t s) {
 }
 template <int L>
 static void blaunch_lr(const LR2Args* t, BatchShape b, hipStream_t s) {
-    hipLaunchKernelGGL(kb_lr_part<L>, dim3((b.max_P + 255) / 256, 6, b.n), dim3(256), 0, s, t);
+    hipLaunchKernelGGL(kb_lr_part<L>, dim3((b.max_P + 255) / 256, 6 * LR_SPLIT_MAX, b.n), dim3(256), 0, s, t);
     hipLaunchKernelGGL(kb_lr_prop2<L>, dim3((b.max_T + 255) / 256, 6, b.n), dim3(256), 0, s, t);
 }
 // accept sweep of a group: the blocked layout over the flat (system, block) table when
