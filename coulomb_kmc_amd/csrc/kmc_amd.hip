@@ -1745,10 +1745,10 @@ static int run_propose_core(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, dou
         if (rc) return rc;
         rc = fork_side(c, s, 0);
         if (rc) return rc;
-        hipEvent_t e0 = prof ? prof_mark(c, c->side) : nullptr;
-        c->fns.lr(lr_args(c, P, T, w), c->side);
-        if (prof) prof_add(c, 2, e0, prof_mark(c, c->side), T);
     }
+    // the near and far fields are recorded before the long range: a captured graph's
+    // first branch after a fork stays on the launching stream's queue, and the near/far
+    // chain is the step's critical path (a dependency across queues costs ~10 us)
     const ProposeWork wk = w ? *w : ctx_work(c);
     if (c->bc != KMC_BC_FF_ONLY) {
         // wait_sweep 2: the near field too waits for the previous step's sweep (only the
@@ -1770,6 +1770,9 @@ static int run_propose_core(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, dou
         if (prof) prof_add(c, 1, e1, prof_mark(c, s), ne);
     }
     if (lr) {
+        hipEvent_t e0 = prof ? prof_mark(c, c->side) : nullptr;
+        c->fns.lr(lr_args(c, P, T, w), c->side);
+        if (prof) prof_add(c, 2, e0, prof_mark(c, c->side), T);
         int rc = join_side(c, s, 0);
         if (rc) return rc;
     }
@@ -1806,7 +1809,6 @@ static int run_accept_tail(kmc_ctx* c, hipStream_t s, int64_t* d_counter, bool d
     // share the move record, which both read: they run side by side
     int rc = fork_side(c, s, 1);
     if (rc) return rc;
-    c->fns.md(ta, c->side);
     const SweepArgs sa = sweep_rec_args(c);
     const bool prof = prof_on(c, s);
     if (defer_sweep) {
@@ -1823,6 +1825,7 @@ static int run_accept_tail(kmc_ctx* c, hipStream_t s, int64_t* d_counter, bool d
         c->fns.sweep(sa, (int)c->ncells, s, c->sweep_block, c->sweep_pairs, c->sweep_split);
         if (prof) prof_add(c, 3, e0, prof_mark(c, s), 1);
     }
+    c->fns.md(ta, c->side);   // (recorded after the sweep, for the same reason as the long range)
     rc = join_side(c, s, 1);
     if (rc) return rc;
     HIPCHK(hipGetLastError());

@@ -1314,9 +1314,21 @@ struct CombineArgs {
     int rate_rule;               // 0: min(1, exp(-dE/kT)) (Metropolis), 1: exp(-dE/kT)
 };
 
-__device__ __forceinline__ void add_parts(CSum& acc, double sign, const double* v, int nparts, size_t stride,
-                                          size_t i) {
-    for (int j = 0; j < nparts; j++) acc.add(sign * v[(size_t)j * stride + i]);
+// Partial sums v[j * stride + i], j < n, loaded into registers before they are added:
+// each load of the chunk is issued before the first add, so a lane waits on memory once
+// per chunk rather than once per partial (n is a kernel argument: a plain loop would
+// wait on each load in turn).
+template <int C>
+__device__ __forceinline__ void load_parts(double (&buf)[C], const double* v, int j0, int n, size_t stride,
+                                           size_t i) {
+#pragma unroll
+    for (int j = 0; j < C; j++) buf[j] = j0 + j < n ? v[(size_t)(j0 + j) * stride + i] : 0.0;
+}
+template <int C>
+__device__ __forceinline__ void add_loaded(CSum& acc, double sign, const double (&buf)[C], int j0, int n) {
+#pragma unroll
+    for (int j = 0; j < C; j++)
+        if (j0 + j < n) acc.add(sign * buf[j]);
 }
 
 // Metropolis (rule 0) or exponential (rule 1) rate of an energy change.  An exponential
@@ -1335,10 +1347,26 @@ __device__ __forceinline__ double rate_of(double de, double kT, int rule) {
 // sum, so the old - new cancellation costs no more than one rounding of dE itself.
 template <bool MIXED>
 __device__ __forceinline__ double combine_one(const CombineArgs& a, int t) {
+    constexpr int NF = (KMAX_L + 1) / 2;      // far-field partials at most
     const int ow = a.owner[t];
     const int po = a.T + ow;
     const double4 xo = a.epos4[po];
     const double4 xn = a.epos4[t];
+    // every partial loaded up front (first 12 per-charge long-range parts included); the
+    // self term below runs while the loads are in flight
+    const bool direct = a.g.bc != KMC_BC_FF_ONLY;
+    const size_t ne = (size_t)a.T + a.P;
+    double un = 0.0, uo = 0.0, fn[NF], fo[NF], lp[12], cp[12];
+    if (direct) {
+        un = a.UD[t];
+        uo = a.UD[po];
+        load_parts(fn, a.UI, 0, a.nfar, ne, t);
+        load_parts(fo, a.UI, 0, a.nfar, ne, po);
+    }
+    if (a.lpart) {
+        load_parts(lp, a.lpart, 0, 12, a.T, t);
+        load_parts(cp, a.cpart, 0, a.ncpart, a.P, ow);
+    }
     // self term (kmc_fmm_self_interaction.py:48-54, 163-182, 227-236)
     CSum e27;
     if (a.g.bc == KMC_BC_FREE_SPACE) {
@@ -1356,19 +1384,22 @@ __device__ __forceinline__ double combine_one(const CombineArgs& a, int t) {
         }
     }
     CSum tot;
-    if (a.g.bc != KMC_BC_FF_ONLY) {
-        const size_t ne = (size_t)a.T + a.P;
-        tot.add(a.UD[t]);
-        add_parts(tot, 1.0, a.UI, a.nfar, ne, t);
-        tot.add(-a.UD[po]);
-        add_parts(tot, -1.0, a.UI, a.nfar, ne, po);
+    if (direct) {
+        tot.add(un);
+        add_loaded(tot, 1.0, fn, 0, a.nfar);
+        tot.add(-uo);
+        add_loaded(tot, -1.0, fo, 0, a.nfar);
     }
     const double q2 = xo.w * xo.w;
     tot.add_prod(-q2, e27.s);
     tot.add(-q2 * e27.c);
     if (a.lpart) {
-        add_parts(tot, 1.0, a.lpart, 12, a.T, t);
-        add_parts(tot, 1.0, a.cpart, a.ncpart, a.P, ow);
+        add_loaded(tot, 1.0, lp, 0, 12);
+        add_loaded(tot, 1.0, cp, 0, a.ncpart);
+        for (int j0 = 12; j0 < a.ncpart; j0 += 12) {   // row-split long range: 12 per chunk
+            load_parts(cp, a.cpart, j0, a.ncpart, a.P, ow);
+            add_loaded(tot, 1.0, cp, j0, a.ncpart);
+        }
     }
     const double de = a.eu * tot.value();
     const int64_t o = a.rate_loc ? a.rate_loc[t] : (int64_t)t;
@@ -1813,7 +1844,12 @@ __device__ __forceinline__ void select_body(const SelectArgs& a) {
     const int chunk = (nb + nt - 1) / nt;
     const int b0 = min(nb, tid * chunk), b1 = min(nb, b0 + chunk);
     double s = 0.0;
-    for (int b = b0; b < b1; b++) s += a.bsum[b];
+    for (int b = b0; b < b1; b += 4) {
+        double v4[4];
+        load_parts(v4, a.bsum, b, b1, 1, 0);
+        for (int j = 0; j < 4; j++)
+            if (b + j < b1) s += v4[j];
+    }
     if (tid == 0) pick = -1;
     double v = s;
 #pragma unroll
@@ -1845,19 +1881,32 @@ __device__ __forceinline__ void select_body(const SelectArgs& a) {
     const double total = wlast[nw - 1];
     const double target = a.uniform[step] * total;
     if (target >= lo && target < incl && b1 > b0) {
+        // the same running sum as one add per block sum, then one per rate, with the
+        // loads issued 4 block sums / 8 rates at a time
         double c = lo;
         int k = -1;
-        for (int b = b0; b < b1 && k < 0; b++) {
-            const double cb = c + a.bsum[b];
-            if (target < cb || b == b1 - 1) {
-                const int t0 = b * 64, t1 = min(a.T, t0 + 64);
-                for (int t = t0; t < t1; t++) {
-                    c += a.rates[t];
-                    if (target < c) { k = t; break; }
+        for (int g = b0; g < b1 && k < 0; g += 4) {
+            double v4[4];
+            load_parts(v4, a.bsum, g, b1, 1, 0);
+            for (int j = 0; j < 4 && k < 0; j++) {
+                const int b = g + j;
+                if (b >= b1) break;
+                const double cb = c + v4[j];
+                if (target < cb || b == b1 - 1) {
+                    const int t0 = b * 64, t1 = min(a.T, t0 + 64);
+                    for (int u = t0; u < t1 && k < 0; u += 8) {
+                        double r8[8];
+                        load_parts(r8, a.rates, u, t1, 1, 0);
+                        for (int i = 0; i < 8; i++) {
+                            if (u + i >= t1) break;
+                            c += r8[i];
+                            if (target < c) { k = u + i; break; }
+                        }
+                    }
+                    if (k < 0) k = t1 - 1;
+                } else {
+                    c = cb;
                 }
-                if (k < 0) k = t1 - 1;
-            } else {
-                c = cb;
             }
         }
         pick = k;
@@ -1885,22 +1934,28 @@ __device__ __forceinline__ void select_body(const SelectArgs& a) {
         MoveRec* r = a.rec;
         a.selected[step] = k;
         if (k < 0) { r->valid = 0; atomicOr(a.flags, 4); return; }
+        // every load before the first store to the record (the compiler cannot move a
+        // load above a store that may alias it)
         const int p = a.owner[k];
+        const double4 np = a.epos4[k];
+        const int new_cell = a.ecell[k];
+        const double dE = a.dE[k];
         const int pid = (int)a.pid[p];
         if (pid < 0 || pid >= a.n) { r->valid = 0; atomicOr(a.flags, 8); return; }
-        const double4 np = a.epos4[k];
         const double4 op = a.ppos4[pid];
+        const int old_cell = a.pcell[pid];
+        const int occ_new = a.occ[new_cell];
         r->old_pos[0] = op.x; r->old_pos[1] = op.y; r->old_pos[2] = op.z;
         r->new_pos[0] = np.x; r->new_pos[1] = np.y; r->new_pos[2] = np.z;
         r->q = op.w;
         r->pid = pid;
-        r->old_cell = a.pcell[pid];
-        r->new_cell = a.ecell[k];
-        r->dE = a.dE[k];
-        const bool full = (r->new_cell != r->old_cell) && (a.occ[r->new_cell] >= a.occmax);
+        r->old_cell = old_cell;
+        r->new_cell = new_cell;
+        r->dE = dE;
+        const bool full = (new_cell != old_cell) && (occ_new >= a.occmax);
         if (full) { r->valid = 0; atomicOr(a.flags, 2); return; }
         r->valid = 1;
-        *a.energy += a.dE[k];
+        *a.energy += dE;
     }
 }
 
