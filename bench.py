@@ -62,13 +62,29 @@ def sweep_flops_per_source(L):
 SWEEP_TAU = 2.0 ** -60
 
 
+SWEEP_BUCKETS = (8, 9, 10, 12)
+
+
+def sweep_bucket_of(lc, L):
+    """Orders the blocked sweep evaluates for an image of truncation order lc (its
+    compile-time bucket; 0 for lc = 0)."""
+    out = np.full(np.shape(lc), L)
+    for b in reversed(SWEEP_BUCKETS):
+        if b < L:
+            out = np.where(lc <= b, b, out)
+    return np.where(np.asarray(lc) <= 0, 0, out)
+
+
 def sweep_work(E, ncps, L, periodic, old, new, blocked):
-    """FP64 flops the accept sweep does for the hop old -> new: over every (leaf, source
-    image) pair that is well separated (the others add nothing), sweep_flops_per_source
-    of the image's order: L, or with the blocked layout (sweep_block 1) the truncation
-    order lc of the image against the leaf's 4x4x4 block (the smallest lc with
-    (rho/D)^lc / (D - rho) <= 2^-60, D the distance from the image to the block's leaf
-    centres, rho a leaf's half diagonal), the rule the kernel applies."""
+    """(FP64 flops, HBM bytes) of the accept sweep for the hop old -> new.  Flops: over
+    every (leaf, source image) pair that is well separated (the others add nothing),
+    sweep_flops_per_source of the image's order: L, or with the blocked layout
+    (sweep_block 1) the truncation order lc of the image against the leaf's 4x4x4 block
+    (the smallest lc with (rho/D)^lc / (D - rho) <= 2^-60, D the distance from the image
+    to the block's leaf centres, rho a leaf's half diagonal), the rule the kernel
+    applies.  Bytes: read + write of each leaf's packed expansion, all L(L+1)/2 entries,
+    or with the blocked layout the entries of the orders below the bucket of the block's
+    largest lc (the others receive no term and are not touched)."""
     w = E / ncps
     c0 = -0.5 * E + 0.5 * w
     rho = 0.5 * np.sqrt(3.0) * w
@@ -76,6 +92,7 @@ def sweep_work(E, ncps, L, periodic, old, new, blocked):
     tx, ty, tz = np.meshgrid(ar, ar, ar, indexing="ij")
     R = 1 if periodic else 0
     total = 0.0
+    lcmax = np.zeros(tx.shape, int)
     for src in (old, new):
         src = np.asarray(src, dtype=np.float64)
         sc = np.minimum(((src + 0.5 * E) / w).astype(int), ncps - 1)
@@ -89,7 +106,7 @@ def sweep_work(E, ncps, L, periodic, old, new, blocked):
                         e = [np.maximum(np.abs(s[d] - (c0 + (4 * (t // 4) + 1.5) * w)) - 1.5 * w, 0.0)
                              for d, t in enumerate((tx, ty, tz))]
                         D = np.sqrt(e[0] ** 2 + e[1] ** 2 + e[2] ** 2)
-                        far = D > 2 * rho
+                        far = D > 3 * rho
                         Ds = np.where(far, D, 1.0)
                         bound = np.where(far, 1.0 / np.where(far, Ds - rho, 1.0), np.inf)
                         lc = np.zeros(D.shape, int)
@@ -100,9 +117,11 @@ def sweep_work(E, ncps, L, periodic, old, new, blocked):
                         lc = np.where(far, lc, L)
                     else:
                         lc = np.full(tx.shape, L)
+                    lcmax = np.maximum(lcmax, lc)
                     f = 9 * lc * (lc + 1) // 2 + 6 * (lc - 1) + 30
                     total += float(np.sum(np.where(near, 0, f)))
-    return total
+    nb = sweep_bucket_of(lcmax, L) if blocked else np.full(tx.shape, L)
+    return total, float(np.sum(2 * 16 * (nb * (nb + 1) // 2)))
 
 
 def distinct_leaves(pts, E, ncps, periodic):
@@ -276,6 +295,17 @@ def measured_peaks(nbytes=2 ** 30, reps=10):
     return {"copy_GBs": out[0], "copy_bytes": nbytes, "fp64_fma_TFLOPs": out[1]}
 
 
+def probe_hop(cfg):
+    """The hop of a unit charge whose accept sweep the traffic probe measures (seeded)."""
+    E = cfg["E"]
+    rng = np.random.default_rng(cfg["seed"] + 11)
+    old = rng.uniform(-0.5 * E, 0.5 * E, 3)
+    new = old + np.array([0.6, -0.5, 0.4]) * cfg["radius"]
+    if cfg["bc"] == "free_space":
+        return old, np.clip(new, -0.5 * E, 0.5 * E)
+    return old, np.where(new < -0.5 * E, new + E, np.where(new >= 0.5 * E, new - E, new))
+
+
 def probe_traffic(kmc, cfg, pids_all, disp_all):
     """Launches for the HBM-traffic counters: one propose (sizes the evaluation arrays),
     5 accept sweeps, then 5 device copies of COPY_PROBE_BYTES (calibration of the
@@ -294,6 +324,8 @@ def probe_traffic(kmc, cfg, pids_all, disp_all):
     kmc.propose_device(d_pid, d_es, d_np, d_dE, precision="mixed")    # sizes the work buffers
     kmc.step_device(d_pid, d_es, d_np, d_dE, torch.full((1,), 0.5, dtype=torch.float64, device="cuda"),
                     1.0, counter, torch.zeros(1, dtype=torch.int64, device="cuda"), precision="mixed")
+    old, new = probe_hop(cfg)
+    kmc.sweep_hop(1.0, old, new)   # the move the timed sweeps re-apply
     torch.cuda.synchronize()
     ms = kmc.time_kernel("sweep", reps=5)
     a = torch.ones(COPY_PROBE_BYTES // 8, dtype=torch.float64, device="cuda")
@@ -355,8 +387,12 @@ def traffic_from_pmc(workload, fetch_dir, write_dir):
         with open(path) as f:
             tab = json.load(f)
     key = kname + "|" + cfg["workload"]
+    old, new = probe_hop(cfg)
+    _, alg = sweep_work(cfg["E"], 2 ** (cfg["r"] - 1), cfg["L"], cfg["bc"] != "free_space", old, new,
+                        kname == "k_sweep_block")
     tab[key] = {"bytes_per_launch": res["fetch"]["bytes_per_launch"] + res["write"]["bytes_per_launch"],
                 "fetch": res["fetch"], "write": res["write"],
+                "algorithmic_bytes": alg, "hop": [list(old), list(new)],
                 "copy_probe_bytes": COPY_PROBE_BYTES}
     with open(path, "w") as f:
         json.dump(tab, f, indent=1, sort_keys=True)
@@ -496,6 +532,11 @@ def run_batch(args):
     # (one per order / periodicity group) timed with HIP events (kmc_batch_time_sweep);
     # bytes and flops of each system's sweep counted as for one system (sweep_work)
     sweep_blocked = tiles.get("choice", {}).get("sweep_block", max(args.sweep_block, 0)) == 1
+    # an (order, periodicity) group sweeps blocked when every system of it can
+    grp_blocked = {}
+    for x in specs:
+        key = (x["L"], x["bc"] != "free_space")
+        grp_blocked[key] = grp_blocked.get(key, True) and sweep_blocked and (2 ** (x["r"] - 1)) % 4 == 0
     b_sw, f_sw = 0.0, 0.0
     hop_rng = np.random.default_rng(spec["seed"] + 7)
     for bs, x in zip(systems, specs):
@@ -506,9 +547,10 @@ def run_batch(args):
         new = np.where(new < -0.5 * E, new + E, np.where(new >= 0.5 * E, new - E, new)) if x["bc"] == "pbc" \
             else np.clip(new, -0.5 * E, 0.5 * E)
         bs.kmc.sweep_hop(1.0, old, new)
-        nh = x["L"] * (x["L"] + 1) // 2
-        b_sw += 2.0 * ncps ** 3 * nh * 16
-        f_sw += sweep_work(E, ncps, x["L"], x["bc"] == "pbc", old, new, blocked=sweep_blocked)
+        f, b = sweep_work(E, ncps, x["L"], x["bc"] != "free_space", old, new,
+                          blocked=grp_blocked[(x["L"], x["bc"] != "free_space")])
+        f_sw += f
+        b_sw += b
     torch.cuda.synchronize()
     t_sw = batch.time_sweep(reps=10)
     sw_gbs = b_sw / (t_sw * 1e-3) / 1e9
@@ -519,8 +561,8 @@ def run_batch(args):
             "traffic": None, "launch_ms": t_sw, "algorithmic_bytes": b_sw, "algorithmic_flops": f_sw,
             "fp64_TFLOPs": sw_tf, "fp64_frac": sw_tf / FP64_PEAK_TFLOPS,
             "note": "launch_ms: all groups' sweep launches of one batch step, back to back on one stream "
-                    "(kmc_batch_time_sweep, HIP events); bytes = read + write of every leaf's packed expansion "
-                    "of the 256 systems; flops as bench.sweep_work for one hop per system"}
+                    "(kmc_batch_time_sweep, HIP events); bytes and flops as bench.sweep_work for one hop per "
+                    "system (blocked groups: only the orders that receive terms are read and written)"}
     log("batch sweep roofline", roof)
     out = {
         "metric": METRIC,
@@ -841,7 +883,7 @@ def main():
     blocked = tiles["sweep_block"] == 1 and ncps % 4 == 0 and bc != "ff-only"
     sweep_kernel = "k_sweep_block" if blocked else "k_sweep_fused"
     hop_rng = np.random.default_rng(cfg["seed"] + 7)
-    t_sum, f_sum = 0.0, 0.0
+    t_sum, f_sum, b_sum = 0.0, 0.0, 0.0
     for _ in range(3):
         old = hop_rng.uniform(-0.5 * E, 0.5 * E, 3)
         d = hop_rng.normal(size=3)
@@ -850,12 +892,14 @@ def main():
             else np.clip(new, -0.5 * E, 0.5 * E)
         kmc.sweep_hop(1.0, old, new)
         t_sum += kmc.time_kernel("sweep", reps=10)
-        f_sum += sweep_work(E, ncps, L, bc != "free_space", old, new, blocked)
+        f, b = sweep_work(E, ncps, L, bc != "free_space", old, new, blocked)
+        f_sum += f
+        b_sum += b
     kt["sweep"] = t_sum / 3
     sweep_flops = f_sum / 3
     log("kernel ms", kt)
     nimg = 54 if bc != "free_space" else 2
-    sweep_bytes = 2 * ncells * (L * (L + 1) // 2) * 16
+    sweep_bytes = b_sum / 3
     ach_tflops = sweep_flops / (kt["sweep"] * 1e-3) / 1e12
     ach_gbs = sweep_bytes / (kt["sweep"] * 1e-3) / 1e9
     ai = sweep_flops / sweep_bytes

@@ -686,9 +686,14 @@ __device__ __forceinline__ void sweep_fused_body(const SweepArgs& a) {
 // ---------------------------------------------------------------------------
 constexpr double SWEEP_TAU = 8.673617379884035e-19;   // 2^-60
 
-// smallest lc with |q| (rho/D)^lc / (D - rho) <= SWEEP_TAU |q|, capped at L
+// smallest lc with |q| (rho/D)^lc / (D - rho) <= SWEEP_TAU |q|, capped at L; every
+// order when D <= 3 rho.  A leaf whose 27-leaf stencil holds the image's leaf has its
+// centre within 1.5 leaf widths of the image on each axis, so D < 1.5 sqrt(3) w = 3 rho:
+// an image truncated below L has no such leaf in the block, and the truncated buckets
+// need no near-leaf test.  (For L <= 20 the bound alone already keeps every order
+// below 3 rho unless the leaves are ~1e12 units wide: (1/3)^L / (D - rho) > 2^-60.)
 __device__ __forceinline__ int sweep_trunc_order(double D, double rho, int L) {
-    if (!(D > 2.0 * rho)) return L;
+    if (!(D > 3.0 * rho)) return L;
     const double t = rho / D;
     double bound = 1.0 / (D - rho);
     int lc = 0;
@@ -709,6 +714,21 @@ __host__ __device__ constexpr int sweep_bucket(int i, int L) {
     constexpr int B[SWEEP_NBUCKET - 1] = {8, 9, 10, 12};
     return i < SWEEP_NBUCKET - 1 ? (B[i] < L ? B[i] : L) : L;
 }
+// bucket index of an image of truncation order lc >= 1
+__host__ __device__ constexpr int sweep_bucket_index(int lc, int L) {
+    for (int i = 0; i < SWEEP_NBUCKET; i++)
+        if (lc <= sweep_bucket(i, L)) return i;
+    return SWEEP_NBUCKET - 1;
+}
+// orders evaluated for an image of truncation order lc: its bucket (0 for lc = 0, an
+// image the blocked sweep skips)
+__host__ __device__ constexpr int sweep_bucket_of(int lc, int L) {
+    return lc <= 0 ? 0 : sweep_bucket(sweep_bucket_index(lc, L), L);
+}
+// per-block image lists of the blocked sweep (shared memory): bucket I's images at
+// [I * SWEEP_LIST ..) in stencil order, each entry k | lc << 8, and the count of bucket
+// I at [SWEEP_NBUCKET * SWEEP_LIST + I]
+constexpr int SWEEP_LIST = 32;   // >= 27 images
 
 // Column pairs of wave G in the blocked sweep: pair G and pair NPAIR-1-G (one pair when
 // they coincide).  With every order kept each wave has 2(L+1) coefficients; with the
@@ -722,12 +742,13 @@ struct BlockPairs {
     static constexpr int B = NPAIR - 1 - G;   // == A: one pair
 };
 // one source image for the lane's leaf into wave G's accumulators (row 0: pair A, row 1: pair B)
-template <int L, int G, int LC>
+// (NEAR: the image may be a near neighbour of the leaf, which then takes qe = 0)
+template <int L, int G, int LC, bool NEAR>
 __device__ __forceinline__ void block_image(double dx, double dy, double dz, double qe, double (*ar)[L + 1],
                                             double (*ai)[L + 1]) {
     using PP = BlockPairs<L, G>;
     double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-    r2 = qe != 0.0 ? r2 : 1.0;
+    if constexpr (NEAR) r2 = qe != 0.0 ? r2 : 1.0;
     const double ir = rsqrt_fp64(r2);
     const double ir2 = ir * ir;
     const double zr = dz * ir2;
@@ -742,41 +763,47 @@ __device__ __forceinline__ void block_image(double dx, double dy, double dz, dou
 template <int L, int G, bool PER, int I>
 __device__ __forceinline__ void sweep_block_images(const SweepArgs& a, const double* dx0, const double* dy0,
                                                    const double* dz0, const double* sq, const int* dcx,
-                                                   const int* dcy, const int* dcz, const int* lcs,
+                                                   const int* dcy, const int* dcz, const int* lists,
                                                    double (*ar)[L + 1], double (*ai)[L + 1]) {
     constexpr int LC = sweep_bucket(I, L);
     constexpr int LPREV = I > 0 ? sweep_bucket(I - 1, L) : 0;
     if constexpr (I < SWEEP_NBUCKET && (I == 0 || LPREV < L)) {
         if constexpr (G < LC) {
             constexpr int R = PER ? 1 : 0;
-            constexpr int NIMG = PER ? 27 : 1;
             const int n = a.g.ncps;
             const double E = a.g.E;
+            const int cnt = __builtin_amdgcn_readfirstlane(lists[SWEEP_NBUCKET * SWEEP_LIST + I]);
 #pragma unroll 1
-            for (int k = 0; k < NIMG; k++) {
-                const int lc = __builtin_amdgcn_readfirstlane(lcs[k]);
-                // images of this bucket that reach this wave's columns (its lowest is G)
-                if (lc <= G || lc > LC || (I > 0 && lc <= LPREV)) continue;
+            for (int i = 0; i < cnt; i++) {
+                const int e = __builtin_amdgcn_readfirstlane(lists[I * SWEEP_LIST + i]);
+                const int k = e & 255, lc = e >> 8;
+                if (lc <= G) continue;   // (the image does not reach this wave's columns, its lowest is G)
                 const int ox = k % (2 * R + 1) - R, oy = (k / (2 * R + 1)) % (2 * R + 1) - R,
                           oz = k / ((2 * R + 1) * (2 * R + 1)) - R;
 #pragma unroll
                 for (int s = 0; s < 2; s++) {
-                    const bool nr = near1(dcx[s] + ox * n) && near1(dcy[s] + oy * n) && near1(dcz[s] + oz * n);
-                    block_image<L, G, LC>(fma((double)ox, E, dx0[s]), fma((double)oy, E, dy0[s]),
-                                          fma((double)oz, E, dz0[s]), nr ? 0.0 : sq[s], ar, ai);
+                    const double dx = fma((double)ox, E, dx0[s]), dy = fma((double)oy, E, dy0[s]),
+                                 dz = fma((double)oz, E, dz0[s]);
+                    if constexpr (LC < L) {   // truncated: no leaf of the block is a near neighbour
+                        block_image<L, G, LC, false>(dx, dy, dz, sq[s], ar, ai);
+                    } else {
+                        const bool nr =
+                            near1(dcx[s] + ox * n) && near1(dcy[s] + oy * n) && near1(dcz[s] + oz * n);
+                        block_image<L, G, LC, true>(dx, dy, dz, nr ? 0.0 : sq[s], ar, ai);
+                    }
                 }
             }
         }
-        sweep_block_images<L, G, PER, I + 1>(a, dx0, dy0, dz0, sq, dcx, dcy, dcz, lcs, ar, ai);
+        sweep_block_images<L, G, PER, I + 1>(a, dx0, dy0, dz0, sq, dcx, dcy, dcz, lists, ar, ai);
     }
 }
 
 // group G (column pairs G*NP ..) of the move's 2 x nimg source images for the leaf
-// (tx, ty, tz) of this lane; lcs[k] = truncation order of image k (both sources).  The
+// (tx, ty, tz) of this lane; lists: the block's images by truncation bucket.  The
 // images go bucket by bucket (each bucket's loop a compile-time truncation: unrolled,
 // branch-free columns), in stencil order within a bucket.
 template <int L, int G, int NP, int NI, bool PER>
-__device__ __forceinline__ void sweep_block_group(const SweepArgs& a, int tx, int ty, int tz, const int* lcs,
+__device__ __forceinline__ void sweep_block_group(const SweepArgs& a, int tx, int ty, int tz, const int* lists,
                                                   double (*ar)[L + 1], double (*ai)[L + 1]) {
     static_assert(NI == 1 && NP == 2, "the blocked sweep: one source image per pass, two column pairs per lane");
     const int n = a.g.ncps;
@@ -794,14 +821,16 @@ __device__ __forceinline__ void sweep_block_group(const SweepArgs& a, int tx, in
         dx0[s] = p[0] - cx; dy0[s] = p[1] - cy; dz0[s] = p[2] - cz;
         dcx[s] = sc % n - tx; dcy[s] = (sc / n) % n - ty; dcz[s] = sc / (n * n) - tz;
     }
-    sweep_block_images<L, G, PER, 0>(a, dx0, dy0, dz0, sq, dcx, dcy, dcz, lcs, ar, ai);
+    sweep_block_images<L, G, PER, 0>(a, dx0, dy0, dz0, sq, dcx, dcy, dcz, lists, ar, ai);
 }
 // the scaled sums of wave G's columns for the lane's leaf, staged through LDS with the
 // other waves' (two halves of 32 leaves), added to the block's expansions with
-// coalesced loads and stores
+// coalesced loads and stores.  Only each leaf's first nh entries (the orders below the
+// largest order any image of the block evaluates) are read and written: the others
+// receive no term.
 template <int L, int G>
 __device__ __forceinline__ void sweep_block_update(const SweepArgs& a, double (*ar)[L + 1], double (*ai)[L + 1],
-                                                   double2* stage, int cell_base) {
+                                                   double2* stage, int cell_base, int nh) {
     constexpr int NH = nhalf(L);
     constexpr int ST = stage_stride(NH);
     const int lane = threadIdx.x;
@@ -821,24 +850,25 @@ __device__ __forceinline__ void sweep_block_update(const SweepArgs& a, double (*
         __syncthreads();
         // the half's 32 leaves: 8 runs (y, z) of 4 consecutive leaves along x
         constexpr int U = 8;
+        const int ne = 32 * nh;
 #pragma unroll 1
-        for (int e0 = tid; e0 < 32 * NH; e0 += nthr * U) {
+        for (int e0 = tid; e0 < ne; e0 += nthr * U) {
             double2 v[U];
             size_t gi[U];
+            int si[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int e = e0 + u * nthr;
-                const int j = e / NH, h = e - (e / NH) * NH;             // staged leaf, coefficient
+                const int j = e / nh, h = e - j * nh;             // staged leaf, coefficient
                 const int jy = (j >> 2) & 3, jz = 2 * half + (j >> 4);
                 gi[u] = (size_t)(cell_base + (j & 3) + n * (jy + n * jz)) * NH + h;
-                if (e < 32 * NH) v[u] = a.lexp[gi[u]];
+                si[u] = j * ST + h;
+                if (e < ne) v[u] = a.lexp[gi[u]];
             }
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                const int e = e0 + u * nthr;
-                if (e < 32 * NH) {
-                    const int j = e / NH, h = e - j * NH;
-                    const double2 sv = stage[j * ST + h];
+                if (e0 + u * nthr < ne) {
+                    const double2 sv = stage[si[u]];
                     v[u].x += sv.x;
                     v[u].y += sv.y;
                     a.lexp[gi[u]] = v[u];
@@ -851,24 +881,24 @@ __device__ __forceinline__ void sweep_block_update(const SweepArgs& a, double (*
 // one wave's share of the blocked sweep: group G's columns of its lane's leaf, then the
 // staged, coalesced update of the block's expansions together with the other waves
 template <int L, int G, int NP, int NI, bool PER>
-__device__ __forceinline__ void sweep_block_wave(const SweepArgs& a, int tx, int ty, int tz, const int* lcs,
-                                                 double2* stage, int cell_base) {
+__device__ __forceinline__ void sweep_block_wave(const SweepArgs& a, int tx, int ty, int tz, const int* lists,
+                                                 double2* stage, int cell_base, int nh) {
     constexpr int W = L + 1;
     double ar[NP][W], ai[NP][W];
 #pragma unroll
     for (int j = 0; j < NP; j++)
 #pragma unroll
         for (int k = 0; k < W; k++) { ar[j][k] = 0.0; ai[j][k] = 0.0; }
-    sweep_block_group<L, G, NP, NI, PER>(a, tx, ty, tz, lcs, ar, ai);
-    sweep_block_update<L, G>(a, ar, ai, stage, cell_base);
+    sweep_block_group<L, G, NP, NI, PER>(a, tx, ty, tz, lists, ar, ai);
+    sweep_block_update<L, G>(a, ar, ai, stage, cell_base, nh);
 }
 template <int L, int NP, int NI, bool PER, int G0>
 __device__ __forceinline__ void sweep_block_dispatch(const SweepArgs& a, int group, int tx, int ty, int tz,
-                                                     const int* lcs, double2* stage, int cell_base) {
+                                                     const int* lists, double2* stage, int cell_base, int nh) {
     constexpr int NG = ((L + 1) / 2 + NP - 1) / NP;
     if constexpr (G0 < NG) {
-        if (group == G0) sweep_block_wave<L, G0, NP, NI, PER>(a, tx, ty, tz, lcs, stage, cell_base);
-        else sweep_block_dispatch<L, NP, NI, PER, G0 + 1>(a, group, tx, ty, tz, lcs, stage, cell_base);
+        if (group == G0) sweep_block_wave<L, G0, NP, NI, PER>(a, tx, ty, tz, lists, stage, cell_base, nh);
+        else sweep_block_dispatch<L, NP, NI, PER, G0 + 1>(a, group, tx, ty, tz, lists, stage, cell_base, nh);
     }
 }
 
@@ -903,17 +933,36 @@ __device__ __forceinline__ void sweep_block_body(const SweepArgs& a, int b) {
     constexpr int ST = stage_stride(NH);
     constexpr int NIMG = PER ? 27 : 1;
     __shared__ double2 stage[32 * ST];
-    __shared__ int s_lc[NIMG];
+    __shared__ int s_list[SWEEP_NBUCKET * SWEEP_LIST + SWEEP_NBUCKET];
+    __shared__ int s_nh;
     const int lane = threadIdx.x;
     const int group = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     const int n = a.g.ncps, nb = n >> 2;
     const int bx = b % nb, by = (b / nb) % nb, bz = b / (nb * nb);
     const int tx = 4 * bx + (lane & 3), ty = 4 * by + ((lane >> 2) & 3), tz = 4 * bz + (lane >> 4);
     if (!a.rec->valid) return;   // (uniform over the grid: nothing to add)
-    if (group == 0 && lane < NIMG) s_lc[lane] = sweep_block_lc<L, PER>(a, bx, by, bz, lane);
+    if (group == 0) {
+        // lane k < NIMG: image k's truncation order; its bucket's list gets k at its rank
+        // among the bucket's images (stencil order)
+        int lc = lane < NIMG ? sweep_block_lc<L, PER>(a, bx, by, bz, lane) : 0;
+        const int bi = lc > 0 ? sweep_bucket_index(lc, L) : -1;
+        const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+        for (int I = 0; I < SWEEP_NBUCKET; I++) {
+            const uint64_t m = __ballot(bi == I);
+            if (bi == I) s_list[I * SWEEP_LIST + __popcll(m & below)] = lane | (lc << 8);
+            if (lane == 0) s_list[SWEEP_NBUCKET * SWEEP_LIST + I] = __popcll(m);
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) lc = max(lc, __shfl_xor(lc, o, 64));
+        // expansion entries that receive terms: the orders below the largest evaluated
+        if (lane == 0) s_nh = nhalf(sweep_bucket_of(lc, L));
+    }
     __syncthreads();
+    const int nh = __builtin_amdgcn_readfirstlane(s_nh);
+    if (nh == 0) return;   // (uniform over the workgroup)
     const int cell_base = 4 * bx + n * (4 * by + n * 4 * bz);
-    sweep_block_dispatch<L, NP, NI, PER, 0>(a, group, tx, ty, tz, s_lc, stage, cell_base);
+    sweep_block_dispatch<L, NP, NI, PER, 0>(a, group, tx, ty, tz, s_list, stage, cell_base, nh);
 }
 
 template <int L, int NP, int NI, bool PER>

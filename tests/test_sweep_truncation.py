@@ -20,7 +20,7 @@ TAU = 2.0 ** -60
 
 def trunc_order(D, rho, L):
     """The kernel's rule (sweep_trunc_order in csrc/kmc_device.h)."""
-    if not D > 2.0 * rho:
+    if not D > 3.0 * rho:
         return L
     t = rho / D
     bound = 1.0 / (D - rho)
@@ -75,3 +75,31 @@ def test_truncation_rule_values():
             assert (rho / D) ** lc / (D - rho) <= TAU
         if lc > 0:
             assert (rho / D) ** (lc - 1) / (D - rho) > TAU
+
+
+def test_near_leaves_keep_every_order():
+    """Truncated images skip the near-leaf test: a leaf of the block whose 27-leaf
+    stencil holds the image's leaf lies within 1.5 leaf widths of the image on each
+    axis, so the image's distance D to the box of the block's leaf centres is below
+    3 rho, where the rule keeps every order.  Brute force over random images and
+    blocks, with the kernel's leaf binning (floor) and its block box."""
+    rng = np.random.default_rng(23)
+    w, ncps = 1.7, 16
+    rho = 0.5 * np.sqrt(3.0) * w
+    lo = -0.5 * ncps * w
+    worst = 0.0
+    for _ in range(4000):
+        b = rng.integers(0, ncps // 4, 3)
+        mid = lo + (4 * b + 2.0) * w                                   # centre of the block's leaf-centre box
+        p = mid + rng.uniform(-4.0 * w, 4.0 * w, 3)                    # images around the block
+        cell = np.floor((p - lo) / w).astype(int)
+        leaves = 4 * b + np.stack(np.meshgrid(*[np.arange(4)] * 3, indexing="ij"), -1).reshape(-1, 3)
+        near = np.all(np.abs(cell - leaves) <= 1, axis=1)
+        e = np.maximum(np.abs(p - mid) - 1.5 * w, 0.0)
+        D = float(np.sqrt((e ** 2).sum()))
+        if near.any():
+            worst = max(worst, D / rho)
+            assert D < 3.0 * rho, (p, b, D / rho)
+            assert trunc_order(D, rho, 16) == 16
+    assert worst > 2.0    # the cases seen come close to the 3 rho limit
+
