@@ -1732,7 +1732,8 @@ static void prof_add(kmc_ctx* c, int kind, hipEvent_t a, hipEvent_t b, int64_t c
 // range); with w given nothing is allocated
 static int run_propose_core(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, double* d_out, const int64_t* rate_loc,
                             bool mixed, double kT = 0.0, double* rates = nullptr, const int64_t* mask = nullptr,
-                            int rate_rule = 0, int wait_sweep = 0, const ProposeWork* w = nullptr) {
+                            int rate_rule = 0, int wait_sweep = 0, const ProposeWork* w = nullptr,
+                            bool lr_first = false) {
     const int ne = (int)(T + P);
     if (!w) {   // sizes of the context's evaluation arrays (kmc_time_kernel reuses them)
         c->last_P = P;
@@ -1746,9 +1747,18 @@ static int run_propose_core(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, dou
         rc = fork_side(c, s, 0);
         if (rc) return rc;
     }
-    // the near and far fields are recorded before the long range: a captured graph's
-    // first branch after a fork stays on the launching stream's queue, and the near/far
-    // chain is the step's critical path (a dependency across queues costs ~10 us)
+    // Plain steps record the near and far fields before the long range: a captured
+    // graph's first branch after a fork stays on the launching stream's queue, and the
+    // near/far chain is then the critical path (a dependency across queues costs
+    // ~10 us; 2M charges: 0.767 -> 0.745 ms per step).  Pipelined runs (lr_first)
+    // record the long range first, the order they were tuned with (100k charges, mode 2:
+    // 0.114 against 0.121 ms the other way round).
+    auto launch_lr = [&]() {
+        hipEvent_t e0 = prof ? prof_mark(c, c->side) : nullptr;
+        c->fns.lr(lr_args(c, P, T, w), c->side);
+        if (prof) prof_add(c, 2, e0, prof_mark(c, c->side), T);
+    };
+    if (lr && lr_first) launch_lr();
     const ProposeWork wk = w ? *w : ctx_work(c);
     if (c->bc != KMC_BC_FF_ONLY) {
         // wait_sweep 2: the near field too waits for the previous step's sweep (only the
@@ -1770,9 +1780,7 @@ static int run_propose_core(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, dou
         if (prof) prof_add(c, 1, e1, prof_mark(c, s), ne);
     }
     if (lr) {
-        hipEvent_t e0 = prof ? prof_mark(c, c->side) : nullptr;
-        c->fns.lr(lr_args(c, P, T, w), c->side);
-        if (prof) prof_add(c, 2, e0, prof_mark(c, c->side), T);
+        if (!lr_first) launch_lr();
         int rc = join_side(c, s, 0);
         if (rc) return rc;
     }
@@ -1809,6 +1817,9 @@ static int run_accept_tail(kmc_ctx* c, hipStream_t s, int64_t* d_counter, bool d
     // share the move record, which both read: they run side by side
     int rc = fork_side(c, s, 1);
     if (rc) return rc;
+    // plain steps record the sweep first (it stays on the launching queue), pipelined
+    // steps the tail first (the order they were tuned with)
+    if (defer_sweep) c->fns.md(ta, c->side);
     const SweepArgs sa = sweep_rec_args(c);
     const bool prof = prof_on(c, s);
     if (defer_sweep) {
@@ -1825,7 +1836,7 @@ static int run_accept_tail(kmc_ctx* c, hipStream_t s, int64_t* d_counter, bool d
         c->fns.sweep(sa, (int)c->ncells, s, c->sweep_block, c->sweep_pairs, c->sweep_split);
         if (prof) prof_add(c, 3, e0, prof_mark(c, s), 1);
     }
-    c->fns.md(ta, c->side);   // (recorded after the sweep, for the same reason as the long range)
+    if (!defer_sweep) c->fns.md(ta, c->side);
     rc = join_side(c, s, 1);
     if (rc) return rc;
     HIPCHK(hipGetLastError());
@@ -2877,7 +2888,7 @@ int kmc_run_steps(kmc_ctx* c, void* stream, int64_t nrun, int64_t nsteps_moves, 
         rc = launch_setup(c, s, P, T, d_pid, d_es, d_new_pos, nullptr);
         if (rc == KMC_OK)
             rc = run_propose_core(c, s, P, T, d_dE, nullptr, mixed, kT, c->rates.p, nullptr, 0,
-                                  (pipe && it > 0) ? (pipelined == 2 ? 2 : 1) : 0);
+                                  (pipe && it > 0) ? (pipelined == 2 ? 2 : 1) : 0, nullptr, pipe);
         if (rc == KMC_OK) {
             hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, s, sa);
             rc = run_accept_tail(c, s, d_counter, pipe);
