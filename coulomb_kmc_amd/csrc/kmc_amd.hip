@@ -69,44 +69,81 @@ __device__ __forceinline__ int tile_of_cell(int c, int n) {
     const int nb = n >> 2;
     return ((c % n) >> 2) + nb * ((((c / n) % n) >> 2) + nb * ((c / (n * n)) >> 2));
 }
-// The per-tile counters (counts, cursors) are TSTRIDE ints apart, one per 128-byte
-// line: every evaluation entry increments its tile's counter atomically, and counters
-// packed into a few lines put those atomics on a few memory channels.
+// The per-tile counts are TSTRIDE ints apart, one per 128-byte line: the entries'
+// atomics on counts packed into a few lines went to a few memory channels.
 constexpr int TSTRIDE = 32;
-__global__ void k_tile_count(int ne, const int32_t* __restrict__ cell, int n, int* __restrict__ tcount) {
+// Each evaluation entry's rank among the entries of its tile (rank[x]) and the
+// per-tile counts.  Entries come in moving-charge order, so a wave holds a few tiles:
+// the lanes that share a tile take one atomic for all of them (the group's first lane
+// adds the group's size), and each lane's rank is the returned base plus its place in
+// the group.
+__global__ void k_tile_count(int ne, const int32_t* __restrict__ cell, int n, int* __restrict__ tcount,
+                             int* __restrict__ rank) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x < ne) atomicAdd(&tcount[tile_of_cell(cell[x], n) * TSTRIDE], 1);
+    const int lane = threadIdx.x & 63;
+    const bool on = x < ne;
+    const int t = on ? tile_of_cell(cell[x], n) : -1;
+    uint64_t todo = __ballot(on);
+    int lead = lane;
+    uint64_t grp = 0;
+    while (todo) {
+        const int l = __ffsll((long long)todo) - 1;   // first lane not yet grouped
+        const int tl = __shfl(t, l, 64);
+        const uint64_t m = __ballot(t == tl) & todo;
+        if (on && t == tl) { lead = l; grp = m; }
+        todo &= ~m;
+    }
+    int base = 0;
+    if (on && lane == lead) base = atomicAdd(&tcount[t * TSTRIDE], (int)__popcll(grp));
+    base = __shfl(base, lead, 64);
+    if (on) rank[x] = base + (int)__popcll(grp & ((1ull << lane) - 1ull));
 }
-// exclusive scan of the counts into toff[0..nt] and the cursors; the counts back to zero
-__global__ void __launch_bounds__(1024) k_tile_scan(int nt, int* __restrict__ tcount, int* __restrict__ toff,
-                                                    int* __restrict__ tcur) {
-    __shared__ int sums[1024];
-    const int tid = threadIdx.x, nth = blockDim.x;
-    const int chunk = (nt + nth - 1) / nth;
+// exclusive scan of the counts into toff[0..nt] (one workgroup: each thread a chunk of
+// tiles, wave scans by shuffles, then the 16 wave totals); the counts back to zero
+__global__ void __launch_bounds__(1024) k_tile_scan(int nt, int* __restrict__ tcount, int* __restrict__ toff) {
+    __shared__ int wsum[16];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int chunk = (nt + 1023) / 1024;
     const int b0 = min(nt, tid * chunk), b1 = min(nt, b0 + chunk);
     int s = 0;
-    for (int i = b0; i < b1; i++) s += tcount[i * TSTRIDE];
-    sums[tid] = s;
-    __syncthreads();
-    for (int o = 1; o < nth; o <<= 1) {
-        const int v = tid >= o ? sums[tid - o] : 0;
-        __syncthreads();
-        sums[tid] += v;
-        __syncthreads();
+    for (int g = b0; g < b1; g += 4) {   // (loads of a group of 4 issued together)
+        int v4[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) v4[j] = g + j < b1 ? tcount[(g + j) * TSTRIDE] : 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) s += v4[j];
     }
-    int c = tid > 0 ? sums[tid - 1] : 0;
+    int v = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(v, o, 64);
+        if (lane >= o) v += y;
+    }
+    if (lane == 63) wsum[wv] = v;
+    __syncthreads();
+    if (wv == 0) {
+        int w = lane < 16 ? wsum[lane] : 0;
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+            const int y = __shfl_up(w, o, 64);
+            if (lane >= o) w += y;
+        }
+        if (lane < 16) wsum[lane] = w;
+    }
+    __syncthreads();
+    int c = v - s + (wv > 0 ? wsum[wv - 1] : 0);
     for (int i = b0; i < b1; i++) {
+        const int k = tcount[i * TSTRIDE];
         toff[i] = c;
-        tcur[i * TSTRIDE] = c;
-        c += tcount[i * TSTRIDE];
+        c += k;
         tcount[i * TSTRIDE] = 0;
     }
-    if (tid == nth - 1) toff[nt] = sums[nth - 1];
+    if (tid == 1023) toff[nt] = wsum[15];
 }
-__global__ void k_tile_scatter(int ne, const int32_t* __restrict__ cell, int n, int* __restrict__ tcur,
-                               int* __restrict__ perm) {
+__global__ void k_tile_scatter(int ne, const int32_t* __restrict__ cell, int n, const int* __restrict__ toff,
+                               const int* __restrict__ rank, int* __restrict__ perm) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x < ne) perm[atomicAdd(&tcur[tile_of_cell(cell[x], n) * TSTRIDE], 1)] = x;
+    if (x < ne) perm[toff[tile_of_cell(cell[x], n)] + rank[x]] = x;
 }
 
 // one point from memory (tiles whose leaves overflow the LDS): near_body's loop
@@ -195,18 +232,31 @@ __global__ void __launch_bounds__(NT_THREADS) k_near_tile(EvalArgs a, const int*
     const int total = s_start[NT_HALO];
     const bool staged = total <= NT_SLOTS;
     if (staged) {
-        // copy the records, each thread a share of the flattened list (leaf by binary search)
-        for (int g = tid; g < total; g += NT_THREADS) {
-            int lo = 0, hi = NT_HALO - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (s_start[mid] <= g) lo = mid;
-                else hi = mid - 1;
+        // copy the records: four threads per halo leaf (864 of the 1024), each every
+        // fourth slot of its leaf, two slots' loads issued at a time
+        static_assert(4 * NT_HALO <= NT_THREADS, "four threads per halo leaf");
+        if (tid < 4 * NT_HALO) {
+            const int h = tid >> 2;
+            const int d0 = s_start[h], cnt = s_start[h + 1] - d0;
+            const size_t src = (size_t)s_leaf[h] * a.occmax;
+            const double sx = s_sh[h][0], sy = s_sh[h][1], sz = s_sh[h][2];
+            for (int j = tid & 3; j < cnt; j += 8) {
+                const bool two = j + 4 < cnt;
+                const double4 q0 = a.slot_pos[src + j];
+                const int i0 = a.slot_id[src + j];
+                double4 q1 = q0;
+                int i1 = i0;
+                if (two) {
+                    q1 = a.slot_pos[src + j + 4];
+                    i1 = a.slot_id[src + j + 4];
+                }
+                s_pos[d0 + j] = make_double4(q0.x + sx, q0.y + sy, q0.z + sz, q0.w);
+                s_id[d0 + j] = i0;
+                if (two) {
+                    s_pos[d0 + j + 4] = make_double4(q1.x + sx, q1.y + sy, q1.z + sz, q1.w);
+                    s_id[d0 + j + 4] = i1;
+                }
             }
-            const size_t sidx = (size_t)s_leaf[lo] * a.occmax + (g - s_start[lo]);
-            const double4 q = a.slot_pos[sidx];
-            s_pos[g] = make_double4(q.x + s_sh[lo][0], q.y + s_sh[lo][1], q.z + s_sh[lo][2], q.w);
-            s_id[g] = a.slot_id[sidx];
         }
         __syncthreads();
     }
@@ -1372,7 +1422,7 @@ struct kmc_ctx {
     DBuf<double> UD, UI, lrd, dE1, rates, bsum;
     DBuf<int64_t> one_pid, one_es;
     DBuf<int64_t> es_copy;            // es of the last propose set-up
-    DBuf<int> tcount, toff, tcur, perm;   // leaf-tiled near field (counting sort by tile)
+    DBuf<int> tcount, toff, rank, perm;   // leaf-tiled near field (counting sort by tile)
     DBuf<double> one_pos;
     // propose_with_dats work buffers
     DBuf<int64_t> d_cnt, d_has, d_slot, d_off, d_pid, d_es, d_rate, d_tot, d_scan;
@@ -1422,7 +1472,7 @@ static int ensure_eval(kmc_ctx* c, int64_t T, int64_t P) {
         HIPCHK(c->tcount.ensure((size_t)nt * TSTRIDE));
         if (fresh) HIPCHK(hipMemset(c->tcount.p, 0, sizeof(int) * nt * TSTRIDE));   // the scan keeps it zero after
         HIPCHK(c->toff.ensure((size_t)nt + 1));
-        HIPCHK(c->tcur.ensure((size_t)nt * TSTRIDE));
+        HIPCHK(c->rank.ensure(ne));
         HIPCHK(c->perm.ensure(ne));
     }
     return KMC_OK;
@@ -1437,13 +1487,13 @@ struct ProposeWork {
     double *UD, *UI, *lrd, *aold, *cold;
     int64_t* es_copy;
     int* err;
-    // leaf-tiled near field: per-tile counts (zero between calls), offsets, cursors, and
-    // the evaluation entries sorted by tile
-    int *tcount, *toff, *tcur, *perm;
+    // leaf-tiled near field: per-tile counts (zero between calls), offsets, each
+    // entry's rank in its tile, and the evaluation entries sorted by tile
+    int *tcount, *toff, *rank, *perm;
 };
 static ProposeWork ctx_work(kmc_ctx* c) {
     return ProposeWork{c->epos4.p, c->ecell.p, c->eexcl.p, c->owner.p, c->UD.p,   c->UI.p,   c->lrd.p,
-                       c->aold.p,  c->cold.p,  c->es_copy.p, c->err.p, c->tcount.p, c->toff.p, c->tcur.p,
+                       c->aold.p,  c->cold.p,  c->es_copy.p, c->err.p, c->tcount.p, c->toff.p, c->rank.p,
                        c->perm.p};
 }
 
@@ -1452,9 +1502,9 @@ static ProposeWork ctx_work(kmc_ctx* c) {
 template <bool MIXED>
 static void launch_near_tiled(const kmc_ctx* c, hipStream_t s, const EvalArgs& a, int G, const ProposeWork& w) {
     const int n = c->ncps, nt = near_tiles(c), ne = a.n;
-    hipLaunchKernelGGL(k_tile_count, dim3((ne + 255) / 256), dim3(256), 0, s, ne, a.cell, n, w.tcount);
-    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, nt, w.tcount, w.toff, w.tcur);
-    hipLaunchKernelGGL(k_tile_scatter, dim3((ne + 255) / 256), dim3(256), 0, s, ne, a.cell, n, w.tcur, w.perm);
+    hipLaunchKernelGGL(k_tile_count, dim3((ne + 255) / 256), dim3(256), 0, s, ne, a.cell, n, w.tcount, w.rank);
+    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, nt, w.tcount, w.toff);
+    hipLaunchKernelGGL(k_tile_scatter, dim3((ne + 255) / 256), dim3(256), 0, s, ne, a.cell, n, w.toff, w.rank, w.perm);
     switch (G) {
         case 4: hipLaunchKernelGGL((k_near_tile<4, MIXED>), dim3(nt), dim3(NT_THREADS), 0, s, a, w.toff, w.perm); break;
         case 8: hipLaunchKernelGGL((k_near_tile<8, MIXED>), dim3(nt), dim3(NT_THREADS), 0, s, a, w.toff, w.perm); break;
@@ -1465,9 +1515,9 @@ static void launch_near_tiled(const kmc_ctx* c, hipStream_t s, const EvalArgs& a
 template <bool MIXED>
 static void launch_near_sorted(const kmc_ctx* c, hipStream_t s, const EvalArgs& a, int G, const ProposeWork& w) {
     const int n = c->ncps, nt = near_tiles(c), ne = a.n;
-    hipLaunchKernelGGL(k_tile_count, dim3((ne + 255) / 256), dim3(256), 0, s, ne, a.cell, n, w.tcount);
-    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, nt, w.tcount, w.toff, w.tcur);
-    hipLaunchKernelGGL(k_tile_scatter, dim3((ne + 255) / 256), dim3(256), 0, s, ne, a.cell, n, w.tcur, w.perm);
+    hipLaunchKernelGGL(k_tile_count, dim3((ne + 255) / 256), dim3(256), 0, s, ne, a.cell, n, w.tcount, w.rank);
+    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, nt, w.tcount, w.toff);
+    hipLaunchKernelGGL(k_tile_scatter, dim3((ne + 255) / 256), dim3(256), 0, s, ne, a.cell, n, w.toff, w.rank, w.perm);
     const dim3 grid((unsigned)(((long long)ne * G + 255) / 256));
     switch (G) {
         case 1: hipLaunchKernelGGL((k_near_sorted<1, MIXED>), grid, dim3(256), 0, s, a, w.perm); break;
@@ -2078,7 +2128,7 @@ static size_t carve_work(const kmc_ctx* c, int64_t P, int64_t T, unsigned char* 
     if (nt > 0) {
         k.tcount = (int*)take((size_t)nt * TSTRIDE * sizeof(int));
         k.toff = (int*)take((size_t)(nt + 1) * sizeof(int));
-        k.tcur = (int*)take((size_t)nt * TSTRIDE * sizeof(int));
+        k.rank = (int*)take(ne * sizeof(int));
         k.perm = (int*)take(ne * sizeof(int));
     }
     if (uses_lr(c)) {
