@@ -40,7 +40,7 @@ WORKLOADS = {
     "pbc-100k": dict(workload="pbc N=100000 box 100 leaves 32^3 L=12, 1024x8 proposals", N=100000, E=100.0, r=6,
                      L=12, bc="pbc", P=1024, k=8, radius=1.5, seed=1001, steps=1000, warmup=50, pipeline=2),
     "pbc-2m": dict(workload="pbc N=2000000 box 271.4 leaves 64^3 L=16, 32768x8 proposals", N=2000000, E=271.4,
-                   r=7, L=16, bc="pbc", P=32768, k=8, radius=1.5, seed=1004, steps=100, warmup=20, pipeline=0),
+                   r=7, L=16, bc="pbc", P=32768, k=8, radius=1.5, seed=1004, steps=100, warmup=20, pipeline=2),
 }
 DEFAULT_WORKLOAD = "pbc-2m"
 
@@ -621,8 +621,8 @@ def main():
     ap.add_argument("--pipeline", default="auto", choices=("auto", "on", "off", "setup"),
                     help="on: kmc_run_steps with each step's sweep beside the next step's near field and long "
                          "range; setup: beside its set-up and long range only; off: kmc_materialize_moves + "
-                         "kmc_step per step; auto: the workload's choice (on for free-1k, setup for pbc-100k, "
-                         "where they measured fastest; off for pbc-2m, where neither gained)")
+                         "kmc_step per step; auto: the workload's choice (on for free-1k, setup for pbc-100k "
+                         "and pbc-2m, where they measured fastest)")
     ap.add_argument("--probe-traffic", action="store_true",
                     help="only launch the accept sweep and a copy of known size (for rocprofv3 --pmc runs)")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),

@@ -223,10 +223,12 @@ struct SweepArgs {
 
 // Rescaled column recurrence of the sweep: e_l = c_l / Gam_l obeys
 //   e_l = a2_lm zr e_{l-1} - ir2 e_{l-2},  Gam_l = beta_lm Gam_{l-2},  a2_lm = alpha_lm Gam_{l-1} / Gam_l,
-// one multiply less per coefficient; accumulators are multiplied by Gam_l once per leaf.
+// one multiply less per coefficient; accumulators are multiplied by Gam_l s_m (gs, the
+// column's diagonal factor s_m folded in) once per leaf.
 struct RecTab2 {
     double a2[KMAX_L + 1][KMAX_L + 1];
     double gam[KMAX_L + 1][KMAX_L + 1];
+    double gs[KMAX_L + 1][KMAX_L + 1];
 };
 constexpr RecTab2 make_rectab2() {
     RecTab2 t{};
@@ -240,6 +242,7 @@ constexpr RecTab2 make_rectab2() {
             t.gam[l][m] = kRec.b[l][m] * t.gam[l - 2][m];
             t.a2[l][m] = kRec.a[l][m] * t.gam[l - 1][m] / t.gam[l][m];
         }
+        for (int l = m; l <= KMAX_L; l++) t.gs[l][m] = t.gam[l][m] * kRec.s[m];
     }
     return t;
 }
@@ -282,7 +285,7 @@ template <int L, int M>
 __device__ __forceinline__ void sweep_store_column(double2* le, const double* ar, const double* ai) {
 #pragma unroll
     for (int k = 0; k < L - M; k++) {
-        const double gm = kRec2.gam[M + k][M];
+        const double gm = kRec2.gs[M + k][M];
         double2 v = le[hidx(M + k, M)];
         v.x = fma(gm, ar[k], v.x);
         v.y = fma(gm, ai[k], v.y);
@@ -291,17 +294,12 @@ __device__ __forceinline__ void sweep_store_column(double2* le, const double* ar
 }
 
 // columns (P, L-1-P) of pair P, NI images, into accumulator row ar/ai (first L-P entries: column P)
-// column M with weight base * s_M * U (U = conj((x+iy)/r^2)^M) into the accumulator row
+// column M with weight U = base conj((x+iy)/r^2)^M into the accumulator row (its factor
+// s_M is applied with Gam when the sums are stored)
 template <int L, int M, int NI, int LC = L>
 __device__ __forceinline__ void sweep_weighted_column(const double* zr, const double* ir2, const Cplx* U,
-                                                      const double* base, double* ar, double* ai) {
-    Cplx W[NI];
-#pragma unroll
-    for (int j = 0; j < NI; j++) {
-        const double f = base[j] * kRec.s[M];
-        W[j] = Cplx{f * U[j].re, f * U[j].im};
-    }
-    if constexpr (M < LC) sweep_column<L, M, NI, LC>(zr, ir2, W, ar, ai);
+                                                      double* ar, double* ai) {
+    if constexpr (M < LC) sweep_column<L, M, NI, LC>(zr, ir2, U, ar, ai);
 }
 // low columns P0 + J, J = 0 .. NPE-1 (ascending powers: one multiply per column)
 template <int L, int P0, int NPE, int NI, int LC, int J = 0>
@@ -312,7 +310,7 @@ __device__ __forceinline__ void sweep_low_cols(const double* zr, const double* i
 #pragma unroll
             for (int j = 0; j < NI; j++) U[j] = cmul(U[j], u[j]);
         }
-        sweep_weighted_column<L, P0 + J, NI, LC>(zr, ir2, U, base, ar[J], ai[J]);
+        sweep_weighted_column<L, P0 + J, NI, LC>(zr, ir2, U, ar[J], ai[J]);
         sweep_low_cols<L, P0, NPE, NI, LC, J + 1>(zr, ir2, u, U, base, ar, ai);
     }
 }
@@ -324,7 +322,7 @@ __device__ __forceinline__ void sweep_high_cols(const double* zr, const double* 
     if constexpr (J >= 0) {
         constexpr int M1 = P0 + J, M2 = L - 1 - P0 - J;
         if constexpr (M2 != M1)
-            sweep_weighted_column<L, M2, NI, LC>(zr, ir2, U, base, ar[J] + (L - M1), ai[J] + (L - M1));
+            sweep_weighted_column<L, M2, NI, LC>(zr, ir2, U, ar[J] + (L - M1), ai[J] + (L - M1));
         if constexpr (J > 0) {
 #pragma unroll
             for (int j = 0; j < NI; j++) U[j] = cmul(U[j], u[j]);
@@ -337,15 +335,17 @@ __device__ __forceinline__ void sweep_high_cols(const double* zr, const double* 
 // LC: orders l >= LC are left out (the blocked sweep's truncation buckets; L: none).
 // Columns ascend within the low and within the high part, so a column at or above LC
 // ends its part.
+// U0[j] = base u^P0, the weight of column P0 (the power chain carries base along)
 template <int L, int P0, int NP, int NI, int LC = L>
-__device__ __forceinline__ void sweep_group_cols(const double* zr, const double* ir2, const Cplx* u,
-                                                 const double* base, double (*ar)[L + 1], double (*ai)[L + 1]) {
+__device__ __forceinline__ void sweep_group_cols_from(const double* zr, const double* ir2, const Cplx* u,
+                                                      const Cplx* U0, const double* base, double (*ar)[L + 1],
+                                                      double (*ai)[L + 1]) {
     constexpr int NPAIR = (L + 1) / 2;
     constexpr int NPE = (P0 + NP <= NPAIR) ? NP : NPAIR - P0;   // pairs in this group
     if constexpr (NPE > 0 && P0 < LC) {
         Cplx U[NI];
 #pragma unroll
-        for (int j = 0; j < NI; j++) U[j] = cpow<P0>(u[j]);
+        for (int j = 0; j < NI; j++) U[j] = U0[j];
         sweep_low_cols<L, P0, NPE, NI, LC>(zr, ir2, u, U, base, ar, ai);
         constexpr int LOW_TOP = P0 + NPE - 1;          // power held in U now
         constexpr int HIGH_BOTTOM = L - 1 - LOW_TOP;    // lowest high column
@@ -359,6 +359,20 @@ __device__ __forceinline__ void sweep_group_cols(const double* zr, const double*
             for (int j = 0; j < NI; j++) U[j] = cmul(U[j], u[j]);
             sweep_high_cols<L, P0, NPE, NI, LC, NPE - 2>(zr, ir2, u, U, base, ar, ai);
         }
+    }
+}
+template <int L, int P0, int NP, int NI, int LC = L>
+__device__ __forceinline__ void sweep_group_cols(const double* zr, const double* ir2, const Cplx* u,
+                                                 const double* base, double (*ar)[L + 1], double (*ai)[L + 1]) {
+    constexpr int NPAIR = (L + 1) / 2;
+    if constexpr (P0 < NPAIR && P0 < LC) {
+        Cplx U0[NI];
+#pragma unroll
+        for (int j = 0; j < NI; j++) {
+            const Cplx c = cpow<P0>(u[j]);
+            U0[j] = Cplx{base[j] * c.re, base[j] * c.im};
+        }
+        sweep_group_cols_from<L, P0, NP, NI, LC>(zr, ir2, u, U0, base, ar, ai);
     }
 }
 template <int L, int P0, int NP, int J = 0>
@@ -531,8 +545,8 @@ template <int L, int M>
 __device__ __forceinline__ void sweep_scale_column(double* ar, double* ai) {
 #pragma unroll
     for (int k = 0; k < L - M; k++) {
-        ar[k] *= kRec2.gam[M + k][M];
-        ai[k] *= kRec2.gam[M + k][M];
+        ar[k] *= kRec2.gs[M + k][M];
+        ai[k] *= kRec2.gs[M + k][M];
     }
 }
 template <int L, int M>
