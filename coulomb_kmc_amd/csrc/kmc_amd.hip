@@ -1373,6 +1373,7 @@ struct kmc_ctx {
     // step i runs beside the set-up, near field and long range of step i + 1
     hipStream_t side2 = nullptr;
     hipEvent_t ev_fork2 = nullptr, ev_sweep = nullptr;
+    hipEvent_t ev_setup = nullptr, ev_far = nullptr;   // pipelined mode 3: far field on side2
     bool have_particles = false, have_lr = false, initialised = false;
     // the host mirrors below lag the device after device-side steps
     bool mirrors_stale = false;
@@ -1783,7 +1784,7 @@ static void prof_add(kmc_ctx* c, int kind, hipEvent_t a, hipEvent_t b, int64_t c
 static int run_propose_core(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, double* d_out, const int64_t* rate_loc,
                             bool mixed, double kT = 0.0, double* rates = nullptr, const int64_t* mask = nullptr,
                             int rate_rule = 0, int wait_sweep = 0, const ProposeWork* w = nullptr,
-                            bool lr_first = false) {
+                            bool lr_first = false, bool far_side2 = false) {
     const int ne = (int)(T + P);
     if (!w) {   // sizes of the context's evaluation arrays (kmc_time_kernel reuses them)
         c->last_P = P;
@@ -1810,6 +1811,19 @@ static int run_propose_core(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, dou
     };
     if (lr && lr_first) launch_lr();
     const ProposeWork wk = w ? *w : ctx_work(c);
+    // far_side2 (pipelined mode 3): the far field on side2, behind the previous step's
+    // sweep there (stream order) and after this step's set-up (event), so that it runs
+    // beside the near field; s waits for it before the combine
+    const bool far2 = far_side2 && c->bc != KMC_BC_FF_ONLY;
+    if (far2) {
+        HIPCHK(hipEventRecord(c->ev_setup, s));
+        HIPCHK(hipStreamWaitEvent(c->side2, c->ev_setup, 0));
+        EvalArgs af = eval_args(c, ne, wk.epos4, wk.ecell, wk.eexcl, wk.UI);
+        hipEvent_t e1 = prof ? prof_mark(c, c->side2) : nullptr;
+        c->fns.far(af, c->side2);
+        if (prof) prof_add(c, 1, e1, prof_mark(c, c->side2), ne);
+        HIPCHK(hipEventRecord(c->ev_far, c->side2));
+    }
     if (c->bc != KMC_BC_FF_ONLY) {
         // wait_sweep 2: the near field too waits for the previous step's sweep (only the
         // set-up and the long range overlap it)
@@ -1822,12 +1836,16 @@ static int run_propose_core(kmc_ctx* c, hipStream_t s, int64_t P, int64_t T, dou
             if (c->prof_pairs.ensure(1) == hipSuccess && ne > 0)
                 hipLaunchKernelGGL(k_pair_count, dim3((ne + 255) / 256), dim3(256), 0, s, an, c->prof_pairs.p);
         }
-        // the far field reads the leaf expansions: after the previous step's sweep
-        if (wait_sweep == 1) HIPCHK(hipStreamWaitEvent(s, c->ev_sweep, 0));
-        EvalArgs af = eval_args(c, ne, wk.epos4, wk.ecell, wk.eexcl, wk.UI);
-        hipEvent_t e1 = prof ? prof_mark(c, s) : nullptr;
-        c->fns.far(af, s);
-        if (prof) prof_add(c, 1, e1, prof_mark(c, s), ne);
+        if (far2) {
+            HIPCHK(hipStreamWaitEvent(s, c->ev_far, 0));
+        } else {
+            // the far field reads the leaf expansions: after the previous step's sweep
+            if (wait_sweep == 1) HIPCHK(hipStreamWaitEvent(s, c->ev_sweep, 0));
+            EvalArgs af = eval_args(c, ne, wk.epos4, wk.ecell, wk.eexcl, wk.UI);
+            hipEvent_t e1 = prof ? prof_mark(c, s) : nullptr;
+            c->fns.far(af, s);
+            if (prof) prof_add(c, 1, e1, prof_mark(c, s), ne);
+        }
     }
     if (lr) {
         if (!lr_first) launch_lr();
@@ -2190,7 +2208,9 @@ int kmc_create(kmc_ctx** out, int64_t n, double extent, int32_t r, int32_t L, in
         if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
             hipStreamCreateWithFlags(&c->side2, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&c->ev_fork2, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&c->ev_sweep, hipEventDisableTiming) != hipSuccess)
+            hipEventCreateWithFlags(&c->ev_sweep, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&c->ev_setup, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&c->ev_far, hipEventDisableTiming) != hipSuccess)
             rc = set_err(KMC_E_HIP, "stream");
         for (int i = 0; i < 2 && rc == KMC_OK; i++)
             if (hipEventCreateWithFlags(&c->ev_fork[i], hipEventDisableTiming) != hipSuccess ||
@@ -2231,6 +2251,8 @@ int kmc_destroy(kmc_ctx* c) {
     if (c->side) hipStreamDestroy(c->side);
     if (c->ev_fork2) hipEventDestroy(c->ev_fork2);
     if (c->ev_sweep) hipEventDestroy(c->ev_sweep);
+    if (c->ev_setup) hipEventDestroy(c->ev_setup);
+    if (c->ev_far) hipEventDestroy(c->ev_far);
     if (c->side2) hipStreamDestroy(c->side2);
     for (auto& q : c->prof_pending) {
         hipEventDestroy(q.a);
@@ -2925,6 +2947,7 @@ int kmc_run_steps(kmc_ctx* c, void* stream, int64_t nrun, int64_t nsteps_moves, 
     const int64_t T = P * k;
     if (!step_sized(c, P, T))
         return set_err(KMC_E_STATE, "work buffers not sized: call kmc_propose once with this T before kmc_run_steps");
+    if (pipelined < 0 || pipelined > 3) return set_err(KMC_E_ARG, "pipelined must be 0, 1, 2 or 3");
     hipStream_t s = (hipStream_t)stream;
     const bool pipe = pipelined != 0 && c->bc != KMC_BC_FF_ONLY;
     const bool mixed = precision == KMC_PREC_MIXED;
@@ -2938,7 +2961,8 @@ int kmc_run_steps(kmc_ctx* c, void* stream, int64_t nrun, int64_t nsteps_moves, 
         rc = launch_setup(c, s, P, T, d_pid, d_es, d_new_pos, nullptr);
         if (rc == KMC_OK)
             rc = run_propose_core(c, s, P, T, d_dE, nullptr, mixed, kT, c->rates.p, nullptr, 0,
-                                  (pipe && it > 0) ? (pipelined == 2 ? 2 : 1) : 0, nullptr, pipe);
+                                  (pipe && it > 0) ? (pipelined >= 2 ? 2 : 1) : 0, nullptr, pipe,
+                                  pipe && pipelined == 3);
         if (rc == KMC_OK) {
             hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, s, sa);
             rc = run_accept_tail(c, s, d_counter, pipe);
@@ -2946,9 +2970,11 @@ int kmc_run_steps(kmc_ctx* c, void* stream, int64_t nrun, int64_t nsteps_moves, 
         }
     }
     // join the last deferred sweep into the call's stream, also when a step failed (a
-    // capture must end joined, and later work on s must follow the sweep)
+    // capture must end joined, and later work on s must follow the sweep); mode 3 also
+    // the last far field (side2's work follows the sweeps there)
     if (pipe && done > 0) {
-        const hipError_t e = hipStreamWaitEvent(s, c->ev_sweep, 0);
+        hipError_t e = hipStreamWaitEvent(s, c->ev_sweep, 0);
+        if (e == hipSuccess && pipelined == 3) e = hipStreamWaitEvent(s, c->ev_far, 0);
         if (rc == KMC_OK && e != hipSuccess)
             return set_err(KMC_E_HIP, std::string("hipStreamWaitEvent: ") + hipGetErrorString(e));
     }

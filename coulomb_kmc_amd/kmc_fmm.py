@@ -475,7 +475,8 @@ class KMCFMM(InjectorExtractor):
                   d_new_pos, d_dE, precision=None, pipelined=True, stream=None):
         """nrun x (materialize_moves + step_device) in one call; pipelined (True or 1): the
         accept sweep of a step overlaps the next step's set-up, near field and long range;
-        2: only its set-up and long range."""
+        2: only its set-up and long range; 3: as 2, with the next far field on the sweep's
+        stream behind it, beside the near field."""
         s = stream if stream is not None else self._stream()
         nsteps = d_pids_all.numel() // int(P)
         assert d_disp_all.numel() >= nsteps * int(P) * int(k) * 3

@@ -72,15 +72,15 @@ def test_run_steps_equal_single_steps(bc):
                           ref_w["sel"])
     ref = _state(ref_k, ref_w)
     assert ref[0] == nrun
-    for pipelined in (0, 1, 2):
+    for pipelined in (0, 1, 2, 3):
         kk, w = make()
         kk.run_steps(nrun, P, k, tabs["pids"], tabs["disp"], tabs["us"], 1.0, w["counter"], w["sel"], w["pid"],
                      w["es"], w["new"], w["dE"], pipelined=pipelined)
         _same(_state(kk, w), ref)
         kk.free()
-    # captured (both pipelined modes): 3 steps per graph, replayed twice with eager work
+    # captured (every pipelined mode): 3 steps per graph, replayed twice with eager work
     # between the replays
-    for pipelined in (1, 2):
+    for pipelined in (1, 2, 3):
         kk, w = make()
         s = torch.cuda.Stream()
         g = torch.cuda.CUDAGraph()
