@@ -40,7 +40,7 @@ WORKLOADS = {
     "pbc-100k": dict(workload="pbc N=100000 box 100 leaves 32^3 L=12, 1024x8 proposals", N=100000, E=100.0, r=6,
                      L=12, bc="pbc", P=1024, k=8, radius=1.5, seed=1001, steps=1000, warmup=50, pipeline=2),
     "pbc-2m": dict(workload="pbc N=2000000 box 271.4 leaves 64^3 L=16, 32768x8 proposals", N=2000000, E=271.4,
-                   r=7, L=16, bc="pbc", P=32768, k=8, radius=1.5, seed=1004, steps=100, warmup=20, pipeline=3),
+                   r=7, L=16, bc="pbc", P=32768, k=8, radius=1.5, seed=1004, steps=100, warmup=20, pipeline=4),
 }
 DEFAULT_WORKLOAD = "pbc-2m"
 
@@ -618,12 +618,14 @@ def main():
     ap.add_argument("--sweep-block", type=int, default=-1,
                     help="accept-sweep layout (kmc_set_tiles sweep_block; -1: autotuned): 0 fused, 1 blocked with "
                          "order truncation, 64..256 leaves per workgroup")
-    ap.add_argument("--pipeline", default="auto", choices=("auto", "on", "off", "setup", "setup-far"),
+    ap.add_argument("--pipeline", default="auto", choices=("auto", "on", "off", "setup", "setup-far", "sweep-main"),
                     help="on: kmc_run_steps with each step's sweep beside the next step's near field and long "
                          "range; setup: beside its set-up and long range only; setup-far: as setup, with the next "
-                         "far field behind the sweep on its stream; off: kmc_materialize_moves + "
+                         "far field behind the sweep on its stream; sweep-main: near field, combine, select and "
+                         "sweep on the caller's stream, tail + next set-up + long range on one side stream, far field "
+                         "on another; off: kmc_materialize_moves + "
                          "kmc_step per step; auto: the workload's choice (on for free-1k, setup for pbc-100k, "
-                         "setup-far for pbc-2m, where they measured fastest)")
+                         "sweep-main for pbc-2m, where they measured fastest)")
     ap.add_argument("--probe-traffic", action="store_true",
                     help="only launch the accept sweep and a copy of known size (for rocprofv3 --pmc runs)")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
@@ -802,7 +804,7 @@ def main():
         log("cpu baseline", cpu)
 
     # ---- timed runs: graph of G steps, replayed
-    pipeline = {"auto": cfg["pipeline"], "on": 1, "off": 0, "setup": 2, "setup-far": 3}[args.pipeline]
+    pipeline = {"auto": cfg["pipeline"], "on": 1, "off": 0, "setup": 2, "setup-far": 3, "sweep-main": 4}[args.pipeline]
     stream = torch.cuda.Stream()
     results = {}
     for prec in ("fp64", "mixed"):

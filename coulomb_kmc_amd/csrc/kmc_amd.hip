@@ -2936,6 +2936,62 @@ int kmc_materialize_moves(kmc_ctx* c, void* stream, int64_t nsteps, int64_t P, i
     return KMC_OK;
 }
 
+// Pipelined mode 4: the caller's stream s carries the near field, combine, select and
+// the accept sweep of every step, one after the other with no wait across queues on
+// that chain; side carries the accept tail and, behind it, the next step's
+// materialize, set-up and long range; side2 the far field (after the set-up and the
+// previous sweep, events).  Same kernels, arguments and results as one step at a time.
+static int run_steps_mode4(kmc_ctx* c, hipStream_t s, int64_t nrun, int64_t P, int64_t T, const MatArgs& ma,
+                           const SelectArgs& sa, int64_t* d_pid, int64_t* d_es, double* d_new_pos, double* d_dE,
+                           int64_t* d_counter, double kT, bool mixed) {
+    const int64_t mthreads = std::max(T, P + 1);
+    const int ne = (int)(T + P);
+    const bool lr = (c->bc == KMC_BC_PBC || c->bc == KMC_BC_FF_ONLY) && c->have_lr;
+    const ProposeWork wk = ctx_work(c);
+    HIPCHK(hipEventRecord(c->ev_fork[0], s));
+    HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork[0], 0));
+    int rc = KMC_OK;
+    for (int64_t it = 0; it < nrun && rc == KMC_OK; it++) {
+        // side: materialize, set-up, long range
+        hipLaunchKernelGGL(k_materialize, dim3((mthreads + 255) / 256), dim3(256), 0, c->side, ma);
+        rc = launch_setup(c, c->side, P, T, d_pid, d_es, d_new_pos, nullptr);
+        if (rc) break;
+        HIPCHK(hipEventRecord(c->ev_setup, c->side));
+        if (lr) {
+            c->fns.lr(lr_args(c, P, T), c->side);
+            HIPCHK(hipEventRecord(c->ev_join[0], c->side));
+        }
+        // side2: far field after the set-up and (stream order on s, event) the previous sweep
+        HIPCHK(hipStreamWaitEvent(c->side2, c->ev_setup, 0));
+        if (it > 0) HIPCHK(hipStreamWaitEvent(c->side2, c->ev_sweep, 0));
+        c->fns.far(eval_args(c, ne, wk.epos4, wk.ecell, wk.eexcl, wk.UI), c->side2);
+        HIPCHK(hipEventRecord(c->ev_far, c->side2));
+        // s: near field (after the previous sweep, stream order), combine, select, sweep
+        HIPCHK(hipStreamWaitEvent(s, c->ev_setup, 0));
+        launch_near_any(c, s, eval_args(c, ne, wk.epos4, wk.ecell, wk.eexcl, wk.UD), mixed, (int)P, (int)T, wk);
+        HIPCHK(hipStreamWaitEvent(s, c->ev_far, 0));
+        if (lr) HIPCHK(hipStreamWaitEvent(s, c->ev_join[0], 0));
+        const CombineArgs ca = combine_args(c, P, T, d_dE, nullptr, kT, c->rates.p, nullptr);
+        if (mixed) hipLaunchKernelGGL(k_combine<true>, dim3((T + 63) / 64), dim3(64), 0, s, ca);
+        else hipLaunchKernelGGL(k_combine<false>, dim3((T + 63) / 64), dim3(64), 0, s, ca);
+        hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, s, sa);
+        HIPCHK(hipEventRecord(c->ev_fork[1], s));
+        c->fns.sweep(sweep_rec_args(c), (int)c->ncells, s, c->sweep_block, c->sweep_pairs, c->sweep_split);
+        HIPCHK(hipEventRecord(c->ev_sweep, s));
+        // side: the accept tail after the select (beside the sweep)
+        HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork[1], 0));
+        c->fns.md(tail_args(c, d_counter), c->side);
+        HIPCHK(hipGetLastError());
+    }
+    // join side (its last tail, or the work a failed step left) and side2 into s
+    hipError_t e = hipEventRecord(c->ev_join[1], c->side);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, c->ev_join[1], 0);
+    if (e == hipSuccess) e = hipEventRecord(c->ev_fork2, c->side2);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, c->ev_fork2, 0);
+    if (rc == KMC_OK && e != hipSuccess) return set_err(KMC_E_HIP, std::string("join: ") + hipGetErrorString(e));
+    return rc;
+}
+
 int kmc_run_steps(kmc_ctx* c, void* stream, int64_t nrun, int64_t nsteps_moves, int64_t P, int64_t k,
                   const int64_t* d_pids_all, const double* d_disp_all, int64_t nsteps_uniform,
                   const double* d_uniform, double kT, int64_t* d_counter, int64_t* d_selected, int64_t* d_pid,
@@ -2947,13 +3003,20 @@ int kmc_run_steps(kmc_ctx* c, void* stream, int64_t nrun, int64_t nsteps_moves, 
     const int64_t T = P * k;
     if (!step_sized(c, P, T))
         return set_err(KMC_E_STATE, "work buffers not sized: call kmc_propose once with this T before kmc_run_steps");
-    if (pipelined < 0 || pipelined > 3) return set_err(KMC_E_ARG, "pipelined must be 0, 1, 2 or 3");
+    if (pipelined < 0 || pipelined > 4) return set_err(KMC_E_ARG, "pipelined must be 0, 1, 2, 3 or 4");
     hipStream_t s = (hipStream_t)stream;
     const bool pipe = pipelined != 0 && c->bc != KMC_BC_FF_ONLY;
     const bool mixed = precision == KMC_PREC_MIXED;
     const int64_t mthreads = std::max(T, P + 1);
     const MatArgs ma = mat_args(c, nsteps_moves, P, k, d_pids_all, d_disp_all, d_counter, d_pid, d_es, d_new_pos);
     const SelectArgs sa = select_args(c, P, T, d_pid, d_dE, nsteps_uniform, d_uniform, kT, d_counter, d_selected);
+    if (pipe && pipelined == 4) {
+        const int rc4 = run_steps_mode4(c, s, nrun, P, T, ma, sa, d_pid, d_es, d_new_pos, d_dE, d_counter, kT, mixed);
+        if (rc4) return rc4;
+        c->mirrors_stale = true;
+        HIPCHK(hipGetLastError());
+        return KMC_OK;
+    }
     int rc = KMC_OK;
     int64_t done = 0;   // steps whose sweep may be pending on side2
     for (int64_t it = 0; it < nrun && rc == KMC_OK; it++) {

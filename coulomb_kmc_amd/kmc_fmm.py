@@ -476,7 +476,9 @@ class KMCFMM(InjectorExtractor):
         """nrun x (materialize_moves + step_device) in one call; pipelined (True or 1): the
         accept sweep of a step overlaps the next step's set-up, near field and long range;
         2: only its set-up and long range; 3: as 2, with the next far field on the sweep's
-        stream behind it, beside the near field."""
+        stream behind it, beside the near field; 4: near field, combine, select and sweep
+        on the call's stream, the tail and the next set-up and long range on a side stream,
+        the far field on another."""
         s = stream if stream is not None else self._stream()
         nsteps = d_pids_all.numel() // int(P)
         assert d_disp_all.numel() >= nsteps * int(P) * int(k) * 3

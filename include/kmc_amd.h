@@ -207,7 +207,10 @@ int kmc_materialize_moves(kmc_ctx* ctx, void* stream, int64_t nsteps, int64_t P,
  * for the last sweep before returning (so a captured call is joined).  pipelined = 2:
  * the near field waits for the sweep too (only set-up and long range overlap it).
  * pipelined = 3: as 2, with step i + 1's far field on the sweep's stream right behind
- * the sweep, beside the near field.  Other values: KMC_E_ARG.
+ * the sweep, beside the near field.  pipelined = 4: the call's stream carries every
+ * step's near field, combine, select and sweep; a side stream the accept tail and,
+ * behind it, the next step's materialize, set-up and long range; a second side stream
+ * the far field (after the set-up and the previous sweep).  Other values: KMC_E_ARG.
  * Needs the work buffers sized by a kmc_propose of the same T.
  */
 int kmc_run_steps(kmc_ctx* ctx, void* stream, int64_t nrun, int64_t nsteps_moves, int64_t P, int64_t k,

@@ -72,7 +72,7 @@ def test_run_steps_equal_single_steps(bc):
                           ref_w["sel"])
     ref = _state(ref_k, ref_w)
     assert ref[0] == nrun
-    for pipelined in (0, 1, 2, 3):
+    for pipelined in (0, 1, 2, 3, 4):
         kk, w = make()
         kk.run_steps(nrun, P, k, tabs["pids"], tabs["disp"], tabs["us"], 1.0, w["counter"], w["sel"], w["pid"],
                      w["es"], w["new"], w["dE"], pipelined=pipelined)
@@ -80,7 +80,7 @@ def test_run_steps_equal_single_steps(bc):
         kk.free()
     # captured (every pipelined mode): 3 steps per graph, replayed twice with eager work
     # between the replays
-    for pipelined in (1, 2, 3):
+    for pipelined in (1, 2, 3, 4):
         kk, w = make()
         s = torch.cuda.Stream()
         g = torch.cuda.CUDAGraph()
