@@ -2957,18 +2957,21 @@ static int run_steps_mode4(kmc_ctx* c, hipStream_t s, int64_t nrun, int64_t P, i
         rc = launch_setup(c, c->side, P, T, d_pid, d_es, d_new_pos, nullptr);
         if (rc) break;
         HIPCHK(hipEventRecord(c->ev_setup, c->side));
-        if (lr) {
-            c->fns.lr(lr_args(c, P, T), c->side);
-            HIPCHK(hipEventRecord(c->ev_join[0], c->side));
-        }
-        // side2: far field after the set-up and (stream order on s, event) the previous sweep
+        // s: near field (after the previous sweep, stream order), recorded first of the
+        // set-up's dependents
+        HIPCHK(hipStreamWaitEvent(s, c->ev_setup, 0));
+        launch_near_any(c, s, eval_args(c, ne, wk.epos4, wk.ecell, wk.eexcl, wk.UD), mixed, (int)P, (int)T, wk);
+        // side2: far field after the set-up and (event) the previous sweep
         HIPCHK(hipStreamWaitEvent(c->side2, c->ev_setup, 0));
         if (it > 0) HIPCHK(hipStreamWaitEvent(c->side2, c->ev_sweep, 0));
         c->fns.far(eval_args(c, ne, wk.epos4, wk.ecell, wk.eexcl, wk.UI), c->side2);
         HIPCHK(hipEventRecord(c->ev_far, c->side2));
-        // s: near field (after the previous sweep, stream order), combine, select, sweep
-        HIPCHK(hipStreamWaitEvent(s, c->ev_setup, 0));
-        launch_near_any(c, s, eval_args(c, ne, wk.epos4, wk.ecell, wk.eexcl, wk.UD), mixed, (int)P, (int)T, wk);
+        // side: long range
+        if (lr) {
+            c->fns.lr(lr_args(c, P, T), c->side);
+            HIPCHK(hipEventRecord(c->ev_join[0], c->side));
+        }
+        // s: combine, select, sweep
         HIPCHK(hipStreamWaitEvent(s, c->ev_far, 0));
         if (lr) HIPCHK(hipStreamWaitEvent(s, c->ev_join[0], 0));
         const CombineArgs ca = combine_args(c, P, T, d_dE, nullptr, kT, c->rates.p, nullptr);
