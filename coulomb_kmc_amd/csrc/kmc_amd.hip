@@ -1462,7 +1462,7 @@ static int ensure_eval(kmc_ctx* c, int64_t T, int64_t P) {
     HIPCHK(c->eexcl.ensure(ne));
     HIPCHK(c->owner.ensure((size_t)std::max<int64_t>(T, 1)));
     HIPCHK(c->UD.ensure(ne));
-    HIPCHK(c->UI.ensure(ne * (size_t)((c->L + 1) / 2)));
+    HIPCHK(c->UI.ensure(ne * (size_t)far_parts(c->L)));
     HIPCHK(c->lrd.ensure((size_t)12 * std::max<int64_t>(T, 1)));
     HIPCHK(c->rates.ensure((size_t)std::max<int64_t>(T, 1)));
     HIPCHK(c->bsum.ensure((size_t)std::max<int64_t>((T + 63) / 64, 1)));
@@ -1635,7 +1635,7 @@ static CombineArgs combine_args(kmc_ctx* c, int64_t P, int64_t T, double* d_out,
     ca.owner = wk.owner;
     ca.UD = wk.UD;
     ca.UI = wk.UI;
-    ca.nfar = (c->L + 1) / 2;
+    ca.nfar = far_parts(c->L);
     ca.lpart = lr ? wk.lrd : nullptr;
     ca.cpart = lr ? wk.cold : nullptr;
     ca.ncpart = 12 * lr_split(P, w);
@@ -2018,7 +2018,7 @@ static int entry_energies(kmc_ctx* c, hipStream_t s, int n, const double4* pos4,
                           const int32_t* excl, double* out, DBuf<double>& ud, DBuf<double>& ui, DBuf<double>& lr,
                           double eu = -1.0) {
     if (eu < 0.0) eu = c->eu;
-    const int nfar = (c->L + 1) / 2;
+    const int nfar = far_parts(c->L);
     const bool near_far = c->bc != KMC_BC_FF_ONLY;
     const bool lrt = uses_lr(c);
     if (near_far) {
@@ -2126,7 +2126,7 @@ static int scratch_ptr(void** out) {
 // Bytes of scratch a propose of P moving charges and T proposals takes, and (base !=
 // null) the carve of the evaluation arrays from base, each 256-byte aligned.
 static size_t carve_work(const kmc_ctx* c, int64_t P, int64_t T, unsigned char* base, ProposeWork* w) {
-    const size_t ne = (size_t)(T + P), nfar = (size_t)((c->L + 1) / 2);
+    const size_t ne = (size_t)(T + P), nfar = (size_t)far_parts(c->L);
     size_t off = 0;
     auto take = [&](size_t bytes) {
         unsigned char* p = base ? base + off : nullptr;
@@ -2578,7 +2578,7 @@ int kmc_initialise(kmc_ctx* c, void* stream, double* h_energy) {
     DBuf<double> ud, ui, esr;
     HIPCHK(excl.ensure(n));
     HIPCHK(ud.ensure(n));
-    HIPCHK(ui.ensure((size_t)n * ((c->L + 1) / 2)));
+    HIPCHK(ui.ensure((size_t)n * far_parts(c->L)));
     HIPCHK(esr.ensure(1));
     {
         std::vector<int32_t> ids(n);
@@ -2590,7 +2590,7 @@ int kmc_initialise(kmc_ctx* c, void* stream, double* h_energy) {
             launch_near(an, c->near_lanes, false, s);
             EvalArgs af = eval_args(c, n, c->ppos4.p, c->pcell.p, excl.p, ui.p);
             c->fns.far(af, s);
-            hipLaunchKernelGGL(k_half_sum, dim3(1), dim3(256), 0, s, n, ud.p, ui.p, (c->L + 1) / 2, esr.p);
+            hipLaunchKernelGGL(k_half_sum, dim3(1), dim3(256), 0, s, n, ud.p, ui.p, far_parts(c->L), esr.p);
         }
         hipLaunchKernelGGL(k_finish_energy, dim3(1), dim3(1), 0, s, esr.p, c->elr.p, c->eu, c->bc, c->energy.p);
         HIPCHK(hipGetLastError());
@@ -3205,7 +3205,7 @@ int kmc_eval_field(kmc_ctx* c, void* stream, int64_t n, const double* d_points, 
     if (n == 0) return KMC_OK;
     if (!d_points || !d_out) return set_err(KMC_E_ARG, "null pointer");
     hipStream_t s = (hipStream_t)stream;
-    const int ni = (int)n, nfar = (c->L + 1) / 2;
+    const int ni = (int)n, nfar = far_parts(c->L);
     DBuf<double4> pos4;
     DBuf<int32_t> cell, excl;
     DBuf<double> ud, ui, lr;

@@ -1073,23 +1073,43 @@ __device__ __forceinline__ double far_dispatch(int pair, double dx, double dy, d
 
 // one lane per evaluation point; the column pairs (m, L-1-m) of 64 points are the
 // waves of one workgroup (pair = blockIdx.y * blockDim.y + threadIdx.y), so the
-// waves reading one leaf expansion run together and share its cache lines;
-// out[pair * n + i]
+// waves reading one leaf expansion run together and share its cache lines.  The
+// workgroup's pair terms are summed in pair order with a compensated sum (through LDS)
+// and written as its (sum, correction): out[(2 blockIdx.y + {0, 1}) n + i], so a point
+// has far_parts(L) partials (2 up to L = 16, 4 above).
+constexpr int FAR_PY_MAX = 8;   // pair waves per workgroup
+__host__ __device__ constexpr int far_pairs_per_group(int L) {
+    return (L + 1) / 2 <= FAR_PY_MAX ? (L + 1) / 2 : ((L + 1) / 2 + 1) / 2;
+}
+__host__ __device__ constexpr int far_parts(int L) {
+    return 2 * (((L + 1) / 2 + far_pairs_per_group(L) - 1) / far_pairs_per_group(L));
+}
 template <int L>
 __device__ __forceinline__ void far_body(const EvalArgs& a) {
+    __shared__ double s_part[FAR_PY_MAX][64];
     const int i = blockIdx.x * 64 + threadIdx.x;
     const int pair = __builtin_amdgcn_readfirstlane((int)(blockIdx.y * blockDim.y + threadIdx.y));
-    if (i >= a.n || pair >= (L + 1) / 2) return;
-    const double4 p = a.pos4[i];
-    const int c = a.cell[i];
-    const int n = a.g.ncps;
-    const int tx = c % n, ty = (c / n) % n, tz = c / (n * n);
-    const double dx = p.x - (a.g.c0 + tx * a.g.w);
-    const double dy = p.y - (a.g.c0 + ty * a.g.w);
-    const double dz = p.z - (a.g.c0 + tz * a.g.w);
-    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-    const double2* le = a.lexp + (size_t)c * nhalf(L);
-    a.out[(size_t)pair * a.n + i] = p.w * far_dispatch<L, 0>(pair, dx, dy, dz, r2, le);
+    double v = 0.0;
+    if (i < a.n && pair < (L + 1) / 2) {
+        const double4 p = a.pos4[i];
+        const int c = a.cell[i];
+        const int n = a.g.ncps;
+        const int tx = c % n, ty = (c / n) % n, tz = c / (n * n);
+        const double dx = p.x - (a.g.c0 + tx * a.g.w);
+        const double dy = p.y - (a.g.c0 + ty * a.g.w);
+        const double dz = p.z - (a.g.c0 + tz * a.g.w);
+        const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+        const double2* le = a.lexp + (size_t)c * nhalf(L);
+        v = p.w * far_dispatch<L, 0>(pair, dx, dy, dz, r2, le);
+    }
+    s_part[threadIdx.y][threadIdx.x] = v;
+    __syncthreads();
+    if (threadIdx.y == 0 && i < a.n) {
+        CSum acc;
+        for (int w = 0; w < (int)blockDim.y; w++) acc.add(s_part[w][threadIdx.x]);
+        a.out[(size_t)(2 * blockIdx.y) * a.n + i] = acc.s;
+        a.out[(size_t)(2 * blockIdx.y + 1) * a.n + i] = acc.c;
+    }
 }
 template <int L>
 __global__ void __launch_bounds__(512) k_far(EvalArgs a) { far_body<L>(a); }
@@ -1362,7 +1382,7 @@ struct CombineArgs {
     const double4* epos4;
     const int32_t* owner;
     const double* UD;
-    const double* UI;            // far-field partials [nfar][T+P]
+    const double* UI;            // far-field partials [nfar][T+P] (nfar = far_parts(L))
     int nfar;
     const double* lpart;         // long range per proposal partials [12][T], or null
     const double* cpart;         // long range per moving charge partials [ncpart][P]
@@ -2346,7 +2366,7 @@ template <int L>
 static void launch_far(EvalArgs a, hipStream_t s) {
     if (a.n <= 0) return;
     constexpr int npair = (L + 1) / 2;
-    constexpr int py = npair <= 8 ? npair : (npair + 1) / 2;   // pairs per workgroup, <= 8 waves
+    constexpr int py = far_pairs_per_group(L);   // pairs per workgroup, <= 8 waves
     hipLaunchKernelGGL(k_far<L>, dim3((a.n + 63) / 64, (npair + py - 1) / py), dim3(64, py), 0, s, a);
 }
 template <int L>
@@ -2380,7 +2400,7 @@ typedef void (*BTailLaunch)(const TailArgs*, BatchShape, hipStream_t);
 template <int L>
 static void blaunch_far(const EvalArgs* t, BatchShape b, hipStream_t s) {
     constexpr int npair = (L + 1) / 2;
-    constexpr int py = npair <= 8 ? npair : (npair + 1) / 2;
+    constexpr int py = far_pairs_per_group(L);
     hipLaunchKernelGGL(kb_far<L>, dim3((b.max_ne + 63) / 64, (npair + py - 1) / py, b.n), dim3(64, py), 0, s, t);
 }
 template <int L>

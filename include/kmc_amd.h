@@ -125,12 +125,13 @@ int kmc_propose(kmc_ctx* ctx, void* stream, int64_t P, int64_t T, const int64_t*
  * a fixed device array of KMC_NOALLOC_SCRATCH_BYTES that the library holds on each GPU
  * it runs on until the process ends; the call makes no hipMalloc / hipFree.  Returns
  * KMC_E_ARG (the message gives the bytes needed) when P moving charges and T proposals
- * need more scratch than that: (T + P)(48 + 8 ceil(L/2)) + 4 T + 8 P bytes, plus
+ * need more scratch than that: (T + P)(48 + 8 f) + 4 T + 8 P bytes (f = 2 far-field
+ * partials per point up to L = 16, 4 above), plus
  * 96 T + 96 P (nb + r) for pbc / ff-only (nb = 22 at L = 12, 36 at L = 16; r = 4 below
  * 4096 moving charges, else 1), plus 8 (T + P) + 132 nt + 4 when the leaf grid has a
  * multiple of 4 leaves per side (nt = leaves / 64 tiles, for the tile-sorted near
- * field), each array rounded up to 256 bytes: 4.2 MiB for 1024 charges x 8 proposals
- * at L = 12 on 32^3 leaves (pbc), 21 MiB for 4096 x 8 at L = 16 on 16^3 leaves.  Every call on one GPU shares that GPU's array (the
+ * field), each array rounded up to 256 bytes: 3.9 MiB for 1024 charges x 8 proposals
+ * at L = 12 on 32^3 leaves (pbc), 19.6 MiB for 4096 x 8 at L = 16 on 16^3 leaves.  Every call on one GPU shares that GPU's array (the
  * last call's contents stay in it): calls that overlap on one GPU (several host
  * threads or streams) race.  Synchronises the stream (bad-move check).
  */
