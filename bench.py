@@ -154,8 +154,9 @@ def kernel_rooflines(kt, N, ncells, L, P, T, bc, sweep_flops, sweep_bytes, leave
     stencils cover (leaves[1], counted on the host from a step's points);
     far: 9 nh flops per point; bytes 40 per point and 16 nh per distinct leaf holding a
     point (leaves[0]);
-    sweep: as in the roofline line.  The long-range kernels are bound by LDS, not by
-    HBM or the FP64 rate, and are reported as durations only."""
+    sweep: as in the roofline line.  The long-range kernels read their operator through
+    the scalar cache and wait on it and on memory latency, not on HBM bandwidth or the
+    FP64 rate; they are reported as durations only."""
     nh = L * (L + 1) // 2
     rho = N / ncells
     ne = T + P
