@@ -1126,48 +1126,69 @@ __global__ void __launch_bounds__(512) kb_far(const EvalArgs* __restrict__ args)
 // leaves around the entry's leaf
 template <int G, bool MIXED>
 __device__ __forceinline__ CSum near_point(const EvalArgs& a, int i, int lane) {
-    // The excluded charge (the moving one) is found by its position: its slot holds
-    // exactly the position in ppos4, and no other charge can sit at that same point
-    // (their pair energy would be infinite), so the slots' ids are never loaded - a
-    // 4-byte id load touches as many cache lines as the 32-byte record load.
+    // The excluded charge (the moving one, for a moving charge's old position) is the
+    // one slot at distance exactly 0 from the point: its slot holds exactly the point's
+    // position, and no other charge can sit at that same point (their pair energy would
+    // be infinite).  For such points r2 gets 1e-30 added (below the rounding of any
+    // other pair's r2, and a normal FP32 number for the mixed 1/sqrt seed), so that pair
+    // stays finite, and its charge is taken as 0: every term is then added without a
+    // branch, the excluded one as an exact 0 (the compensated sum is unchanged by it).
+    // No slot id is loaded: a 4-byte id load touches as many cache lines as the 32-byte
+    // record load.
     CSum acc;
     {
         const double4 p = a.pos4[i];
         const int c = a.cell[i];
         const int ex = a.excl[i];
         const bool has_ex = ex >= 0 && ex < a.np;
-        double ex_x = 0.0, ex_y = 0.0, ex_z = 0.0;
-        if (has_ex) {
-            const double4 e4 = a.ppos4[ex];
-            ex_x = e4.x;
-            ex_y = e4.y;
-            ex_z = e4.z;
-        }
+        const double tiny = has_ex ? 1e-30 : 0.0;
         const int n = a.g.ncps;
         const int tx = c % n, ty = (c / n) % n, tz = c / (n * n);
         const bool periodic = a.g.bc != KMC_BC_FREE_SPACE;
-        for (int k = 0; k < 27; k++) {
-            int ox, oy, oz;
-            stencil(k, ox, oy, oz);
-            int nx = tx + ox, ny = ty + oy, nz = tz + oz;
-            double shx = 0.0, shy = 0.0, shz = 0.0;
-            if (periodic) {
-                if (nx < 0) { nx += n; shx = -a.g.E; } else if (nx >= n) { nx -= n; shx = a.g.E; }
-                if (ny < 0) { ny += n; shy = -a.g.E; } else if (ny >= n) { ny -= n; shy = a.g.E; }
-                if (nz < 0) { nz += n; shz = -a.g.E; } else if (nz >= n) { nz -= n; shz = a.g.E; }
-            } else if (nx < 0 || nx >= n || ny < 0 || ny >= n || nz < 0 || nz >= n) {
-                continue;
+        // per axis, the leaf index (wrapped, or -1 outside a free-space box) and the
+        // image shift of offsets -1, 0, +1, worked out once per point; the 27 leaves are
+        // then walked in stencil order (x fastest, then y, then z) with no per-leaf
+        // wrapping
+        int xi[3], yi[3], zi[3];
+        double xs[3], ys[3], zs[3];
+        auto axis = [&](int t, int* id, double* sh) {
+#pragma unroll
+            for (int o = 0; o < 3; o++) {
+                int v = t + o - 1;
+                double h = 0.0;
+                if (periodic) {
+                    if (v < 0) { v += n; h = -a.g.E; } else if (v >= n) { v -= n; h = a.g.E; }
+                } else if (v < 0 || v >= n) {
+                    v = -1;
+                }
+                id[o] = v;
+                sh[o] = h;
             }
-            const int nc = nx + n * (ny + n * nz);
-            const int occ = min(a.occ[nc], a.occmax);
-            const double px = p.x - shx, py = p.y - shy, pz = p.z - shz;
-            const size_t base = (size_t)nc * a.occmax;
-            for (int s = lane; s < occ; s += G) {
-                const double4 q = a.slot_pos[base + s];
-                const double dx = px - q.x, dy = py - q.y, dz = pz - q.z;
-                const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-                const double contrib = q.w * rsqrt_sel<MIXED>(r2);
-                if (!(has_ex && q.x == ex_x && q.y == ex_y && q.z == ex_z)) acc.add(contrib);
+        };
+        axis(tx, xi, xs);
+        axis(ty, yi, ys);
+        axis(tz, zi, zs);
+#pragma unroll
+        for (int kz = 0; kz < 3; kz++) {
+#pragma unroll
+            for (int ky = 0; ky < 3; ky++) {
+                const int row = n * (yi[ky] + n * zi[kz]);
+                const double py = p.y - ys[ky], pz = p.z - zs[kz];
+#pragma unroll
+                for (int kx = 0; kx < 3; kx++) {
+                    if (xi[kx] < 0 || yi[ky] < 0 || zi[kz] < 0) continue;   // free space: outside the box
+                    const int nc = xi[kx] + row;
+                    const int occ = min(a.occ[nc], a.occmax);
+                    const double px = p.x - xs[kx];
+                    const size_t base = (size_t)nc * a.occmax;
+                    for (int s = lane; s < occ; s += G) {
+                        const double4 q = a.slot_pos[base + s];
+                        const double dx = px - q.x, dy = py - q.y, dz = pz - q.z;
+                        const double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, tiny)));
+                        const double qs = (has_ex && r2 == tiny) ? 0.0 : q.w;
+                        acc.add(qs * rsqrt_sel<MIXED>(r2));
+                    }
+                }
             }
         }
     }
