@@ -772,6 +772,166 @@ __device__ __forceinline__ void block_image(double dx, double dy, double dz, dou
     if constexpr (PP::B > PP::A) sweep_group_cols<L, PP::B, 1, 1, LC>(&zr, &ir2, &u, &base, ar + 1, ai + 1);
 }
 
+// Truncated images in wave pairs.  Waves G and G^1 (pairs G, NPAIR-1-G and G^1,
+// NPAIR-1-(G^1)) share the four column pairs P0, P0+1, Q0, Q0+1 (P0 = G & ~1,
+// Q0 = NPAIR-2-P0); for the images of the truncated buckets wave G evaluates all four
+// for source G & 1 only (old position: even G, new: odd), so each source image's
+// displacement, 1/r and weights are worked out by one wave of the pair instead of both.
+// The two waves then swap the sums of each other's pairs through LDS.  The waves pair
+// only when every wave of the workgroup has a partner with two distinct column pairs
+// (NPAIR a multiple of 4: L = 15, 16 among the orders with truncated buckets): a wave
+// left alone would still evaluate both sources and hold the workgroup at the swap
+// (L = 12, three waves: the same sweep time, 5% slower 100k-charge steps).
+template <int L, int G>
+__host__ __device__ constexpr bool block_paired() {
+    constexpr int NPAIR = (L + 1) / 2;
+    return L > sweep_bucket(0, L) && NPAIR % 4 == 0 && G < NPAIR / 2;
+}
+// entries of pair P's two columns below order LC (the swap's message for that pair)
+template <int L, int P, int LC>
+__host__ __device__ constexpr int pair_entries() {
+    constexpr int M1 = P, M2 = L - 1 - P;
+    return (M1 < LC ? LC - M1 : 0) + ((M2 != M1 && M2 < LC) ? LC - M2 : 0);
+}
+// largest truncated bucket (orders below it; 0: none below L)
+template <int L>
+__host__ __device__ constexpr int sweep_top_trunc() {
+    int t = 0;
+    for (int i = 0; i < SWEEP_NBUCKET; i++)
+        if (sweep_bucket(i, L) < L) t = sweep_bucket(i, L);
+    return t;
+}
+// bucket index of the first bucket holding every order
+template <int L>
+__host__ __device__ constexpr int sweep_full_bucket() {
+    return sweep_bucket_index(L, L);
+}
+// one source image into the four column pairs of wave pair P0 (rows P0, P0+1, Q0, Q0+1)
+template <int L, int P0, int LC>
+__device__ __forceinline__ void block_image_quad(double dx, double dy, double dz, double qe,
+                                                 double (*ar)[L + 1], double (*ai)[L + 1]) {
+    constexpr int Q0 = (L + 1) / 2 - 2 - P0;
+    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    const double ir = rsqrt_fp64(r2);
+    const double ir2 = ir * ir;
+    const double zr = dz * ir2;
+    const Cplx u = {dx * ir2, -dy * ir2};     // conj((x+iy)/r^2)
+    const double base = qe * ir;
+    sweep_group_cols<L, P0, 2, 1, LC>(&zr, &ir2, &u, &base, ar, ai);
+    sweep_group_cols<L, Q0, 2, 1, LC>(&zr, &ir2, &u, &base, ar + 2, ai + 2);
+}
+// the truncated buckets' images of one source (in bucket, then stencil order) for a
+// paired wave: all four pairs of the wave pair, rows P0, P0+1, Q0, Q0+1
+template <int L, int G, bool PER, int I>
+__device__ __forceinline__ void sweep_block_trunc(const SweepArgs& a, double dx0, double dy0, double dz0,
+                                                  double q, const int* lists, double (*ar)[L + 1],
+                                                  double (*ai)[L + 1]) {
+    constexpr int LC = sweep_bucket(I, L);
+    if constexpr (I < SWEEP_NBUCKET && LC < L) {
+        constexpr int P0 = G & ~1;
+        if constexpr (P0 < LC) {
+            constexpr int R = PER ? 1 : 0;
+            const double E = a.g.E;
+            const int cnt = __builtin_amdgcn_readfirstlane(lists[SWEEP_NBUCKET * SWEEP_LIST + I]);
+#pragma unroll 1
+            for (int i = 0; i < cnt; i++) {
+                const int e = __builtin_amdgcn_readfirstlane(lists[I * SWEEP_LIST + i]);
+                const int k = e & 255, lc = e >> 8;
+                if (lc <= P0) continue;   // (the image does not reach the pair's columns, the lowest is P0)
+                const int ox = k % (2 * R + 1) - R, oy = (k / (2 * R + 1)) % (2 * R + 1) - R,
+                          oz = k / ((2 * R + 1) * (2 * R + 1)) - R;
+                block_image_quad<L, P0, LC>(fma((double)ox, E, dx0), fma((double)oy, E, dy0),
+                                            fma((double)oz, E, dz0), q, ar, ai);
+            }
+        }
+        sweep_block_trunc<L, G, PER, I + 1>(a, dx0, dy0, dz0, q, lists, ar, ai);
+    }
+}
+// one pair's entries below order LC between a row and an LDS slot: PUT copies the row
+// out, otherwise the slot is added to the row
+template <int L, int P, int LC, bool PUT>
+__device__ __forceinline__ void swap_pair(double2* slot, double* ar, double* ai) {
+    constexpr int M1 = P, M2 = L - 1 - P;
+    constexpr int N1 = M1 < LC ? LC - M1 : 0;
+    constexpr int N2 = (M2 != M1 && M2 < LC) ? LC - M2 : 0;
+#pragma unroll
+    for (int k = 0; k < N1; k++) {
+        if constexpr (PUT) {
+            slot[k] = make_double2(ar[k], ai[k]);
+        } else {
+            const double2 v = slot[k];
+            ar[k] += v.x;
+            ai[k] += v.y;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < N2; k++) {
+        if constexpr (PUT) {
+            slot[N1 + k] = make_double2(ar[L - M1 + k], ai[L - M1 + k]);
+        } else {
+            const double2 v = slot[N1 + k];
+            ar[L - M1 + k] += v.x;
+            ai[L - M1 + k] += v.y;
+        }
+    }
+}
+// LDS slot size of the swap (entries of the larger message, conflict-free b128 rows)
+template <int L>
+__host__ __device__ constexpr int swap_stride() {
+    constexpr int LT = sweep_top_trunc<L>();
+    int m = 0;
+    for (int p = 0; p < (L + 1) / 2; p++) {
+        const int m1 = p, m2 = L - 1 - p;
+        const int n = (m1 < LT ? LT - m1 : 0) + ((m2 != m1 && m2 < LT) ? LT - m2 : 0);
+        m = n > m ? n : m;
+    }
+    return stage_stride(2 * m);
+}
+// the swap: each paired wave writes the sums of its partner's two pairs (rows of
+// pairs G^1 and NPAIR-1-(G^1)) to its LDS slots and adds the partner's slots (the
+// other source's sums of its own pairs) to its rows; 32 leaves at a time.  Every wave
+// of the workgroup passes the same four barriers.
+template <int L, int G>
+__device__ __forceinline__ void sweep_block_swap(double2* stage, double (*ar)[L + 1], double (*ai)[L + 1]) {
+    constexpr int LT = sweep_top_trunc<L>();
+    constexpr int XS = swap_stride<L>();
+    constexpr int P0 = G & ~1, Q0 = (L + 1) / 2 - 2 - P0;
+    constexpr int p = G & 1;
+    static_assert(((G | 1) + 1) * XS <= stage_stride(nhalf(L)), "the swap's slots exceed the staging buffer");
+    // rows: 0 pair P0, 1 pair P0+1, 2 pair Q0, 3 pair Q0+1; wave P0 holds pairs P0 and
+    // Q0+1, wave P0+1 pairs P0+1 and Q0
+    constexpr int OWN0 = p ? 1 : 0, OWN1 = p ? 2 : 3, OTH0 = p ? 0 : 1, OTH1 = p ? 3 : 2;
+    constexpr int OWN0P = p ? P0 + 1 : P0, OWN1P = p ? Q0 : Q0 + 1;
+    constexpr int OTH0P = p ? P0 : P0 + 1, OTH1P = p ? Q0 + 1 : Q0;
+    constexpr int N_OTH0 = pair_entries<L, OTH0P, LT>(), N_OWN0 = pair_entries<L, OWN0P, LT>();
+    const int lane = threadIdx.x;
+#pragma unroll 1
+    for (int half = 0; half < 2; half++) {
+        const bool mine = (lane >> 5) == half;
+        if (mine) {
+            double2* slot = stage + (G * 32 + (lane & 31)) * XS;
+            swap_pair<L, OTH0P, LT, true>(slot, ar[OTH0], ai[OTH0]);
+            swap_pair<L, OTH1P, LT, true>(slot + N_OTH0, ar[OTH1], ai[OTH1]);
+        }
+        __syncthreads();
+        if (mine) {
+            double2* slot = stage + ((G ^ 1) * 32 + (lane & 31)) * XS;
+            swap_pair<L, OWN0P, LT, false>(slot, ar[OWN0], ai[OWN0]);
+            swap_pair<L, OWN1P, LT, false>(slot + N_OWN0, ar[OWN1], ai[OWN1]);
+        }
+        __syncthreads();
+    }
+}
+// truncated images in every bucket below L, from the block's lists
+template <int L>
+__device__ __forceinline__ int sweep_trunc_count(const int* lists) {
+    int t = 0;
+#pragma unroll
+    for (int i = 0; i < SWEEP_NBUCKET; i++)
+        if (sweep_bucket(i, L) < L) t += lists[SWEEP_NBUCKET * SWEEP_LIST + i];
+    return __builtin_amdgcn_readfirstlane(t);
+}
+
 // the images of bucket I (truncation order lc in (bucket I-1, bucket I]) of both sources,
 // in stencil order, for the lane's leaf
 template <int L, int G, bool PER, int I>
@@ -818,7 +978,7 @@ __device__ __forceinline__ void sweep_block_images(const SweepArgs& a, const dou
 // branch-free columns), in stencil order within a bucket.
 template <int L, int G, int NP, int NI, bool PER>
 __device__ __forceinline__ void sweep_block_group(const SweepArgs& a, int tx, int ty, int tz, const int* lists,
-                                                  double (*ar)[L + 1], double (*ai)[L + 1]) {
+                                                  double2* stage, double (*ar)[L + 1], double (*ai)[L + 1]) {
     static_assert(NI == 1 && NP == 2, "the blocked sweep: one source image per pass, two column pairs per lane");
     const int n = a.g.ncps;
     const double cx = a.g.c0 + tx * a.g.w;
@@ -835,7 +995,28 @@ __device__ __forceinline__ void sweep_block_group(const SweepArgs& a, int tx, in
         dx0[s] = p[0] - cx; dy0[s] = p[1] - cy; dz0[s] = p[2] - cz;
         dcx[s] = sc % n - tx; dcy[s] = (sc / n) % n - ty; dcz[s] = sc / (n * n) - tz;
     }
-    sweep_block_images<L, G, PER, 0>(a, dx0, dy0, dz0, sq, dcx, dcy, dcz, lists, ar, ai);
+    if constexpr (block_paired<L, G>()) {   // (every wave of the workgroup, or none)
+        // truncated buckets: source G & 1, four pairs, rows P0, P0+1, Q0, Q0+1; then the
+        // swap; then this wave's own pairs (rows 0, 1) take the full bucket's images
+        constexpr int p = G & 1;
+        constexpr int W = L + 1;
+        double tr[4][W], ti[4][W];
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int k = 0; k < W; k++) { tr[j][k] = 0.0; ti[j][k] = 0.0; }
+        sweep_block_trunc<L, G, PER, 0>(a, dx0[p], dy0[p], dz0[p], sq[p], lists, tr, ti);
+        if (sweep_trunc_count<L>(lists) > 0) sweep_block_swap<L, G>(stage, tr, ti);
+        constexpr int OWN0 = p ? 1 : 0, OWN1 = p ? 2 : 3;
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            ar[0][k] = tr[OWN0][k]; ai[0][k] = ti[OWN0][k];
+            ar[1][k] = tr[OWN1][k]; ai[1][k] = ti[OWN1][k];
+        }
+        sweep_block_images<L, G, PER, sweep_full_bucket<L>()>(a, dx0, dy0, dz0, sq, dcx, dcy, dcz, lists, ar, ai);
+    } else {
+        sweep_block_images<L, G, PER, 0>(a, dx0, dy0, dz0, sq, dcx, dcy, dcz, lists, ar, ai);
+    }
 }
 // the scaled sums of wave G's columns for the lane's leaf, staged through LDS with the
 // other waves' (two halves of 32 leaves), added to the block's expansions with
@@ -903,7 +1084,7 @@ __device__ __forceinline__ void sweep_block_wave(const SweepArgs& a, int tx, int
     for (int j = 0; j < NP; j++)
 #pragma unroll
         for (int k = 0; k < W; k++) { ar[j][k] = 0.0; ai[j][k] = 0.0; }
-    sweep_block_group<L, G, NP, NI, PER>(a, tx, ty, tz, lists, ar, ai);
+    sweep_block_group<L, G, NP, NI, PER>(a, tx, ty, tz, lists, stage, ar, ai);
     sweep_block_update<L, G>(a, ar, ai, stage, cell_base, nh);
 }
 template <int L, int NP, int NI, bool PER, int G0>
