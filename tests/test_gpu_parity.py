@@ -398,16 +398,18 @@ def test_use_before_initialise_raises():
     kmc.free()
 
 
-@pytest.mark.parametrize("bc", ("pbc", "free_space"))
-def test_blocked_sweep_truncation_matches_oracle(bc):
+@pytest.mark.parametrize("bc,L", [("pbc", 16), ("free_space", 16), ("pbc", 15)])
+def test_blocked_sweep_truncation_matches_oracle(bc, L):
     """The blocked accept sweep (sweep_block 1: a 4x4x4 block of leaves per workgroup,
     orders of each source image dropped where they change the block's potential by less
     than 2^-60 per unit charge) in a box wide enough that most images lose orders:
     expansions after accepts against the oracle's full P2L sweep (1e-11 of their
-    scale), then dE of new proposals against the extended reference (1e-10)."""
+    scale), then dE of new proposals against the extended reference (1e-10).  L = 15
+    and 16 take the truncated buckets' images in wave pairs (each wave one source, all
+    four column pairs of its group; L = 15 with a one-column middle pair)."""
     from coulomb_kmc_amd import KMCFMM
     from coulomb_kmc_amd.kmc_fmm import lr_operator_reference_layout
-    N, E, r, L = 3000, 60.0, 5, 16
+    N, E, r = 3000, 60.0, 5
     rng = np.random.default_rng(77)
     pos, q = neutral_system(rng, N, E)
     kmc = KMCFMM(pos.copy(), q.reshape(-1, 1), E, r=r, l=L, boundary_condition=bc, max_move=1.5,

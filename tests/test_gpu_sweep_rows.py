@@ -7,7 +7,8 @@ the fused layout's (every order of every image, no truncation) to rounding.  Tha
 skipped entries are not read or written is what the HBM counters show (FETCH_SIZE +
 WRITE_SIZE per launch against the algorithmic bytes).  The boxes are large (free space: 10^4 per
 side on 16^3 leaves; periodic, where an image of each source is within half a box of
-every block: 10^5 per side on 32^3 leaves; L = 16) so that the bound
+every block: 10^5 per side on 32^3 leaves; L = 16, and 15 for the wave pairs with a
+one-column middle pair) so that the bound
 (rho/D)^lc / (D - rho) <= 2^-60, which is absolute, keeps 12 or fewer orders for many
 blocks; the test checks that such blocks exist."""
 
@@ -60,11 +61,11 @@ def block_orders(E, ncps, L, periodic, old, new):
     return out
 
 
-@pytest.mark.parametrize("bc,E,r", [("pbc", 1.0e5, 6), ("free_space", 1.0e4, 5)])
-def test_blocked_sweep_touches_only_reached_orders(bc, E, r):
+@pytest.mark.parametrize("bc,E,r,L", [("pbc", 1.0e5, 6, 16), ("free_space", 1.0e4, 5, 16), ("pbc", 1.0e5, 6, 15)])
+def test_blocked_sweep_touches_only_reached_orders(bc, E, r, L):
     from coulomb_kmc_amd import KMCFMM
     rng = np.random.default_rng(71)
-    L, N = 16, 4000
+    N = 4000
     ncps = 2 ** (r - 1)
     pos = rng.uniform(-0.5 * E, 0.5 * E, (N, 3))
     q = np.ones(N)
